@@ -1,0 +1,144 @@
+/*
+ * scirs_gpu.h — C ABI of the MI355X (gfx950) dense linear-solve engine.
+ *
+ * This is the whole boundary between a host binding (the scirs2-linalg Rust
+ * shim, a ctypes test harness, ...) and the HIP kernels.  Plain pointers and
+ * sizes only.  Matrices are dense and ROW-MAJOR (an ndarray in standard
+ * order) with an explicit leading dimension in elements.
+ *
+ * Return value of every call (also copied to info->status):
+ *      0   success
+ *    k>0   the reference's breakdown at 1-based column / row k:
+ *          LU:       max |pivot candidate| < eps       (scirs2-linalg/src/lapack.rs:106-120)
+ *          Cholesky: a_ii - sum_{k<i} l_ik^2 <= 0       (scirs2-linalg/src/lapack.rs:661-670)
+ *    k<0   SCIRS_ERR_* (message in scirs_gpu_last_error())
+ *
+ * Host-memory entry points copy their inputs to HBM, compute, and copy the
+ * results back; they keep no pointer after returning and serialise on one
+ * internal mutex (thread safe).  The *_dev entry points take HBM pointers and
+ * a hipStream_t (NULL = the library's stream) for data that is already
+ * resident; they return after the stream has drained.
+ */
+#ifndef SCIRS_GPU_H
+#define SCIRS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCIRS_OK 0
+#define SCIRS_ERR_ARG (-1)
+#define SCIRS_ERR_HIP (-2)
+#define SCIRS_ERR_NOMEM (-3)
+#define SCIRS_ERR_TIMEOUT (-4)
+#define SCIRS_ERR_NOT_CONVERGED (-5)
+#define SCIRS_ERR_UNSUPPORTED (-6)
+
+/* solve paths reported in scirs_solve_info.path */
+#define SCIRS_PATH_FP64_LU 0        /* plain FP64 LU + substitution            */
+#define SCIRS_PATH_FP64_CHOL 1      /* plain FP64 Cholesky + substitution      */
+#define SCIRS_PATH_MIXED_IR 2       /* low-precision LU + FP64 refinement       */
+#define SCIRS_PATH_MIXED_GMRES 3    /* low-precision LU + FP64 GMRES-IR         */
+#define SCIRS_PATH_FP64_FALLBACK 4  /* mixed path gave up; FP64 LU was used     */
+#define SCIRS_PATH_FP32_LU 5
+#define SCIRS_PATH_FP32_CHOL 6
+
+/* precision modes of the mixed-precision solve (trailing-update arithmetic) */
+#define SCIRS_MODE_AUTO 0
+#define SCIRS_MODE_BF16 1           /* BF16 x BF16 -> FP32 matrix cores         */
+#define SCIRS_MODE_FP16 2           /* FP16 with power-of-two scaling -> FP32   */
+#define SCIRS_MODE_BF16X3 3         /* split BF16 (hi+lo), three products       */
+#define SCIRS_MODE_FP32 4           /* FP32 matrix cores                        */
+
+typedef struct scirs_solve_info {
+    int32_t status;          /* same as the return value                          */
+    int32_t path;            /* SCIRS_PATH_*                                      */
+    int32_t ir_iters;        /* refinement steps (mixed path)                     */
+    int32_t gmres_iters;     /* total GMRES inner iterations (mixed path)         */
+    double pivot_value;      /* |pivot| or a_ii - sum at the breakdown (status>0) */
+    double backward_error;   /* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf); <0 = not computed */
+    double time_factor_ms;   /* device time of the factorisation                  */
+    double time_solve_ms;    /* device time of substitution / refinement          */
+    double time_total_ms;    /* wall time of the call                             */
+} scirs_solve_info;
+
+/* ---- library ------------------------------------------------------------ */
+/* Binds the library to a HIP device (default 0); called implicitly by every
+ * entry point on first use. */
+int scirs_gpu_init(int device);
+/* Message of the last error on any thread ("" when none). */
+const char *scirs_gpu_last_error(void);
+
+/* ---- LU ------------------------------------------------------------------ */
+/* Replaces scirs2-linalg/src/lapack.rs:72-147 `lu_factor` (called by
+ * decomposition.rs:137 `lu`).  In place: a (m x n) receives the packed factors
+ * (unit-lower L below the diagonal, U on and above); perm (m entries) receives
+ * the reference's `piv` vector: row i of P*A is row perm[i] of A. */
+int scirs_gpu_dgetrf(int m, int n, double *a, int64_t lda, int64_t *perm, scirs_solve_info *info);
+int scirs_gpu_sgetrf(int m, int n, float *a, int64_t lda, int64_t *perm, scirs_solve_info *info);
+
+/* Replaces the body of scirs2-linalg/src/solve.rs:54-111 `solve` (n > 4) and
+ * solve.rs:366-424 `solve_multiple`: X = A^-1 B for B n x nrhs. */
+int scirs_gpu_dgesv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
+                    double *x, int64_t ldx, scirs_solve_info *info);
+int scirs_gpu_sgesv(int n, int nrhs, const float *a, int64_t lda, const float *b, int64_t ldb,
+                    float *x, int64_t ldx, scirs_solve_info *info);
+
+/* ---- Cholesky -------------------------------------------------------------- */
+/* Replaces scirs2-linalg/src/lapack.rs:632-679 `cholesky` (called by
+ * decomposition.rs:44 `cholesky`).  Reads only the lower triangle of a (n x n)
+ * and overwrites a with L (zeros above the diagonal). */
+int scirs_gpu_dpotrf(int n, double *a, int64_t lda, scirs_solve_info *info);
+int scirs_gpu_spotrf(int n, float *a, int64_t lda, scirs_solve_info *info);
+
+/* The Cholesky-solve composition `cholesky` -> solve_triangular(L) ->
+ * solve_triangular(L^T) (decomposition.rs:44, solve.rs:139): X = A^-1 B. */
+int scirs_gpu_dposv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
+                    double *x, int64_t ldx, scirs_solve_info *info);
+
+/* ---- triangular ------------------------------------------------------------ */
+/* Replaces scirs2-linalg/src/solve.rs:139-202 `solve_triangular` for nrhs
+ * right-hand sides.  Returns i+1 when a non-unit diagonal has |a_ii| < eps. */
+int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
+                    double *x, int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info);
+
+/* ---- mixed precision ---------------------------------------------------------- */
+/* The factor-once counterpart of scirs2-linalg/src/mixed_precision/adaptive.rs:282-378
+ * `iterative_refinement_solve::<f64,f64,f64,f64,f32>`: FP32 panel, low-precision
+ * trailing update (mode), FP64 residual on the GPU, refinement until the
+ * normwise backward error is <= tol (tol <= 0: n * 2^-53), GMRES-IR
+ * (restarted, preconditioned by the low-precision factors) when refinement
+ * stalls, and FP64 LU when both fail.  max_iter / max_gmres <= 0 take the
+ * defaults (30 refinement steps, 300 GMRES iterations). */
+int scirs_gpu_dgesv_ir(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
+                       double *x, int64_t ldx, int mode, int max_iter, int max_gmres, double tol,
+                       scirs_solve_info *info);
+
+/* ---- device-resident variants ---------------------------------------------- */
+/* dA is not modified; dLU (n x n, ld ldlu) receives the factors and dPerm
+ * (n) the permutation; dX receives the solution.  dLU / dPerm may be NULL
+ * (library scratch is used). */
+int scirs_gpu_dgesv_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                        double *dX, int64_t ldx, double *dLU, int64_t ldlu, int64_t *dPerm,
+                        void *stream, scirs_solve_info *info);
+int scirs_gpu_dposv_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                        double *dX, int64_t ldx, double *dL, int64_t ldl, void *stream,
+                        scirs_solve_info *info);
+int scirs_gpu_dgesv_ir_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                           double *dX, int64_t ldx, int mode, int max_iter, int max_gmres, double tol,
+                           void *stream, scirs_solve_info *info);
+
+/* ---- diagnostics -------------------------------------------------------------- */
+/* Normwise backward error of X for A X = B, all in HBM (FP64 residual on the
+ * GPU).  No reference counterpart: the reference's tests compute residuals
+ * with ndarray `dot` (tests/scipy_compat_validation.rs:455-476). */
+int scirs_gpu_backward_error_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
+                                 const double *dB, int64_t ldb, void *stream, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCIRS_GPU_H */

@@ -1,0 +1,200 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of ``oracle/_build/liboracle.so`` (built from
+``oracle/ref_linalg.c`` by ``oracle/Makefile``), the plain-C restatement of
+scirs2-linalg's CPU solve path:
+
+* ``lu_factor``        -> scirs2-linalg/src/lapack.rs:72-147
+* ``cholesky``         -> scirs2-linalg/src/lapack.rs:632-679
+* ``solve_triangular`` -> scirs2-linalg/src/solve.rs:139-202
+* ``solve``            -> scirs2-linalg/src/solve.rs:54-111 (+ basic.rs:117-186)
+* ``solve_multiple``   -> scirs2-linalg/src/solve.rs:366-424
+* ``lu``               -> scirs2-linalg/src/decomposition.rs:157-180 (P, L, U build)
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module; it is the checker, never the product.
+Pinning of each function is listed in ``tests/test_oracle.py``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+class OracleError(RuntimeError):
+    """Raised when the oracle reports a singular / non-PD / empty input."""
+
+    def __init__(self, kind: str, code: int):
+        super().__init__(f"{kind} (code {code})")
+        self.kind = kind
+        self.code = code
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    L = ctypes.c_long
+    for s in ("d", "s"):
+        getattr(lib, f"oracle_lu_factor_{s}").argtypes = [I, I, P, L, P, P, P, P]
+        getattr(lib, f"oracle_cholesky_{s}").argtypes = [I, P, L, P]
+        getattr(lib, f"oracle_solve_triangular_{s}").argtypes = [I, P, L, P, P, I, I]
+        getattr(lib, f"oracle_solve_{s}").argtypes = [I, P, L, P, P]
+        getattr(lib, f"oracle_solve_multiple_{s}").argtypes = [I, I, P, L, P, L, P, L]
+    _lib = lib
+    return lib
+
+
+def _sfx(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return "d"
+    if dt == np.float32:
+        return "s"
+    raise TypeError(f"oracle supports float32/float64, got {dt}")
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def lu_factor(a):
+    """Packed LU, permutation vector and sign, exactly as lapack.rs:72-147.
+
+    Returns (lu, piv, sign).  Raises OracleError('singular', k) with the
+    1-based column k where the reference's |pivot| < eps test fires.
+    """
+    a = np.asarray(a)
+    dt = a.dtype if a.dtype in (np.float32, np.float64) else np.float64
+    a = _c(a, dt)
+    n, m = a.shape
+    lu = np.empty_like(a)
+    piv = np.empty(n, dtype=np.int64)
+    sign = np.zeros(1, dtype=dt)
+    cond = np.zeros(1, dtype=np.float64)
+    rc = getattr(_load(), f"oracle_lu_factor_{_sfx(dt)}")(
+        n, m, _ptr(a), m, _ptr(lu), _ptr(piv), _ptr(sign), _ptr(cond))
+    if rc > 0:
+        err = OracleError("singular", rc)
+        err.cond_estimate = None if cond[0] < 0 else float(cond[0])
+        raise err
+    if rc < 0:
+        raise OracleError("empty", rc)
+    return lu, piv, dt.type(sign[0]) if hasattr(dt, "type") else sign[0]
+
+
+def lu(a):
+    """(P, L, U) built from lu_factor exactly as decomposition.rs:157-180."""
+    a = np.asarray(a)
+    packed, piv, _ = lu_factor(a)
+    n, m = packed.shape
+    k = min(n, m)
+    p = np.zeros((n, n), dtype=packed.dtype)
+    p[np.arange(n), piv] = 1
+    l = np.zeros((n, k), dtype=packed.dtype)
+    for i in range(n):
+        l[i, : min(i, m)] = packed[i, : min(i, m)]
+        if i < m:
+            l[i, i] = 1
+    u = np.zeros((k, m), dtype=packed.dtype)
+    for i in range(k):
+        u[i, i:] = packed[i, i:]
+    return p, l, u
+
+
+def cholesky(a):
+    """Lower Cholesky factor (lapack.rs:632-679); reads only the lower triangle."""
+    a = np.asarray(a)
+    dt = a.dtype if a.dtype in (np.float32, np.float64) else np.float64
+    a = _c(a, dt)
+    n = a.shape[0]
+    l = np.empty_like(a)
+    rc = getattr(_load(), f"oracle_cholesky_{_sfx(dt)}")(n, _ptr(a), n, _ptr(l))
+    if rc > 0:
+        raise OracleError("not_positive_definite", rc)
+    if rc < 0:
+        raise OracleError("empty", rc)
+    return l
+
+
+def solve_triangular(a, b, lower: bool, unit_diagonal: bool):
+    a = np.asarray(a)
+    dt = a.dtype if a.dtype in (np.float32, np.float64) else np.float64
+    a = _c(a, dt)
+    b = _c(b, dt)
+    n = a.shape[0]
+    x = np.empty(n, dtype=dt)
+    rc = getattr(_load(), f"oracle_solve_triangular_{_sfx(dt)}")(
+        n, _ptr(a), n, _ptr(b), _ptr(x), int(lower), int(unit_diagonal))
+    if rc > 0:
+        raise OracleError("singular", rc)
+    return x
+
+
+def solve(a, b):
+    a = np.asarray(a)
+    dt = a.dtype if a.dtype in (np.float32, np.float64) else np.float64
+    a = _c(a, dt)
+    b = _c(b, dt)
+    n = a.shape[0]
+    x = np.empty(n, dtype=dt)
+    rc = getattr(_load(), f"oracle_solve_{_sfx(dt)}")(n, _ptr(a), n, _ptr(b), _ptr(x))
+    if rc > 0:
+        raise OracleError("singular", rc)
+    if rc < 0:
+        raise OracleError("empty", rc)
+    return x
+
+
+def solve_multiple(a, b):
+    a = np.asarray(a)
+    dt = a.dtype if a.dtype in (np.float32, np.float64) else np.float64
+    a = _c(a, dt)
+    b = _c(b, dt)
+    n = a.shape[0]
+    nrhs = b.shape[1]
+    x = np.empty((n, nrhs), dtype=dt)
+    rc = getattr(_load(), f"oracle_solve_multiple_{_sfx(dt)}")(
+        n, nrhs, _ptr(a), n, _ptr(b), nrhs, _ptr(x), nrhs)
+    if rc > 0:
+        raise OracleError("singular", rc)
+    if rc < 0:
+        raise OracleError("empty", rc)
+    return x
+
+
+def cholesky_solve(a, b):
+    """Composition used as the Cholesky-solve oracle: cholesky (lapack.rs:632)
+    then solve_triangular(L, b, lower) and solve_triangular(L^T, y, upper)
+    (solve.rs:139-202), column by column for a 2-D b."""
+    l = cholesky(a)
+    b = np.asarray(b, dtype=l.dtype)
+    cols = b.reshape(b.shape[0], -1)
+    out = np.empty_like(cols)
+    lt = np.ascontiguousarray(l.T)
+    for j in range(cols.shape[1]):
+        y = solve_triangular(l, cols[:, j], True, False)
+        out[:, j] = solve_triangular(lt, y, False, False)
+    return out.reshape(b.shape)

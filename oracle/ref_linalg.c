@@ -1,0 +1,229 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A plain-C restatement of the CPU algorithms that scirs2-linalg uses on the
+ * dense-solve path.  Only the test suite, the smoke check and the CPU
+ * baseline leg of the benchmark may load this file's shared library; the
+ * product path (libscirs_gpu.so) never links or calls it.
+ *
+ * Each function follows the reference's operation order so that, compiled
+ * with -O2 -ffp-contract=off (Rust never fuses a*b+c), its results are the
+ * ones the Rust code produces bit for bit:
+ *
+ *   oracle_lu_factor_*        scirs2-linalg/src/lapack.rs:72-147
+ *   oracle_cholesky_*         scirs2-linalg/src/lapack.rs:632-679
+ *   oracle_solve_triangular_* scirs2-linalg/src/solve.rs:139-202
+ *   oracle_solve_*            scirs2-linalg/src/solve.rs:54-111
+ *                             (n <= 4 branch: basic.rs:117-186 inv, basic.rs:29-93 det)
+ *   oracle_solve_multiple_*   scirs2-linalg/src/solve.rs:366-424
+ *
+ * All matrices are dense row-major with an explicit leading dimension, the
+ * layout of an ndarray in standard order.
+ *
+ * Return codes: 0 = success; k > 0 = singular / not positive definite, with
+ * k the 1-based column (LU) or row (Cholesky) at which the reference raises
+ * its error; -1 = empty input; -2 = dimension error; -3 = memory.
+ *
+ * Pinning (which check fixes which function) is listed in the header of
+ * tests/test_oracle.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define ORACLE_SINGULAR_COND_NONE (-1.0)
+
+/* Instantiate every routine for one scalar type.  T = element type,
+ * S = name suffix, EPS = T's machine epsilon (num_traits Float::epsilon). */
+#define ORACLE_DEFINE(T, S, EPS, FABS, SQRT)                                        \
+                                                                                    \
+/* lapack.rs:72-147.  lu (m_rows x n_cols, row-major, ld = n_cols) receives the   \
+ * packed factors, piv the permutation vector (row i of P*A is row piv[i] of A),   \
+ * sign the permutation sign and cond_est the reference's 1/max estimate when the  \
+ * pivot is tiny but non-zero (else -1). */                                        \
+int oracle_lu_factor_##S(int n, int m, const T *a, long lda, T *lu, int64_t *piv,  \
+                         T *sign, double *cond_est)                                \
+{                                                                                   \
+    if (n == 0 || m == 0) return -1; /* lapack.rs:79-83 */                          \
+    for (int i = 0; i < n; ++i)                                                     \
+        memcpy(lu + (size_t)i * m, a + (size_t)i * lda, sizeof(T) * (size_t)m);     \
+    for (int i = 0; i < n; ++i) piv[i] = i;                                         \
+    T sg = (T)1;                                                                    \
+    if (cond_est) *cond_est = ORACLE_SINGULAR_COND_NONE;                            \
+    int kmax = n < m ? n : m;                                                       \
+    for (int k = 0; k < kmax; ++k) {                                                \
+        /* pivot search, strict '>' so the first maximum wins (lapack.rs:95-104) */ \
+        int p = k;                                                                  \
+        T max_val = FABS(lu[(size_t)k * m + k]);                                    \
+        for (int i = k + 1; i < n; ++i) {                                           \
+            T av = FABS(lu[(size_t)i * m + k]);                                     \
+            if (av > max_val) { max_val = av; p = i; }                              \
+        }                                                                           \
+        /* absolute singularity test (lapack.rs:106-120) */                         \
+        if (max_val < EPS) {                                                        \
+            if (cond_est && max_val > (T)0) *cond_est = (double)((T)1 / max_val);   \
+            if (sign) *sign = sg;                                                   \
+            return k + 1;                                                           \
+        }                                                                           \
+        /* full-row swap (lapack.rs:123-134) */                                     \
+        if (p != k) {                                                               \
+            for (int j = 0; j < m; ++j) {                                           \
+                T t = lu[(size_t)k * m + j];                                        \
+                lu[(size_t)k * m + j] = lu[(size_t)p * m + j];                      \
+                lu[(size_t)p * m + j] = t;                                          \
+            }                                                                       \
+            int64_t tp = piv[k]; piv[k] = piv[p]; piv[p] = tp;                      \
+            sg = -sg;                                                               \
+        }                                                                           \
+        /* multipliers by true division, then the row update (lapack.rs:137-143) */ \
+        for (int i = k + 1; i < n; ++i) {                                           \
+            T *ri = lu + (size_t)i * m;                                             \
+            const T *rk = lu + (size_t)k * m;                                       \
+            ri[k] = ri[k] / rk[k];                                                  \
+            T l = ri[k];                                                            \
+            for (int j = k + 1; j < m; ++j) ri[j] = ri[j] - l * rk[j];              \
+        }                                                                           \
+    }                                                                               \
+    if (sign) *sign = sg;                                                           \
+    return 0;                                                                       \
+}                                                                                   \
+                                                                                    \
+/* lapack.rs:632-679, Cholesky-Banachiewicz.  Reads only the lower triangle of a;  \
+ * l (n x n, ld = n) gets the factor with zeros above the diagonal.  Returns i+1   \
+ * when a_ii - sum <= 0 at row i. */                                               \
+int oracle_cholesky_##S(int n, const T *a, long lda, T *l)                          \
+{                                                                                   \
+    if (n == 0) return -1;                                                          \
+    memset(l, 0, sizeof(T) * (size_t)n * (size_t)n);                                \
+    for (int i = 0; i < n; ++i) {                                                   \
+        for (int j = 0; j <= i; ++j) {                                              \
+            T sum = (T)0;                                                           \
+            for (int k = 0; k < j; ++k)                                             \
+                sum += l[(size_t)i * n + k] * l[(size_t)j * n + k];                 \
+            if (i == j) {                                                           \
+                T val = a[(size_t)i * lda + i] - sum;                               \
+                if (val <= (T)0) return i + 1;                                      \
+                l[(size_t)i * n + j] = SQRT(val);                                   \
+            } else {                                                                \
+                l[(size_t)i * n + j] = (a[(size_t)i * lda + j] - sum) /             \
+                                       l[(size_t)j * n + j];                        \
+            }                                                                       \
+        }                                                                           \
+    }                                                                               \
+    return 0;                                                                       \
+}                                                                                   \
+                                                                                    \
+/* solve.rs:139-202 (the numeric part; the finiteness checks live in the host     \
+ * layer).  Returns i+1 when |a_ii| < eps on a non-unit diagonal. */               \
+int oracle_solve_triangular_##S(int n, const T *a, long lda, const T *b, T *x,      \
+                                int lower, int unit_diagonal)                       \
+{                                                                                   \
+    if (n == 0) return -1;                                                          \
+    if (lower) {                                                                    \
+        for (int i = 0; i < n; ++i) {                                               \
+            T sum = b[i];                                                           \
+            for (int j = 0; j < i; ++j) sum -= a[(size_t)i * lda + j] * x[j];       \
+            if (unit_diagonal) {                                                    \
+                x[i] = sum;                                                         \
+            } else {                                                                \
+                T d = a[(size_t)i * lda + i];                                       \
+                if (FABS(d) < EPS) return i + 1;                                    \
+                x[i] = sum / d;                                                     \
+            }                                                                       \
+        }                                                                           \
+    } else {                                                                        \
+        for (int i = n - 1; i >= 0; --i) {                                          \
+            T sum = b[i];                                                           \
+            for (int j = i + 1; j < n; ++j) sum -= a[(size_t)i * lda + j] * x[j];   \
+            if (unit_diagonal) {                                                    \
+                x[i] = sum;                                                         \
+            } else {                                                                \
+                T d = a[(size_t)i * lda + i];                                       \
+                if (FABS(d) < EPS) return i + 1;                                    \
+                x[i] = sum / d;                                                     \
+            }                                                                       \
+        }                                                                           \
+    }                                                                               \
+    return 0;                                                                       \
+}                                                                                   \
+                                                                                    \
+/* LU-based solve of one column, the body of solve.rs:84-110 and of the per-column \
+ * loop solve.rs:399-421.  lu/piv come from oracle_lu_factor.  P*b is exactly     \
+ * b[piv[i]] (a sum of one b entry and zeros, solve.rs:96-102). */                 \
+static int oracle_lu_apply_##S(int n, const T *lu, const int64_t *piv,             \
+                               const T *b, long bstride, T *x, long xstride,        \
+                               T *work)                                             \
+{                                                                                   \
+    T *pb = work, *y = work + n, *xx = work + 2 * n;                                \
+    for (int i = 0; i < n; ++i) pb[i] = b[(size_t)piv[i] * bstride];                \
+    /* L is the strict lower part with a unit diagonal (decomposition.rs:165-173) */\
+    oracle_solve_triangular_##S(n, lu, n, pb, y, 1, 1);                             \
+    int rc = oracle_solve_triangular_##S(n, lu, n, y, xx, 0, 0);                    \
+    if (rc) return rc;                                                              \
+    for (int i = 0; i < n; ++i) x[(size_t)i * xstride] = xx[i];                     \
+    return 0;                                                                       \
+}                                                                                   \
+                                                                                    \
+/* solve.rs:366-424.  b is n x nrhs (ld ldb), x is n x nrhs (ld ldx). */            \
+int oracle_solve_multiple_##S(int n, int nrhs, const T *a, long lda, const T *b,    \
+                              long ldb, T *x, long ldx)                             \
+{                                                                                   \
+    if (n == 0 || nrhs == 0) return -1;                                             \
+    T *lu = (T *)malloc(sizeof(T) * (size_t)n * (size_t)n);                         \
+    int64_t *piv = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);                  \
+    T *work = (T *)malloc(sizeof(T) * 3 * (size_t)n);                               \
+    if (!lu || !piv || !work) { free(lu); free(piv); free(work); return -3; }       \
+    int rc = oracle_lu_factor_##S(n, n, a, lda, lu, piv, NULL, NULL);               \
+    for (int j = 0; rc == 0 && j < nrhs; ++j)                                       \
+        rc = oracle_lu_apply_##S(n, lu, piv, b + j, ldb, x + j, ldx, work);         \
+    free(lu); free(piv); free(work);                                                \
+    return rc;                                                                      \
+}                                                                                   \
+                                                                                    \
+/* basic.rs:117-186 for n <= 4: the 2x2 closed form with its |det| < eps test     \
+ * (det from basic.rs:47), otherwise solve_multiple(A, I). */                      \
+static int oracle_inv_small_##S(int n, const T *a, long lda, T *ainv)               \
+{                                                                                   \
+    if (n == 2) {                                                                   \
+        T a00 = a[0], a01 = a[1], a10 = a[lda], a11 = a[lda + 1];                   \
+        T det_val = a00 * a11 - a01 * a10;                                          \
+        if (FABS(det_val) < EPS) return 1;                                          \
+        T inv_det = (T)1 / det_val;                                                 \
+        ainv[0] = a11 * inv_det;                                                    \
+        ainv[1] = -a01 * inv_det;                                                   \
+        ainv[2] = -a10 * inv_det;                                                   \
+        ainv[3] = a00 * inv_det;                                                    \
+        return 0;                                                                   \
+    }                                                                               \
+    T eye[16];                                                                      \
+    memset(eye, 0, sizeof(eye));                                                    \
+    for (int i = 0; i < n; ++i) eye[i * n + i] = (T)1;                              \
+    return oracle_solve_multiple_##S(n, n, a, lda, eye, n, ainv, n);                \
+}                                                                                   \
+                                                                                    \
+/* solve.rs:54-111.  For n <= 4 the reference forms inv(A) and multiplies         \
+ * (solve.rs:65-76); above that it factors and substitutes. */                     \
+int oracle_solve_##S(int n, const T *a, long lda, const T *b, T *x)                \
+{                                                                                   \
+    if (n == 0) return -1;                                                          \
+    if (n <= 4) {                                                                   \
+        T ainv[16];                                                                 \
+        int rc = oracle_inv_small_##S(n, a, lda, ainv);                             \
+        if (rc) return rc;                                                          \
+        for (int i = 0; i < n; ++i) {                                               \
+            T acc = (T)0;                                                           \
+            for (int j = 0; j < n; ++j) acc += ainv[i * n + j] * b[j];              \
+            x[i] = acc;                                                             \
+        }                                                                           \
+        return 0;                                                                   \
+    }                                                                               \
+    return oracle_solve_multiple_##S(n, 1, a, lda, b, 1, x, 1);                     \
+}
+
+ORACLE_DEFINE(double, d, DBL_EPSILON, fabs, sqrt)
+ORACLE_DEFINE(float, s, FLT_EPSILON, fabsf, sqrtf)
+
+/* Version tag so a test can tell that the built library matches this source. */
+int oracle_version(void) { return 1; }

@@ -1,0 +1,599 @@
+// extern "C" entry points of include/scirs_gpu.h: one process-wide context per
+// device (stream, scratch, workspace), host<->HBM staging for the host-memory
+// entry points, and the plain FP64 / FP32 factor-and-solve drivers.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "api_internal.hpp"
+#include "../../include/scirs_gpu.h"
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_err_mu;
+std::string g_last_error_any;
+} // namespace
+
+extern "C" void scirs_set_error(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_error_any = buf;
+    if (getenv("SCIRS_GPU_LOG")) fprintf(stderr, "[scirs_gpu] %s\n", buf);
+}
+
+int scirs_allow_lds(const void *kernel, size_t dynamic_bytes)
+{
+    static std::mutex mu;
+    static std::vector<std::pair<const void *, size_t>> granted;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &g : granted)
+        if (g.first == kernel && g.second >= dynamic_bytes) return SCIRS_OK;
+    hipFuncAttributes attr;
+    SCIRS_HIP_CHECK(hipFuncGetAttributes(&attr, kernel));
+    const size_t limit = 160 * 1024 - attr.sharedSizeBytes;
+    if (dynamic_bytes > limit) {
+        scirs_set_error("kernel needs %zu B of dynamic LDS; only %zu B are available", dynamic_bytes, limit);
+        return SCIRS_ERR_HIP;
+    }
+    SCIRS_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit));
+    for (auto &g : granted)
+        if (g.first == kernel) { g.second = limit; return SCIRS_OK; }
+    granted.emplace_back(kernel, limit);
+    return SCIRS_OK;
+}
+
+extern "C" const char *scirs_gpu_last_error(void)
+{
+    if (!g_last_error.empty()) return g_last_error.c_str();
+    return "";
+}
+
+namespace scirs {
+
+static Context g_ctx;
+
+Context &ctx() { return g_ctx; }
+
+static int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+int Context::ensure(void **p, size_t *have, size_t bytes)
+{
+    if (*have >= bytes && *p) return SCIRS_OK;
+    if (*p) {
+        SCIRS_HIP_CHECK(hipStreamSynchronize(stream));
+        SCIRS_HIP_CHECK(hipFree(*p));
+        *p = nullptr;
+        *have = 0;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        scirs_set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return SCIRS_ERR_NOMEM;
+    }
+    *have = bytes;
+    return SCIRS_OK;
+}
+
+int Context::init(int dev)
+{
+    if (initialised) return SCIRS_OK;
+    int count = 0;
+    SCIRS_HIP_CHECK(hipGetDeviceCount(&count));
+    if (count <= 0) {
+        scirs_set_error("no HIP device visible");
+        return SCIRS_ERR_HIP;
+    }
+    if (dev < 0 || dev >= count) {
+        scirs_set_error("device %d out of range (%d devices)", dev, count);
+        return SCIRS_ERR_ARG;
+    }
+    device = dev;
+    SCIRS_HIP_CHECK(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    SCIRS_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    cu_count = prop.multiProcessorCount;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        scirs_set_error("device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
+        return SCIRS_ERR_UNSUPPORTED;
+    }
+    SCIRS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // factorisation workspace
+    const int max_g = SCIRS_MAX_G;
+    const int kbmax = SCIRS_KB_MAX;
+    work.max_g = std::min(env_int("SCIRS_GPU_MAX_G", cu_count), max_g);
+    work.max_panels = 1 << 14;   // n up to 16384 * kb
+    work.spin_limit = (unsigned)env_int("SCIRS_GPU_SPIN_LIMIT", 1 << 22);
+    work.lowp_mode = SCIRS_LOWP_FP32;   // plain FP32 entry points compute in FP32; the mixed solve switches it
+    work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 64);
+    size_t bytes = 0;
+    bytes += 2 * (size_t)max_g * sizeof(double);          // cval
+    bytes += 2 * (size_t)max_g * sizeof(int);             // cidx
+    bytes += 2 * (size_t)max_g * kbmax * sizeof(double);  // crow
+    bytes += 2 * (size_t)kbmax * sizeof(double);          // rowj
+    bytes += 64 * sizeof(int);                            // bar, done, abort, info, info_val (+pad)
+    bytes += (size_t)work.max_panels * 4 * kbmax * sizeof(int);
+    bytes += (size_t)work.max_panels * sizeof(int);
+    bytes = (bytes + 255) & ~(size_t)255;
+    char *base = nullptr;
+    SCIRS_HIP_CHECK(hipMalloc((void **)&base, bytes));
+    SCIRS_HIP_CHECK(hipMemset(base, 0, bytes));
+    char *p = base;
+    auto take = [&](size_t n) { char *r = p; p += (n + 255) & ~(size_t)255; return r; };
+    work.cval = take(2 * (size_t)max_g * sizeof(double));
+    work.cidx = (int *)take(2 * (size_t)max_g * sizeof(int));
+    work.crow = take(2 * (size_t)max_g * kbmax * sizeof(double));
+    work.rowj = take(2 * (size_t)kbmax * sizeof(double));
+    char *flags = take(64 * sizeof(int));
+    work.bar = (unsigned *)flags;
+    work.done = (unsigned *)(flags + 4);
+    work.abort = (int *)(flags + 8);
+    work.info = (int *)(flags + 12);
+    work.info_val = (double *)(flags + 16);
+    work.pairs = (int *)take((size_t)work.max_panels * 4 * kbmax * sizeof(int));
+    work.npairs = (int *)take((size_t)work.max_panels * sizeof(int));
+    work_base = base;
+    SCIRS_HIP_CHECK(hipHostMalloc((void **)&host_flags, 64, hipHostMallocDefault));
+    kb_f64 = env_int("SCIRS_GPU_KB", 128);
+    kb_f32 = env_int("SCIRS_GPU_KB_F32", 128);
+    kb_chol = env_int("SCIRS_GPU_KB_CHOL", 128);
+    initialised = true;
+    return SCIRS_OK;
+}
+
+// Reset info / abort before a factorisation (a kernel, so it is safe under capture).
+int Context::reset_flags(hipStream_t s)
+{
+    return scirs_fill_int(work.abort, 2, 0, s);   // abort and info are adjacent
+}
+
+// Read info, info_val and abort after the stream drains.
+int Context::read_flags(hipStream_t s, int *info, double *info_val)
+{
+    SCIRS_HIP_CHECK(hipMemcpyAsync(host_flags, work.abort, 8 + 8, hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    int abort_flag = ((int *)host_flags)[0];
+    *info = ((int *)host_flags)[1];
+    double v;
+    memcpy(&v, (char *)host_flags + 8, 8);
+    if (info_val) *info_val = v;
+    if (abort_flag) {
+        // clear barrier state for the next call
+        (void)hipMemsetAsync(work.bar, 0, 12, s);
+        (void)hipStreamSynchronize(s);
+        scirs_set_error("a cooperative panel kernel timed out at its grid barrier");
+        return SCIRS_ERR_TIMEOUT;
+    }
+    return SCIRS_OK;
+}
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+void info_init(scirs_solve_info *info, int path)
+{
+    if (!info) return;
+    memset(info, 0, sizeof(*info));
+    info->path = path;
+    info->backward_error = -1.0;
+}
+
+int finish(scirs_solve_info *info, int rc, double t0)
+{
+    if (info) {
+        info->status = rc;
+        info->time_total_ms = now_ms() - t0;
+    }
+    return rc;
+}
+
+// normwise backward error  max_j ||R_j||_inf / (||A||_inf ||X_j||_inf + ||B_j||_inf)
+int backward_error(int n, int nrhs, const double *dA, long lda, const double *dX, long ldx, const double *dB,
+                   long ldb, hipStream_t s, double *out)
+{
+    Context &c = ctx();
+    int rc;
+    size_t need = ((size_t)n * nrhs + (size_t)n + 3 * 64 + 8) * sizeof(double);
+    if ((rc = c.ensure(&c.scratch_bwd, &c.scratch_bwd_bytes, need))) return rc;
+    double *R = (double *)c.scratch_bwd;
+    double *rowsum = R + (size_t)n * nrhs;
+    double *norms = rowsum + n;   // [0..63] r, [64..127] x, [128..191] b, [192] A
+    if ((rc = scirs_residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, nrhs, s))) return rc;
+    if ((rc = scirs_convert_rowsum(n, n, dA, lda, nullptr, 0, rowsum, s))) return rc;
+    if ((rc = scirs_rowsum_max(n, rowsum, norms + 192, s))) return rc;
+    double worst = 0.0;
+    for (int j0 = 0; j0 < nrhs; j0 += 64) {
+        int nj = std::min(64, nrhs - j0);
+        if ((rc = scirs_colmax_d(n, nj, R + j0, nrhs, norms, s))) return rc;
+        if ((rc = scirs_colmax_d(n, nj, dX + j0, ldx, norms + 64, s))) return rc;
+        if ((rc = scirs_colmax_d(n, nj, dB + j0, ldb, norms + 128, s))) return rc;
+        double h[193];
+        SCIRS_HIP_CHECK(hipMemcpyAsync(h, norms, 193 * sizeof(double), hipMemcpyDeviceToHost, s));
+        SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+        for (int j = 0; j < nj; ++j) {
+            double den = h[192] * h[64 + j] + h[128 + j];
+            double e = den > 0 ? h[j] / den : (h[j] > 0 ? INFINITY : 0.0);
+            worst = std::max(worst, e);
+        }
+    }
+    *out = worst;
+    return SCIRS_OK;
+}
+
+} // namespace scirs
+
+using namespace scirs;
+
+#define SCIRS_ENTER(dev_needed)                                   \
+    std::lock_guard<std::recursive_mutex> _lk(ctx().mu);          \
+    {                                                             \
+        int _rc = ctx().init(env_int_dev());                      \
+        if (_rc) return _rc;                                      \
+        SCIRS_HIP_CHECK(hipSetDevice(ctx().device));              \
+    }
+
+static int env_int_dev()
+{
+    const char *v = getenv("SCIRS_GPU_DEVICE");
+    return v && *v ? atoi(v) : 0;
+}
+
+extern "C" int scirs_gpu_init(int device)
+{
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    return ctx().init(device < 0 ? 0 : device);
+}
+
+// ------------------------------------------------------------------ LU core
+template <typename T>
+static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm, const T *dB, long ldb, T *dX,
+                              long ldx, hipStream_t s, scirs_solve_info *info)
+{
+    Context &c = ctx();
+    int rc;
+    hipEvent_t e0, e1, e2;
+    SCIRS_HIP_CHECK(hipEventCreate(&e0));
+    SCIRS_HIP_CHECK(hipEventCreate(&e1));
+    SCIRS_HIP_CHECK(hipEventCreate(&e2));
+    SCIRS_HIP_CHECK(hipEventRecord(e0, s));
+    if ((rc = c.reset_flags(s))) return rc;
+    const int kb = sizeof(T) == 8 ? c.kb_f64 : c.kb_f32;
+    if ((rc = scirs_getrf_device<T>(n, n, dLU, ldlu, kb, c.work, s))) return rc;
+    if ((rc = scirs_perm_from_pairs(n, n, kb, c.work, dPerm, s))) return rc;
+    SCIRS_HIP_CHECK(hipEventRecord(e1, s));
+    if (nrhs > 0) {
+        if ((rc = scirs_permute_rows<T>(n, nrhs, dPerm, dB, ldb, dX, ldx, s))) return rc;
+        if ((rc = scirs_getrs_device<T>(n, nrhs, dLU, ldlu, dX, ldx, c.work.info, s))) return rc;
+    }
+    SCIRS_HIP_CHECK(hipEventRecord(e2, s));
+    int st = 0;
+    double pv = 0;
+    if ((rc = c.read_flags(s, &st, &pv))) return rc;
+    if (info) {
+        float ms1 = 0, ms2 = 0;
+        (void)hipEventElapsedTime(&ms1, e0, e1);
+        (void)hipEventElapsedTime(&ms2, e1, e2);
+        info->time_factor_ms = ms1;
+        info->time_solve_ms = ms2;
+        info->pivot_value = st ? pv : 0.0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return st;   // 0 or the 1-based singular column
+}
+
+template <typename T>
+static int host_getrf(int m, int n, T *a, int64_t lda, int64_t *perm, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, sizeof(T) == 8 ? SCIRS_PATH_FP64_LU : SCIRS_PATH_FP32_LU);
+    if (m <= 0 || n <= 0 || !a || !perm || lda < n) {
+        scirs_set_error("getrf: invalid arguments (m=%d n=%d lda=%lld)", m, n, (long long)lda);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    SCIRS_ENTER(true);
+    Context &c = ctx();
+    hipStream_t s = c.stream;
+    int rc;
+    const size_t abytes = (size_t)m * n * sizeof(T);
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, abytes))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_perm, &c.scratch_perm_bytes, (size_t)m * sizeof(int64_t)))) return finish(info, rc, t0);
+    T *dA = (T *)c.scratch_a;
+    int64_t *dP = (int64_t *)c.scratch_perm;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), m, hipMemcpyHostToDevice, s));
+    if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
+    const int kb = sizeof(T) == 8 ? c.kb_f64 : c.kb_f32;
+    if ((rc = scirs_getrf_device<T>(m, n, dA, n, kb, c.work, s))) return finish(info, rc, t0);
+    if ((rc = scirs_perm_from_pairs(m, n, kb, c.work, dP, s))) return finish(info, rc, t0);
+    int st = 0;
+    double pv = 0;
+    if ((rc = c.read_flags(s, &st, &pv))) return finish(info, rc, t0);
+    if (info) info->pivot_value = st ? pv : 0.0;
+    if (st == 0) {
+        SCIRS_HIP_CHECK(hipMemcpy2DAsync(a, lda * sizeof(T), dA, n * sizeof(T), n * sizeof(T), m, hipMemcpyDeviceToHost, s));
+        SCIRS_HIP_CHECK(hipMemcpyAsync(perm, dP, (size_t)m * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return finish(info, st, t0);
+}
+
+extern "C" int scirs_gpu_dgetrf(int m, int n, double *a, int64_t lda, int64_t *perm, scirs_solve_info *info)
+{
+    return host_getrf<double>(m, n, a, lda, perm, info);
+}
+
+extern "C" int scirs_gpu_sgetrf(int m, int n, float *a, int64_t lda, int64_t *perm, scirs_solve_info *info)
+{
+    return host_getrf<float>(m, n, a, lda, perm, info);
+}
+
+template <typename T>
+static int host_gesv(int n, int nrhs, const T *a, int64_t lda, const T *b, int64_t ldb, T *x, int64_t ldx,
+                     scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, sizeof(T) == 8 ? SCIRS_PATH_FP64_LU : SCIRS_PATH_FP32_LU);
+    if (n <= 0 || nrhs <= 0 || !a || !b || !x || lda < n || ldb < nrhs || ldx < nrhs) {
+        scirs_set_error("gesv: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    SCIRS_ENTER(true);
+    Context &c = ctx();
+    hipStream_t s = c.stream;
+    int rc;
+    const size_t abytes = (size_t)n * n * sizeof(T), bbytes = (size_t)n * nrhs * sizeof(T);
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, abytes))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, 2 * bbytes))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_perm, &c.scratch_perm_bytes, (size_t)n * sizeof(int64_t)))) return finish(info, rc, t0);
+    T *dA = (T *)c.scratch_a, *dB = (T *)c.scratch_b, *dX = dB + (size_t)n * nrhs;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dB, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n, hipMemcpyHostToDevice, s));
+    int st = getrf_solve_device<T>(n, nrhs, dA, n, (int64_t *)c.scratch_perm, dB, nrhs, dX, nrhs, s, info);
+    if (st < 0) return finish(info, st, t0);
+    if (st == 0) {
+        SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n, hipMemcpyDeviceToHost, s));
+        SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return finish(info, st, t0);
+}
+
+extern "C" int scirs_gpu_dgesv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb, double *x,
+                               int64_t ldx, scirs_solve_info *info)
+{
+    return host_gesv<double>(n, nrhs, a, lda, b, ldb, x, ldx, info);
+}
+
+extern "C" int scirs_gpu_sgesv(int n, int nrhs, const float *a, int64_t lda, const float *b, int64_t ldb, float *x,
+                               int64_t ldx, scirs_solve_info *info)
+{
+    return host_gesv<float>(n, nrhs, a, lda, b, ldb, x, ldx, info);
+}
+
+extern "C" int scirs_gpu_dgesv_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                                   double *dX, int64_t ldx, double *dLU, int64_t ldlu, int64_t *dPerm, void *stream,
+                                   scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (n <= 0 || nrhs < 0 || !dA || lda < n || (nrhs > 0 && (!dB || !dX || ldb < nrhs || ldx < nrhs))) {
+        scirs_set_error("dgesv_dev: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    SCIRS_ENTER(true);
+    Context &c = ctx();
+    hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+    int rc;
+    if (!dLU) {
+        if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(double)))) return finish(info, rc, t0);
+        dLU = (double *)c.scratch_a;
+        ldlu = n;
+    }
+    if (!dPerm) {
+        if ((rc = c.ensure(&c.scratch_perm, &c.scratch_perm_bytes, (size_t)n * sizeof(int64_t)))) return finish(info, rc, t0);
+        dPerm = (int64_t *)c.scratch_perm;
+    }
+    // the reference factors a copy of A (lapack.rs:86); so does this entry point
+    if ((rc = scirs_d2d_copy(n, n, dA, lda, dLU, ldlu, s))) return finish(info, rc, t0);
+    int st = getrf_solve_device<double>(n, nrhs, dLU, ldlu, dPerm, dB, ldb, dX, ldx, s, info);
+    return finish(info, st, t0);
+}
+
+// ------------------------------------------------------------------ Cholesky
+template <typename T>
+static int host_potrf(int n, T *a, int64_t lda, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, sizeof(T) == 8 ? SCIRS_PATH_FP64_CHOL : SCIRS_PATH_FP32_CHOL);
+    if (n <= 0 || !a || lda < n) {
+        scirs_set_error("potrf: invalid arguments (n=%d)", n);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    SCIRS_ENTER(true);
+    Context &c = ctx();
+    hipStream_t s = c.stream;
+    int rc;
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(T)))) return finish(info, rc, t0);
+    T *dA = (T *)c.scratch_a;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, hipMemcpyHostToDevice, s));
+    if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
+    if ((rc = scirs_potrf_device<T>(n, dA, n, c.kb_chol, c.work, s))) return finish(info, rc, t0);
+    int st = 0;
+    double pv = 0;
+    if ((rc = c.read_flags(s, &st, &pv))) return finish(info, rc, t0);
+    if (info) info->pivot_value = st ? pv : 0.0;
+    if (st == 0) {
+        SCIRS_HIP_CHECK(hipMemcpy2DAsync(a, lda * sizeof(T), dA, n * sizeof(T), n * sizeof(T), n, hipMemcpyDeviceToHost, s));
+        SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return finish(info, st, t0);
+}
+
+extern "C" int scirs_gpu_dpotrf(int n, double *a, int64_t lda, scirs_solve_info *info)
+{
+    return host_potrf<double>(n, a, lda, info);
+}
+
+extern "C" int scirs_gpu_spotrf(int n, float *a, int64_t lda, scirs_solve_info *info)
+{
+    return host_potrf<float>(n, a, lda, info);
+}
+
+static int posv_device(int n, int nrhs, const double *dA, long lda, const double *dB, long ldb, double *dX, long ldx,
+                       double *dL, long ldl, hipStream_t s, scirs_solve_info *info)
+{
+    Context &c = ctx();
+    int rc;
+    hipEvent_t e0, e1, e2;
+    SCIRS_HIP_CHECK(hipEventCreate(&e0));
+    SCIRS_HIP_CHECK(hipEventCreate(&e1));
+    SCIRS_HIP_CHECK(hipEventCreate(&e2));
+    SCIRS_HIP_CHECK(hipEventRecord(e0, s));
+    if ((rc = scirs_d2d_copy(n, n, dA, lda, dL, ldl, s))) return rc;
+    if ((rc = c.reset_flags(s))) return rc;
+    if ((rc = scirs_potrf_device<double>(n, dL, ldl, c.kb_chol, c.work, s))) return rc;
+    SCIRS_HIP_CHECK(hipEventRecord(e1, s));
+    if ((rc = scirs_d2d_copy(n, nrhs, dB, ldb, dX, ldx, s))) return rc;
+    if ((rc = scirs_potrs_device<double>(n, nrhs, dL, ldl, dX, ldx, c.work.info, s))) return rc;
+    SCIRS_HIP_CHECK(hipEventRecord(e2, s));
+    int st = 0;
+    double pv = 0;
+    if ((rc = c.read_flags(s, &st, &pv))) return rc;
+    if (info) {
+        float ms1 = 0, ms2 = 0;
+        (void)hipEventElapsedTime(&ms1, e0, e1);
+        (void)hipEventElapsedTime(&ms2, e1, e2);
+        info->time_factor_ms = ms1;
+        info->time_solve_ms = ms2;
+        info->pivot_value = st ? pv : 0.0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return st;
+}
+
+extern "C" int scirs_gpu_dposv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb, double *x,
+                               int64_t ldx, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_CHOL);
+    if (n <= 0 || nrhs <= 0 || !a || !b || !x || lda < n || ldb < nrhs || ldx < nrhs) {
+        scirs_set_error("dposv: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    SCIRS_ENTER(true);
+    Context &c = ctx();
+    hipStream_t s = c.stream;
+    int rc;
+    const size_t abytes = (size_t)n * n * sizeof(double), bbytes = (size_t)n * nrhs * sizeof(double);
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, 2 * abytes))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, 2 * bbytes))) return finish(info, rc, t0);
+    double *dA = (double *)c.scratch_a, *dL = dA + (size_t)n * n;
+    double *dB = (double *)c.scratch_b, *dX = dB + (size_t)n * nrhs;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n,
+                                     hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dB, nrhs * sizeof(double), b, ldb * sizeof(double), nrhs * sizeof(double), n,
+                                     hipMemcpyHostToDevice, s));
+    int st = posv_device(n, nrhs, dA, n, dB, nrhs, dX, nrhs, dL, n, s, info);
+    if (st < 0) return finish(info, st, t0);
+    if (st == 0) {
+        SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(double), dX, nrhs * sizeof(double), nrhs * sizeof(double), n,
+                                         hipMemcpyDeviceToHost, s));
+        SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return finish(info, st, t0);
+}
+
+extern "C" int scirs_gpu_dposv_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                                   double *dX, int64_t ldx, double *dL, int64_t ldl, void *stream,
+                                   scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_CHOL);
+    if (n <= 0 || nrhs <= 0 || !dA || !dB || !dX || lda < n || ldb < nrhs || ldx < nrhs) {
+        scirs_set_error("dposv_dev: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    SCIRS_ENTER(true);
+    Context &c = ctx();
+    hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+    int rc;
+    if (!dL) {
+        if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(double)))) return finish(info, rc, t0);
+        dL = (double *)c.scratch_a;
+        ldl = n;
+    }
+    int st = posv_device(n, nrhs, dA, lda, dB, ldb, dX, ldx, dL, ldl, s, info);
+    return finish(info, st, t0);
+}
+
+// ------------------------------------------------------------------ triangular
+extern "C" int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb, double *x,
+                               int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (n <= 0 || nrhs <= 0 || !a || !b || !x || lda < n || ldb < nrhs || ldx < nrhs) {
+        scirs_set_error("dtrsv: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    // |a_ii| < eps on a non-unit diagonal is singular (solve.rs:163-170, 184-191);
+    // the first such row in substitution order is reported.
+    if (!unit_diagonal) {
+        for (int k = 0; k < n; ++k) {
+            int i = lower ? k : n - 1 - k;
+            double d = a[(size_t)i * lda + i];
+            if (fabs(d) < DBL_EPSILON) {
+                if (info) info->pivot_value = fabs(d);
+                return finish(info, i + 1, t0);
+            }
+        }
+    }
+    SCIRS_ENTER(true);
+    Context &c = ctx();
+    hipStream_t s = c.stream;
+    int rc;
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(double)))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, (size_t)n * nrhs * sizeof(double)))) return finish(info, rc, t0);
+    double *dA = (double *)c.scratch_a, *dX = (double *)c.scratch_b;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n,
+                                     hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dX, nrhs * sizeof(double), b, ldb * sizeof(double), nrhs * sizeof(double), n,
+                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs_trsv_device<double>(n, nrhs, dA, n, dX, nrhs, lower, unit_diagonal, nullptr, s))) return finish(info, rc, t0);
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(double), dX, nrhs * sizeof(double), nrhs * sizeof(double), n,
+                                     hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    return finish(info, SCIRS_OK, t0);
+}
+
+// ------------------------------------------------------------------ diagnostics
+extern "C" int scirs_gpu_backward_error_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX,
+                                            int64_t ldx, const double *dB, int64_t ldb, void *stream, double *out)
+{
+    if (n <= 0 || nrhs <= 0 || !dA || !dX || !dB || !out) return SCIRS_ERR_ARG;
+    SCIRS_ENTER(true);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx().stream;
+    return backward_error(n, nrhs, dA, lda, dX, ldx, dB, ldb, s, out);
+}

@@ -1,0 +1,58 @@
+// Process-wide context and helpers shared by the C-ABI translation units.
+#pragma once
+#include "common.hpp"
+#include "kernels.hpp"
+#include "../../include/scirs_gpu.h"
+#include <mutex>
+#include <algorithm>
+#include <cmath>
+
+namespace scirs {
+
+struct Context {
+    std::recursive_mutex mu;
+    bool initialised = false;
+    int device = -1;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    ScirsWork work{};
+    void *work_base = nullptr;
+    void *host_flags = nullptr;
+    int kb_f64 = 128, kb_f32 = 128, kb_chol = 128;
+    // scratch grown on demand (host-memory entry points and internal temporaries)
+    void *scratch_a = nullptr; size_t scratch_a_bytes = 0;
+    void *scratch_b = nullptr; size_t scratch_b_bytes = 0;
+    void *scratch_perm = nullptr; size_t scratch_perm_bytes = 0;
+    void *scratch_bwd = nullptr; size_t scratch_bwd_bytes = 0;
+    void *scratch_ir = nullptr; size_t scratch_ir_bytes = 0;
+    void *scratch_ir2 = nullptr; size_t scratch_ir2_bytes = 0;
+
+    int init(int dev);
+    int ensure(void **p, size_t *have, size_t bytes);
+    int reset_flags(hipStream_t s);
+    int read_flags(hipStream_t s, int *info, double *info_val);
+};
+
+Context &ctx();
+double now_ms();
+void info_init(scirs_solve_info *info, int path);
+int finish(scirs_solve_info *info, int rc, double t0);
+int backward_error(int n, int nrhs, const double *dA, long lda, const double *dX, long ldx, const double *dB,
+                   long ldb, hipStream_t s, double *out);
+
+} // namespace scirs
+
+// vec.hip
+int scirs_residual(int n, int nrhs, const double *A, long lda, const double *X, long ldx, const double *B, long ldb,
+                   double *R, long ldr, hipStream_t s);
+int scirs_colmax_d(int n, int ncols, const double *V, long ldv, double *out, hipStream_t s);
+int scirs_colmax_s(int n, int ncols, const float *V, long ldv, double *out, hipStream_t s);
+int scirs_convert_rowsum(int n, int m, const double *A, long lda, float *A32, long lda32, double *rowsum,
+                         hipStream_t s);
+int scirs_rowsum_max(int n, const double *rowsum, double *out, hipStream_t s);
+int scirs_d2s(int rows, int cols, const double *src, long lds, float *dst, long ldd, hipStream_t s);
+int scirs_s2d(int rows, int cols, const float *src, long lds, double *dst, long ldd, hipStream_t s);
+int scirs_d2d_copy(int rows, int cols, const double *src, long lds, double *dst, long ldd, hipStream_t s);
+int scirs_add_s_to_d(int rows, int cols, const float *D, long ldd, double *X, long ldx, hipStream_t s);
+int scirs_add_d_to_d(int rows, int cols, const double *D, long ldd, double *X, long ldx, hipStream_t s);
+int scirs_fill_int(int *p, int count, int value, hipStream_t s);

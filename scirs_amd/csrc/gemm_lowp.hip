@@ -1,0 +1,278 @@
+// Low-precision Schur-complement update of the FP32 factorisation:
+//     C[M x N] (FP32) -= A[M x K] * op(B)   with FP32 storage everywhere.
+// The operands are rounded to 16-bit floats while they are staged into LDS
+// and multiplied on the gfx950 v_mfma_f32_16x16x32_{bf16,f16} matrix cores
+// with FP32 accumulation; C stays FP32.
+//   SCIRS_LOWP_BF16    one BF16 product (u = 2^-8 per operand)
+//   SCIRS_LOWP_FP16    one FP16 product; A is pre-scaled by 2^a_shift and
+//                      B column c by 2^-colexp[c] (powers of two, undone exactly
+//                      in the epilogue) so that neither operand leaves FP16 range
+//   SCIRS_LOWP_BF16X3  x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
+//                      C -= Ahi Bhi + Ahi Blo + Alo Bhi  (about 16 significant
+//                      bits per operand, three matrix-core products)
+// Tiling: 128 x 128 per 256-thread workgroup, 2 x 2 waves of 64 x 64, K slabs
+// of 32 (one MFMA k-step) double-buffered in LDS.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "tile_map.hpp"
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int PITCH = BK + 8;   // 16-bit elements per LDS row (80 B: keeps 16 B alignment)
+constexpr int NTHREADS = 256;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+__device__ __forceinline__ u16 f32_to_bf16_rne(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
+__device__ __forceinline__ u16 f32_to_f16(float x)
+{
+    _Float16 h = (_Float16)x;
+    return *reinterpret_cast<u16 *>(&h);
+}
+
+template <int MODE>
+struct LowpArgs {
+    int M, N, K;
+    const float *A; long lda;
+    const float *B; long ldb;
+    float *C; long ldc;
+    int tiles_m, tiles_n;
+    int lower;
+    const int *info;
+    const int *colexp;   // FP16: per-column exponent of B (may be null)
+    int a_shift;         // FP16: A is multiplied by 2^a_shift before rounding
+};
+
+template <int MODE, bool TRANS_B>
+__global__ __launch_bounds__(NTHREADS) void lowp_gemm_kernel(LowpArgs<MODE> g)
+{
+    if (g.info && *g.info != 0) return;
+    int tm, tn;
+    scirs_tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n, tm, tn);
+    if (tm >= g.tiles_m || tn >= g.tiles_n) return;
+    const int row0 = tm * BM, col0 = tn * BN;
+    if (g.lower && col0 > row0 + BM - 1) return;
+
+    constexpr int NSPLIT = (MODE == SCIRS_LOWP_BF16X3) ? 2 : 1;
+    __shared__ __align__(16) u16 As[2][NSPLIT][BM * PITCH];
+    __shared__ __align__(16) u16 Bs[2][NSPLIT][BN * PITCH];
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    // A slab: 128 rows x 32 k; thread t loads row t/2, k (t%2)*16 .. +16
+    const int a_r = t >> 1, a_k = (t & 1) * 16;
+    // B slab (row-major K x N): thread t loads k = t%32, cols (t/32)*16 .. +16
+    const int b_k = t & 31, b_c = (t >> 5) * 16;
+    float ra[16], rb[16];
+    const float ascale = (MODE == SCIRS_LOWP_FP16) ? __builtin_ldexpf(1.0f, g.a_shift) : 1.0f;
+
+    auto load_slab = [&](int k0) {
+        const int gr = row0 + a_r;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int gk = k0 + a_k + i;
+            ra[i] = (gr < g.M && gk < g.K) ? g.A[(long)gr * g.lda + gk] : 0.0f;
+        }
+        if (!TRANS_B) {
+            const int gk = k0 + b_k;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int gc = col0 + b_c + i;
+                rb[i] = (gk < g.K && gc < g.N) ? g.B[(long)gk * g.ldb + gc] : 0.0f;
+            }
+        } else {
+            const int gc = col0 + a_r;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int gk = k0 + a_k + i;
+                rb[i] = (gc < g.N && gk < g.K) ? g.B[(long)gc * g.ldb + gk] : 0.0f;
+            }
+        }
+    };
+    auto cvt = [&](float x, u16 &hi, u16 &lo) {
+        if (MODE == SCIRS_LOWP_FP16) {
+            hi = f32_to_f16(x);
+        } else {
+            hi = f32_to_bf16_rne(x);
+            if (MODE == SCIRS_LOWP_BF16X3) lo = f32_to_bf16_rne(x - bf16_to_f32(hi));
+        }
+    };
+    auto store_slab = [&](int buf) {
+        u16 hi[16], lo[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) cvt(ra[i] * ascale, hi[i], lo[i]);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) As[buf][0][a_r * PITCH + a_k + i] = hi[i];
+        if (NSPLIT == 2) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) As[buf][NSPLIT - 1][a_r * PITCH + a_k + i] = lo[i];
+        }
+        if (!TRANS_B) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                float x = rb[i];
+                if (MODE == SCIRS_LOWP_FP16 && g.colexp) {
+                    const int gc = col0 + b_c + i;
+                    x = __builtin_ldexpf(x, gc < g.N ? -g.colexp[gc] : 0);
+                }
+                cvt(x, hi[i], lo[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Bs[buf][0][(b_c + i) * PITCH + b_k] = hi[i];
+            if (NSPLIT == 2) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) Bs[buf][NSPLIT - 1][(b_c + i) * PITCH + b_k] = lo[i];
+            }
+        } else {
+            float sc = 1.0f;
+            if (MODE == SCIRS_LOWP_FP16 && g.colexp) {
+                const int gc = col0 + a_r;
+                sc = __builtin_ldexpf(1.0f, gc < g.N ? -g.colexp[gc] : 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) cvt(rb[i] * sc, hi[i], lo[i]);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Bs[buf][0][a_r * PITCH + a_k + i] = hi[i];
+            if (NSPLIT == 2) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) Bs[buf][NSPLIT - 1][a_r * PITCH + a_k + i] = lo[i];
+            }
+        }
+    };
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+    const int nslab = (g.K + BK - 1) / BK;
+    load_slab(0);
+    store_slab(0);
+    __syncthreads();
+
+    const int lr = lane & 15;        // row of A / column of B inside a 16-block
+    const int lk = (lane >> 4) * 8;  // first of the lane's 8 k values
+
+    for (int s = 0; s < nslab; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nslab) load_slab((s + 1) * BK);
+        typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+        u16x8 ah[4], bh[4], al[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ah[i] = *reinterpret_cast<const u16x8 *>(&As[buf][0][(wm * 64 + i * 16 + lr) * PITCH + lk]);
+            bh[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][0][(wn * 64 + i * 16 + lr) * PITCH + lk]);
+            if (NSPLIT == 2) {
+                al[i] = *reinterpret_cast<const u16x8 *>(&As[buf][NSPLIT - 1][(wm * 64 + i * 16 + lr) * PITCH + lk]);
+                bl[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][NSPLIT - 1][(wn * 64 + i * 16 + lr) * PITCH + lk]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (MODE == SCIRS_LOWP_FP16) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(f16x8, ah[i]), __builtin_bit_cast(f16x8, bh[j]), acc[i][j], 0, 0, 0);
+                } else {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, ah[i]), __builtin_bit_cast(bf16x8, bh[j]), acc[i][j], 0, 0, 0);
+                    if (MODE == SCIRS_LOWP_BF16X3) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, ah[i]), __builtin_bit_cast(bf16x8, bl[j]), acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, al[i]), __builtin_bit_cast(bf16x8, bh[j]), acc[i][j], 0, 0, 0);
+                    }
+                }
+            }
+        if (s + 1 < nslab) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+
+    const int lq = (lane >> 4) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = col0 + wn * 64 + j * 16 + lr;
+        if (c >= g.N) continue;
+        float sc = 1.0f;
+        if (MODE == SCIRS_LOWP_FP16) sc = __builtin_ldexpf(1.0f, (g.colexp ? g.colexp[c] : 0) - g.a_shift);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = row0 + wm * 64 + i * 16 + lq + q;
+                if (r < g.M) {
+                    float *p = g.C + (long)r * g.ldc + c;
+                    *p = *p - acc[i][j][q] * sc;
+                }
+            }
+        }
+    }
+}
+
+template <int MODE>
+int launch_lowp(int M, int N, int K, const float *A, long lda, const float *B, long ldb, int trans_b,
+                float *C, long ldc, int lower, const int *info, const int *colexp, int a_shift,
+                hipStream_t stream)
+{
+    LowpArgs<MODE> g;
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+    g.tiles_m = (M + BM - 1) / BM;
+    g.tiles_n = (N + BN - 1) / BN;
+    g.lower = lower; g.info = info; g.colexp = colexp; g.a_shift = a_shift;
+    const int nwg = (g.tiles_m * g.tiles_n + 7) / 8 * 8;
+    if (trans_b)
+        hipLaunchKernelGGL((lowp_gemm_kernel<MODE, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    else
+        hipLaunchKernelGGL((lowp_gemm_kernel<MODE, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+
+} // namespace
+
+int scirs_launch_sgemm_f32_sub(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
+                               int trans_b, float *C, long ldc, int lower, const int *info,
+                               hipStream_t stream);
+
+int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
+                           int trans_b, float *C, long ldc, int lower, int mode, const int *info,
+                           hipStream_t stream)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return SCIRS_OK;
+    switch (mode) {
+    case SCIRS_LOWP_FP32:
+        return scirs_launch_sgemm_f32_sub(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, stream);
+    case SCIRS_LOWP_BF16:
+        return launch_lowp<SCIRS_LOWP_BF16>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 0, stream);
+    case SCIRS_LOWP_BF16X3:
+        return launch_lowp<SCIRS_LOWP_BF16X3>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 0, stream);
+    case SCIRS_LOWP_FP16:
+        // |l_ij| <= 1 under partial pivoting: 2^14 keeps A in range and lifts small multipliers
+        return launch_lowp<SCIRS_LOWP_FP16>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 14, stream);
+    default:
+        scirs_set_error("unknown low-precision mode %d", mode);
+        return SCIRS_ERR_ARG;
+    }
+}
+
+int scirs_launch_sgemm_fp16_scaled(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
+                                   float *C, long ldc, const int *colexp, int a_shift, const int *info,
+                                   hipStream_t stream)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return SCIRS_OK;
+    return launch_lowp<SCIRS_LOWP_FP16>(M, N, K, A, lda, B, ldb, 0, C, ldc, 0, info, colexp, a_shift, stream);
+}

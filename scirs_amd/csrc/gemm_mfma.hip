@@ -1,0 +1,232 @@
+// Schur-complement update on the gfx950 matrix cores, FP64 and FP32:
+//     C[M x N] -= A[M x K] * op(B)      (all row-major)
+// op(B) = B (K x N)  or  B^T with B stored N x K (the Cholesky SYRK case).
+//
+// Tiling: 128 x 128 output tile per 256-thread workgroup (4 wave64s in a 2 x 2
+// arrangement, 64 x 64 per wave = 4 x 4 v_mfma_{f64,f32}_16x16x4 blocks), K
+// staged through LDS in slabs of 16 with a register prefetch of the next slab,
+// so each slab costs one barrier.  Both operand tiles live k-major in LDS,
+// which makes every MFMA operand read a run of 16 consecutive elements.
+//
+// Workgroups are renumbered so that the tiles one XCD works on share A rows
+// and B columns in that XCD's L2: hardware dispatches workgroup w to XCD w % 8.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "tile_map.hpp"
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int PAD = 4;
+constexpr int LDA_S = BM + PAD;
+constexpr int LDB_S = BN + PAD;
+constexpr int NTHREADS = 256;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct acc_t;
+template <> struct acc_t<double> { typedef d4 type; };
+template <> struct acc_t<float> { typedef f4 type; };
+
+__device__ __forceinline__ d4 mfma16x16x4(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma16x16x4(float a, float b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+struct GemmArgs {
+    int M, N, K;
+    const T *A; long lda;
+    const T *B; long ldb;
+    T *C; long ldc;
+    int tiles_m, tiles_n;
+    int lower;
+    const int *info;
+};
+
+template <typename T, bool TRANS_B>
+__global__ __launch_bounds__(NTHREADS) void gemm_sub_kernel(GemmArgs<T> g)
+{
+    if (g.info && *g.info != 0) return;
+    int tm, tn;
+    scirs_tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n, tm, tn);
+    if (tm >= g.tiles_m || tn >= g.tiles_n) return;
+    const int row0 = tm * BM, col0 = tn * BN;
+    if (g.lower && col0 > row0 + BM - 1) return;
+    typedef typename acc_t<T>::type V4;
+
+    __shared__ T As[2][BK * LDA_S];
+    __shared__ T Bs[2][BK * LDB_S];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    const int a_r = t >> 1, a_k = (t & 1) * 8;
+    const int b_k = t >> 4, b_c = (t & 15) * 8;
+    T ra[8], rb[8];
+
+    auto load_slab = [&](int k0) {
+        {
+            int gr = row0 + a_r;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                int gk = k0 + a_k + i;
+                ra[i] = (gr < g.M && gk < g.K) ? g.A[(long)gr * g.lda + gk] : (T)0;
+            }
+        }
+        if (!TRANS_B) {
+            int gk = k0 + b_k;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                int gc = col0 + b_c + i;
+                rb[i] = (gk < g.K && gc < g.N) ? g.B[(long)gk * g.ldb + gc] : (T)0;
+            }
+        } else {
+            int gc = col0 + a_r;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                int gk = k0 + a_k + i;
+                rb[i] = (gc < g.N && gk < g.K) ? g.B[(long)gc * g.ldb + gk] : (T)0;
+            }
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) As[buf][(a_k + i) * LDA_S + a_r] = ra[i];
+        if (!TRANS_B) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) Bs[buf][b_k * LDB_S + b_c + i] = rb[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) Bs[buf][(a_k + i) * LDB_S + a_r] = rb[i];
+        }
+    };
+
+    V4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = V4{0, 0, 0, 0};
+
+    const int nslab = (g.K + BK - 1) / BK;
+    load_slab(0);
+    store_slab(0);
+    __syncthreads();
+
+    const int lr = lane & 15;
+    const int lk = lane >> 4;
+
+    for (int s = 0; s < nslab; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nslab) load_slab((s + 1) * BK);
+        const T *as = As[buf];
+        const T *bs = Bs[buf];
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = as[(kk + lk) * LDA_S + wm * 64 + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = bs[(kk + lk) * LDB_S + wn * 64 + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma16x16x4(af[i], bf[j], acc[i][j]);
+        }
+        if (s + 1 < nslab) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue: C -= acc.  Lane l, register q of a 16x16 block holds column l%16 and
+    // row 4*(l/16)+q for v_mfma_f32_16x16x4f32 but row (l/16)+4*q for
+    // v_mfma_f64_16x16x4f64 (measured on gfx950).
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = col0 + wn * 64 + j * 16 + lr;
+            if (c >= g.N) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rin = (sizeof(T) == 8) ? (lk + 4 * q) : (lk * 4 + q);
+                const int r = row0 + wm * 64 + i * 16 + rin;
+                if (r < g.M) {
+                    T *p = g.C + (long)r * g.ldc + c;
+                    *p = *p - acc[i][j][q];
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+int launch_gemm_sub(int M, int N, int K, const T *A, long lda, const T *B, long ldb, int trans_b, T *C,
+                    long ldc, int lower, const int *info, hipStream_t stream)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return SCIRS_OK;
+    GemmArgs<T> g;
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+    g.tiles_m = (M + BM - 1) / BM;
+    g.tiles_n = (N + BN - 1) / BN;
+    g.lower = lower;
+    g.info = info;
+    const int nwg = (g.tiles_m * g.tiles_n + 7) / 8 * 8;
+    if (trans_b)
+        hipLaunchKernelGGL((gemm_sub_kernel<T, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    else
+        hipLaunchKernelGGL((gemm_sub_kernel<T, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+
+} // namespace
+
+int scirs_launch_dgemm_sub(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                           int trans_b, double *C, long ldc, int lower, const int *info,
+                           hipStream_t stream)
+{
+    return launch_gemm_sub<double>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, stream);
+}
+
+int scirs_launch_sgemm_f32_sub(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
+                               int trans_b, float *C, long ldc, int lower, const int *info,
+                               hipStream_t stream)
+{
+    return launch_gemm_sub<float>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, stream);
+}
+
+template <>
+int scirs_trailing_update<double>(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                                  double *C, long ldc, const ScirsWork &w, hipStream_t stream)
+{
+    return scirs_launch_dgemm_sub(M, N, K, A, lda, B, ldb, 0, C, ldc, 0, w.info, stream);
+}
+
+template <>
+int scirs_trailing_update<float>(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
+                                 float *C, long ldc, const ScirsWork &w, hipStream_t stream)
+{
+    return scirs_launch_sgemm_sub(M, N, K, A, lda, B, ldb, 0, C, ldc, 0, w.lowp_mode, w.info, stream);
+}
+
+template <>
+int scirs_syrk_update<double>(int N, int K, const double *A, long lda, double *C, long ldc, const ScirsWork &w,
+                              hipStream_t stream)
+{
+    return scirs_launch_dgemm_sub(N, N, K, A, lda, A, lda, 1, C, ldc, 1, w.info, stream);
+}
+
+template <>
+int scirs_syrk_update<float>(int N, int K, const float *A, long lda, float *C, long ldc, const ScirsWork &w,
+                             hipStream_t stream)
+{
+    return scirs_launch_sgemm_sub(N, N, K, A, lda, A, lda, 1, C, ldc, 1, w.lowp_mode, w.info, stream);
+}

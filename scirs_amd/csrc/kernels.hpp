@@ -1,0 +1,80 @@
+// Internal launchers shared between the kernel files and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// C[M x N] -= A[M x K] * op(B); op(B) = B (K x N) or, when trans_b, B stored N x K.
+// lower != 0 skips 128x128 tiles strictly above C's diagonal.  Does nothing when *info != 0.
+int scirs_launch_dgemm_sub(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                           int trans_b, double *C, long ldc, int lower, const int *info,
+                           hipStream_t stream);
+
+// Device scratch used by the factorisations (allocated once per context).
+struct ScirsWork {
+    void *cval;          // [2][MAX_G] candidate |max| (element type of the panel)
+    int *cidx;           // [2][MAX_G] candidate row (panel-relative), -1 = none
+    void *crow;          // [2][MAX_G][KB_MAX] candidate rows
+    void *rowj;          // [2][KB_MAX] current diagonal row
+    unsigned *bar;       // grid-barrier counter
+    unsigned *done;      // workgroups finished (resets bar)
+    int *abort;          // raised by a timed-out barrier
+    int *pairs;          // [n_panels_max][2*KB_MAX][2] row gather lists (dst, src), absolute
+    int *npairs;         // [n_panels_max]
+    int *info;           // 1-based failing column / row, 0 = ok
+    double *info_val;    // |pivot| (LU) or a_ii - sum (Cholesky) at the failure
+    int max_panels;
+    int max_g;           // co-resident workgroup limit for cooperative launches
+    unsigned spin_limit;
+    int lowp_mode;       // trailing-update arithmetic of the FP32 factorisation (SCIRS_LOWP_*)
+    int panel_rows;      // target rows per workgroup in the cooperative panel
+};
+
+enum {
+    SCIRS_LOWP_FP32 = 0,   // FP32 operands (VALU FMA)
+    SCIRS_LOWP_BF16 = 1,   // BF16 operands, FP32 accumulation (MFMA)
+    SCIRS_LOWP_FP16 = 2,   // FP16 operands with per-block power-of-two scaling, FP32 accumulation
+    SCIRS_LOWP_BF16X3 = 3, // FP32 split into hi+lo BF16 pairs, three MFMA products
+};
+
+enum { SCIRS_KB_MAX = 256, SCIRS_MAX_G = 512 };
+
+// Trailing (Schur) update C -= A B of the factorisations: FP64 MFMA for double,
+// the arithmetic chosen by w.lowp_mode for float.
+template <typename T>
+int scirs_trailing_update(int M, int N, int K, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
+                          const ScirsWork &w, hipStream_t stream);
+// C (N x N, lower tiles) -= A A^T with A N x K.
+template <typename T>
+int scirs_syrk_update(int N, int K, const T *A, long lda, T *C, long ldc, const ScirsWork &w,
+                      hipStream_t stream);
+
+template <typename T>
+int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);
+template <typename T>
+int scirs_potrs_device(int n, int nrhs, const T *L, long ldl, T *B, long ldb, const int *info,
+                       hipStream_t stream);
+template <typename T>
+int scirs_trsv_device(int n, int nrhs, const T *A, long lda, T *B, long ldb, int lower, int unit,
+                      const int *info, hipStream_t stream);
+
+// FP32 update C -= A * op(B) in the arithmetic of `mode`.
+int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
+                           int trans_b, float *C, long ldc, int lower, int mode, const int *info,
+                           hipStream_t stream);
+
+template <typename T>
+int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);
+
+// Row permutation produced by the last getrf: perm[i] = original row now at i
+// (the `piv` vector of scirs2-linalg's LUDecomposition).
+int scirs_perm_from_pairs(int m, int n, int kb, const ScirsWork &w, int64_t *perm, hipStream_t stream);
+
+// X[i, :] = B[perm[i], :] for an n x nrhs row-major block.
+template <typename T>
+int scirs_permute_rows(int n, int nrhs, const int64_t *perm, const T *B, long ldb, T *X, long ldx,
+                       hipStream_t stream);
+
+// Solve L U X = B in place on B (B already row-permuted); L unit lower, U upper, packed in A.
+template <typename T>
+int scirs_getrs_device(int n, int nrhs, const T *A, long lda, T *B, long ldb, const int *info,
+                       hipStream_t stream);

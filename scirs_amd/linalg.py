@@ -1,0 +1,187 @@
+"""numpy / torch front end of the C ABI (test and benchmark harness).
+
+Host arrays go through the host-memory entry points; torch CUDA tensors go
+through the *_dev entry points without leaving HBM.  Every routine calls the
+HIP library; there is no CPU path here.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+
+class GpuSolveError(RuntimeError):
+    def __init__(self, fn: str, rc: int, info: N.SolveInfo | None = None):
+        msg = N.last_error() if rc < 0 else ""
+        super().__init__(f"{fn} returned {rc} {msg}".strip())
+        self.rc = rc
+        self.info = info
+
+
+class SingularError(GpuSolveError):
+    """rc > 0 from an LU routine: |pivot| < eps at 1-based column rc."""
+
+
+class NotPositiveDefiniteError(GpuSolveError):
+    """rc > 0 from a Cholesky routine: a_ii - sum <= 0 at 1-based row rc."""
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _check(fn, rc, info, chol=False):
+    if rc > 0:
+        raise (NotPositiveDefiniteError if chol else SingularError)(fn, rc, info)
+    if rc < 0:
+        raise GpuSolveError(fn, rc, info)
+
+
+def getrf(a):
+    """Packed LU and the reference-style permutation vector (P A = L U)."""
+    lib = N.load()
+    a = np.asarray(a)
+    dt = np.float32 if a.dtype == np.float32 else np.float64
+    lu = _c(a, dt).copy()
+    m, n = lu.shape
+    perm = np.empty(m, dtype=np.int64)
+    info = N.SolveInfo()
+    fn = lib.scirs_gpu_sgetrf if dt == np.float32 else lib.scirs_gpu_dgetrf
+    rc = fn(m, n, _ptr(lu), n, _ptr(perm), ctypes.byref(info))
+    _check("getrf", rc, info)
+    return lu, perm, info
+
+
+def gesv(a, b):
+    lib = N.load()
+    a = np.asarray(a)
+    dt = np.float32 if a.dtype == np.float32 else np.float64
+    a = _c(a, dt)
+    b = np.asarray(b, dtype=dt)
+    vec = b.ndim == 1
+    b2 = _c(b.reshape(b.shape[0], -1), dt)
+    n, nrhs = b2.shape
+    x = np.empty_like(b2)
+    info = N.SolveInfo()
+    fn = lib.scirs_gpu_sgesv if dt == np.float32 else lib.scirs_gpu_dgesv
+    rc = fn(n, nrhs, _ptr(a), a.shape[1], _ptr(b2), nrhs, _ptr(x), nrhs, ctypes.byref(info))
+    _check("gesv", rc, info)
+    return (x[:, 0] if vec else x), info
+
+
+def potrf(a):
+    lib = N.load()
+    a = np.asarray(a)
+    dt = np.float32 if a.dtype == np.float32 else np.float64
+    l = _c(a, dt).copy()
+    n = l.shape[0]
+    info = N.SolveInfo()
+    fn = lib.scirs_gpu_spotrf if dt == np.float32 else lib.scirs_gpu_dpotrf
+    rc = fn(n, _ptr(l), n, ctypes.byref(info))
+    _check("potrf", rc, info, chol=True)
+    return l, info
+
+
+def posv(a, b):
+    lib = N.load()
+    a = _c(a, np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    vec = b.ndim == 1
+    b2 = _c(b.reshape(b.shape[0], -1), np.float64)
+    n, nrhs = b2.shape
+    x = np.empty_like(b2)
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dposv(n, nrhs, _ptr(a), n, _ptr(b2), nrhs, _ptr(x), nrhs, ctypes.byref(info))
+    _check("posv", rc, info, chol=True)
+    return (x[:, 0] if vec else x), info
+
+
+def trsv(a, b, lower, unit_diagonal):
+    lib = N.load()
+    a = _c(a, np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    vec = b.ndim == 1
+    b2 = _c(b.reshape(b.shape[0], -1), np.float64)
+    n, nrhs = b2.shape
+    x = np.empty_like(b2)
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dtrsv(n, nrhs, _ptr(a), n, _ptr(b2), nrhs, _ptr(x), nrhs, int(lower), int(unit_diagonal),
+                             ctypes.byref(info))
+    _check("trsv", rc, info)
+    return (x[:, 0] if vec else x), info
+
+
+def gesv_ir(a, b, mode=N.MODE_AUTO, max_iter=0, max_gmres=0, tol=0.0):
+    lib = N.load()
+    a = _c(a, np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    vec = b.ndim == 1
+    b2 = _c(b.reshape(b.shape[0], -1), np.float64)
+    n, nrhs = b2.shape
+    x = np.empty_like(b2)
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dgesv_ir(n, nrhs, _ptr(a), n, _ptr(b2), nrhs, _ptr(x), nrhs, int(mode), int(max_iter),
+                                int(max_gmres), float(tol), ctypes.byref(info))
+    _check("gesv_ir", rc, info)
+    return (x[:, 0] if vec else x), info
+
+
+# ---------------------------------------------------------------- torch (HBM)
+def _tp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_ptr():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def gesv_dev(A, B, X, LU=None, perm=None):
+    """A, B, X: contiguous float64 CUDA tensors (n x n, n x nrhs, n x nrhs)."""
+    lib = N.load()
+    n, nrhs = B.shape
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dgesv_dev(n, nrhs, _tp(A), A.stride(0), _tp(B), B.stride(0), _tp(X), X.stride(0),
+                                 _tp(LU) if LU is not None else None, LU.stride(0) if LU is not None else 0,
+                                 _tp(perm) if perm is not None else None, _stream_ptr(), ctypes.byref(info))
+    _check("gesv_dev", rc, info)
+    return info
+
+
+def posv_dev(A, B, X, L=None):
+    lib = N.load()
+    n, nrhs = B.shape
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dposv_dev(n, nrhs, _tp(A), A.stride(0), _tp(B), B.stride(0), _tp(X), X.stride(0),
+                                 _tp(L) if L is not None else None, L.stride(0) if L is not None else 0,
+                                 _stream_ptr(), ctypes.byref(info))
+    _check("posv_dev", rc, info, chol=True)
+    return info
+
+
+def gesv_ir_dev(A, B, X, mode=N.MODE_AUTO, max_iter=0, max_gmres=0, tol=0.0):
+    lib = N.load()
+    n, nrhs = B.shape
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dgesv_ir_dev(n, nrhs, _tp(A), A.stride(0), _tp(B), B.stride(0), _tp(X), X.stride(0),
+                                    int(mode), int(max_iter), int(max_gmres), float(tol), _stream_ptr(),
+                                    ctypes.byref(info))
+    _check("gesv_ir_dev", rc, info)
+    return info
+
+
+def backward_error_dev(A, X, B):
+    lib = N.load()
+    n, nrhs = B.shape
+    out = ctypes.c_double(0.0)
+    rc = lib.scirs_gpu_backward_error_dev(n, nrhs, _tp(A), A.stride(0), _tp(X), X.stride(0), _tp(B), B.stride(0),
+                                          _stream_ptr(), ctypes.byref(out))
+    _check("backward_error_dev", rc, None)
+    return out.value

@@ -1,0 +1,316 @@
+"""Pins the CPU oracle (oracle/ref_linalg.c) before anything is checked against it.
+
+What pins which oracle function:
+
+* lu_factor  — exact known answer of lapack.rs:688-734; pivot sequence equal to
+  LAPACK dgetrf's at n = 5..512 (same first-maximum rule) and factors equal to
+  rounding; exact rational Gaussian elimination with the same pivot rule on
+  small integer matrices; the first-maximum tie rule; the absolute-eps
+  singularity rule (lapack.rs:106-120); invariants P*A = L*U, |l_ij| <= 1.
+* cholesky   — exact answers of lapack.rs:737-754 and the textbook 3x3 of
+  tests/scipy_validation.rs:131-152; LAPACK dpotrf at n = 5..64; exact
+  rational Cholesky of perfect-square cases; reads only the lower triangle;
+  the a_ii - sum <= 0 rule.
+* solve_triangular — the four cases of solve.rs:474-507 and exact rationals.
+* solve / solve_multiple — solve.rs:458-472, scipy_validation.rs:178-189,
+  residual cases of scipy_compat_*; LAPACK dgetrs at n = 5..512; exact
+  rational solutions of integer systems; the n <= 4 inverse branch.
+
+The mutation tests at the end plant plausible mistakes in a pure-Python copy
+of the elimination (not in the oracle itself) and show that these checks
+reject each of them, i.e. the checks are not a restatement of the formula.
+"""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+with open(os.path.join(GOLD, "reference_kats.json")) as f:
+    KATS = json.load(f)
+
+EPS64 = np.finfo(np.float64).eps
+
+
+def _lapack(n):
+    return np.load(os.path.join(GOLD, f"lapack_n{n}.npz"))
+
+
+# ---------------------------------------------------------------- checks ---
+def check_lu_kat(lu_factor):
+    k = KATS["lu_factor_2x2"]
+    lu, piv, _ = lu_factor(np.array(k["a"]))
+    assert np.array_equal(lu, np.array(k["lu"]))
+    assert list(piv) == k["piv"]
+
+
+def check_lu_lapack(lu_factor, n):
+    d = _lapack(n)
+    lu, piv, _ = lu_factor(d["a"])
+    assert np.array_equal(piv, d["perm"]), "pivot sequence differs from LAPACK dgetrf"
+    if "lu" in d:
+        assert np.allclose(lu, d["lu"], rtol=1e-11, atol=1e-11 * n)
+
+
+def check_lu_invariants(lu_factor, a):
+    lu, piv, _ = lu_factor(a)
+    n = a.shape[0]
+    l = np.tril(lu, -1) + np.eye(n)
+    u = np.triu(lu)
+    pa = a[piv]
+    assert np.all(np.abs(np.tril(lu, -1)) <= 1.0)
+    res = np.abs(pa - l @ u).max() / (np.abs(a).max() * n)
+    assert res < 10 * EPS64
+
+
+def exact_lu(a_int):
+    """Exact rational GE with partial pivoting, first-max rule."""
+    a = [[Fraction(v) for v in row] for row in a_int]
+    n = len(a)
+    piv = list(range(n))
+    for k in range(n):
+        p = max(range(k, n), key=lambda i: (abs(a[i][k]), -i))
+        a[k], a[p] = a[p], a[k]
+        piv[k], piv[p] = piv[p], piv[k]
+        for i in range(k + 1, n):
+            a[i][k] = a[i][k] / a[k][k]
+            for j in range(k + 1, n):
+                a[i][j] -= a[i][k] * a[k][j]
+    return a, piv
+
+
+def check_lu_exact(lu_factor, seed):
+    rng = np.random.default_rng(seed)
+    n = 6
+    a = rng.integers(-9, 10, size=(n, n)).astype(np.float64)
+    a += np.eye(n) * 0.5  # keeps it non-singular and avoids exact ties
+    ex, piv_ex = exact_lu(a.tolist())
+    lu, piv, _ = lu_factor(a)
+    assert list(piv) == piv_ex
+    ex_f = np.array([[float(v) for v in row] for row in ex])
+    assert np.allclose(lu, ex_f, rtol=1e-13, atol=1e-13)
+
+
+def check_lu_tie_rule(lu_factor):
+    # column 0 has |1| at rows 0 and 2 and |-1| at row 3: the first maximum (row 0) wins
+    a = np.array([[1.0, 2.0, 0.0, 1.0],
+                  [0.5, 1.0, 3.0, 0.0],
+                  [1.0, 0.0, 1.0, 2.0],
+                  [-1.0, 1.0, 1.0, 1.0]])
+    _, piv, _ = lu_factor(a)
+    assert piv[0] == 0
+
+
+# ------------------------------------------------------------- the tests ---
+def test_lu_kat():
+    check_lu_kat(oracle.lu_factor)
+
+
+@pytest.mark.parametrize("n", [5, 8, 16, 64, 512])
+def test_lu_matches_lapack(n):
+    check_lu_lapack(oracle.lu_factor, n)
+
+
+@pytest.mark.parametrize("n", [5, 8, 16, 64])
+def test_lu_invariants(n):
+    check_lu_invariants(oracle.lu_factor, _lapack(n)["a"])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_lu_exact_rational(seed):
+    check_lu_exact(oracle.lu_factor, seed)
+
+
+def test_lu_tie_rule():
+    check_lu_tie_rule(oracle.lu_factor)
+
+
+def test_lu_property_set():
+    k = KATS["lu_property_set"]
+    for m in k["mats"]:
+        a = np.array(m)
+        p, l, u = oracle.lu(a)
+        assert np.allclose(p @ a, l @ u, atol=k["tol"])
+        assert np.allclose(np.diag(l), 1.0)
+        assert np.all(np.triu(l, 1) == 0) and np.all(np.tril(u, -1) == 0)
+
+
+def test_lu_singular_absolute_eps():
+    # a column whose largest entry is below 2^-52 is "singular" for the reference
+    a = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 1.0], [1.0, 2.0, 7.0]])
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.lu_factor(a)
+    assert e.value.code == 2
+    # a perfectly well-conditioned matrix scaled below eps is singular too
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.lu_factor(np.eye(3) * 1e-17)
+    assert e.value.code == 1
+    assert e.value.cond_estimate == pytest.approx(1e17)
+    # just above eps it is not
+    oracle.lu_factor(np.eye(3) * 4 * EPS64)
+
+
+def test_lu_rectangular():
+    rng = np.random.default_rng(5)
+    for shape in ((7, 4), (4, 7)):
+        a = rng.standard_normal(shape)
+        p, l, u = oracle.lu(a)
+        assert p.shape == (shape[0], shape[0])
+        assert l.shape == (shape[0], min(shape))
+        assert u.shape == (min(shape), shape[1])
+        assert np.allclose(p @ a, l @ u, atol=1e-13)
+
+
+def test_cholesky_kats():
+    for key in ("cholesky_2x2", "cholesky_3x3_textbook"):
+        k = KATS[key]
+        l = oracle.cholesky(np.array(k["a"]))
+        assert np.array_equal(l, np.array(k["l"]))
+    k = KATS["cholesky_spd_set"]
+    for m in k["mats"]:
+        a = np.array(m)
+        l = oracle.cholesky(a)
+        assert np.allclose(l @ l.T, a, atol=k["tol"])
+        assert np.all(np.triu(l, 1) == 0)
+
+
+@pytest.mark.parametrize("n", [5, 8, 16, 64])
+def test_cholesky_matches_lapack(n):
+    d = _lapack(n)
+    l = oracle.cholesky(d["spd"])
+    assert np.allclose(l, d["lchol"], rtol=1e-12, atol=1e-13)
+    x = oracle.cholesky_solve(d["spd"], d["b"])
+    assert np.allclose(x, d["xs"], rtol=1e-10, atol=1e-12)
+
+
+def test_cholesky_reads_lower_only():
+    a = np.array(KATS["cholesky_3x3_textbook"]["a"])
+    junk = a.copy()
+    junk[np.triu_indices(3, 1)] = 1e300
+    assert np.array_equal(oracle.cholesky(junk), oracle.cholesky(a))
+
+
+def test_cholesky_exact_rational():
+    # A = L L^T with small integer L: the factor is recovered exactly
+    rng = np.random.default_rng(7)
+    l = np.tril(rng.integers(-4, 5, size=(6, 6))).astype(np.float64)
+    np.fill_diagonal(l, rng.integers(1, 5, size=6))
+    a = l @ l.T
+    assert np.array_equal(oracle.cholesky(a), l)
+
+
+def test_cholesky_not_pd():
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.cholesky(np.array([[1.0, 2.0], [2.0, 1.0]]))
+    assert e.value.kind == "not_positive_definite" and e.value.code == 2
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.cholesky(np.array([[0.0, 0.0], [0.0, 1.0]]))
+    assert e.value.code == 1
+
+
+def test_solve_triangular_kats():
+    for c in KATS["solve_triangular_cases"]["cases"]:
+        x = oracle.solve_triangular(np.array(c["a"]), np.array(c["b"]), c["lower"], c["unit"])
+        assert np.allclose(x, c["x"], rtol=1e-15, atol=0)
+
+
+def test_solve_kats():
+    k = KATS["solve_cases"]
+    for c in k["cases"]:
+        x = oracle.solve(np.array(c["a"]), np.array(c["b"]))
+        assert np.allclose(x, c["x"], atol=k["tol"])
+    for c in KATS["solve_residual_cases"]["cases"]:
+        a, b = np.array(c["a"]), np.array(c["b"])
+        x = oracle.solve_multiple(a, b)
+        assert np.linalg.norm(a @ x - b) < c["tol"]
+    k = KATS["singular_solve"]
+    with pytest.raises(oracle.OracleError):
+        oracle.solve(np.array(k["a"]), np.array(k["b"]))
+
+
+@pytest.mark.parametrize("n", [5, 8, 16, 64, 512])
+def test_solve_matches_lapack(n):
+    d = _lapack(n)
+    x = oracle.solve(d["a"], d["b"])
+    cond = np.linalg.cond(d["a"], np.inf)
+    assert np.abs(x - d["x"]).max() / np.abs(d["x"]).max() < 10 * n * EPS64 * cond
+    xm = oracle.solve_multiple(d["a"], d["bm"])
+    assert np.abs(xm - d["xm"]).max() / np.abs(d["xm"]).max() < 10 * n * EPS64 * cond
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
+def test_solve_exact_rational(n):
+    rng = np.random.default_rng(100 + n)
+    a = rng.integers(-5, 6, size=(n, n)).astype(np.float64) + 7 * np.eye(n)
+    xt = rng.integers(-3, 4, size=n).astype(np.float64)
+    b = a @ xt  # exact in binary64 (small integers)
+    x = oracle.solve(a, b)
+    assert np.allclose(x, xt, rtol=0, atol=1e-13)
+
+
+def test_solve_f32():
+    d = _lapack(64)
+    x = oracle.solve(d["a"].astype(np.float32), d["b"].astype(np.float32))
+    assert x.dtype == np.float32
+    assert np.abs(x - d["x"]).max() / np.abs(d["x"]).max() < 1e-3
+
+
+# --------------------------------------------- planted-mistake mutations ---
+def _py_lu_factor(a, mutation=None):
+    """Pure-Python elimination used only to plant mistakes in."""
+    lu = np.array(a, dtype=np.float64).copy()
+    n, m = lu.shape
+    piv = np.arange(n)
+    for k in range(min(n, m)):
+        col = np.abs(lu[k:, k])
+        if mutation == "last_max_tie":
+            p = k + len(col) - 1 - int(np.argmax(col[::-1]))
+        else:
+            p = k + int(np.argmax(col))
+        if p != k:
+            lu[[k, p]] = lu[[p, k]]
+            piv[[k, p]] = piv[[p, k]]
+        for i in range(k + 1, n):
+            lu[i, k] = lu[i, k] / lu[k, k]
+            for j in range(k + 1, m):
+                if mutation == "sign":
+                    lu[i, j] = lu[i, j] + lu[i, k] * lu[k, j]
+                elif mutation == "index":
+                    lu[i, j] = lu[i, j] - lu[i, k] * lu[k, min(j + 1, m - 1)]
+                elif mutation == "transpose":
+                    lu[i, j] = lu[i, j] - lu[i, k] * lu[j, k]
+                else:
+                    lu[i, j] = lu[i, j] - lu[i, k] * lu[k, j]
+    return lu, piv, 1.0
+
+
+@pytest.mark.parametrize("mutation", ["last_max_tie", "sign", "index", "transpose"])
+def test_checks_reject_planted_mistakes(mutation):
+    def mutant(a):
+        return _py_lu_factor(a, mutation)
+
+    checks = [
+        check_lu_kat,
+        lambda f: check_lu_lapack(f, 16),
+        lambda f: check_lu_invariants(f, _lapack(16)["a"]),
+        lambda f: check_lu_exact(f, 0),
+        check_lu_tie_rule,
+    ]
+    caught = 0
+    for chk in checks:
+        try:
+            chk(mutant)
+        except AssertionError:
+            caught += 1
+    assert caught >= 1, f"mutation {mutation} passed every check"
+
+
+def test_unmutated_python_copy_passes():
+    f = lambda a: _py_lu_factor(a)  # noqa: E731
+    check_lu_kat(f)
+    check_lu_lapack(f, 16)
+    check_lu_tie_rule(f)
