@@ -131,6 +131,21 @@ int scirs_gpu_dgesv_ir_dev(int n, int nrhs, const double *dA, int64_t lda, const
                            void *stream, scirs_solve_info *info);
 
 /* ---- diagnostics -------------------------------------------------------------- */
+/* Kernel-class timing with HIP events on the launching stream (no reference
+ * counterpart; the reference's profiling layer, scirs2-core/src/profiling.rs,
+ * is not wired into linalg).  Classes, in array order: 0 panel, 1 row
+ * interchanges, 2 TRSM, 3 trailing update, 4 substitution, 5 residual,
+ * 6 conversion, 7 other.  scirs_gpu_prof_end fills SCIRS_PROF_NCAT records. */
+#define SCIRS_PROF_NCAT 8
+typedef struct scirs_prof_stats {
+    int64_t launches;    /* timed scopes (one scope may cover several launches) */
+    double total_ms;     /* summed event time                                    */
+    double flops;        /* algorithmic floating-point operations                */
+    double bytes;        /* algorithmic HBM bytes                                */
+} scirs_prof_stats;
+int scirs_gpu_prof_begin(void);
+int scirs_gpu_prof_end(scirs_prof_stats *out);
+
 /* Normwise backward error of X for A X = B, all in HBM (FP64 residual on the
  * GPU).  No reference counterpart: the reference's tests compute residuals
  * with ndarray `dot` (tests/scipy_compat_validation.rs:455-476). */

@@ -42,6 +42,17 @@ class SolveInfo(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ProfStats(ctypes.Structure):
+    _fields_ = [
+        ("launches", ctypes.c_int64),
+        ("total_ms", ctypes.c_double),
+        ("flops", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+    ]
+
+
+PROF_CLASSES = ["panel", "laswp", "trsm", "update", "trsv", "residual", "convert", "other"]
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _L = ctypes.c_int64
@@ -63,6 +74,8 @@ SIGNATURES = {
     "scirs_gpu_dgesv_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P, _P, _INFO], _I),
     "scirs_gpu_dposv_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P, _INFO], _I),
     "scirs_gpu_dgesv_ir_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _I, _D, _P, _INFO], _I),
+    "scirs_gpu_prof_begin": ([], _I),
+    "scirs_gpu_prof_end": ([ctypes.POINTER(ProfStats)], _I),
     "scirs_gpu_backward_error_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, ctypes.POINTER(ctypes.c_double)], _I),
 }
 

@@ -77,7 +77,8 @@ int Context::ensure(void **p, size_t *have, size_t bytes)
 {
     if (*have >= bytes && *p) return SCIRS_OK;
     if (*p) {
-        SCIRS_HIP_CHECK(hipStreamSynchronize(stream));
+        // the old buffer may still be in use on a caller's stream
+        SCIRS_HIP_CHECK(hipDeviceSynchronize());
         SCIRS_HIP_CHECK(hipFree(*p));
         *p = nullptr;
         *have = 0;
@@ -122,33 +123,28 @@ int Context::init(int dev)
     work.max_panels = 1 << 14;   // n up to 16384 * kb
     work.spin_limit = (unsigned)env_int("SCIRS_GPU_SPIN_LIMIT", 1 << 22);
     work.lowp_mode = SCIRS_LOWP_FP32;   // plain FP32 entry points compute in FP32; the mixed solve switches it
-    work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 64);
+    work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 128);
     size_t bytes = 0;
-    bytes += 2 * (size_t)max_g * sizeof(double);          // cval
-    bytes += 2 * (size_t)max_g * sizeof(int);             // cidx
-    bytes += 2 * (size_t)max_g * kbmax * sizeof(double);  // crow
-    bytes += 2 * (size_t)kbmax * sizeof(double);          // rowj
-    bytes += 64 * sizeof(int);                            // bar, done, abort, info, info_val (+pad)
-    bytes += (size_t)work.max_panels * 4 * kbmax * sizeof(int);
-    bytes += (size_t)work.max_panels * sizeof(int);
-    bytes = (bytes + 255) & ~(size_t)255;
+    const size_t xw_b = 2 * (size_t)max_g * 4 * 8, xrow_b = 2 * (size_t)max_g * kbmax * 2 * 8,
+                 xrj_b = 2 * (size_t)kbmax * 2 * 8, flags_b = 64 * sizeof(int),
+                 pairs_b = (size_t)work.max_panels * 4 * kbmax * sizeof(int),
+                 npairs_b = (size_t)work.max_panels * sizeof(int);
+    for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b}) bytes += (b + 255) & ~(size_t)255;
     char *base = nullptr;
     SCIRS_HIP_CHECK(hipMalloc((void **)&base, bytes));
     SCIRS_HIP_CHECK(hipMemset(base, 0, bytes));
     char *p = base;
     auto take = [&](size_t n) { char *r = p; p += (n + 255) & ~(size_t)255; return r; };
-    work.cval = take(2 * (size_t)max_g * sizeof(double));
-    work.cidx = (int *)take(2 * (size_t)max_g * sizeof(int));
-    work.crow = take(2 * (size_t)max_g * kbmax * sizeof(double));
-    work.rowj = take(2 * (size_t)kbmax * sizeof(double));
-    char *flags = take(64 * sizeof(int));
-    work.bar = (unsigned *)flags;
-    work.done = (unsigned *)(flags + 4);
+    work.xw = (unsigned long long *)take(xw_b);
+    work.xrow = (unsigned long long *)take(xrow_b);
+    work.xrj = (unsigned long long *)take(xrj_b);
+    char *flags = take(flags_b);
     work.abort = (int *)(flags + 8);
     work.info = (int *)(flags + 12);
     work.info_val = (double *)(flags + 16);
-    work.pairs = (int *)take((size_t)work.max_panels * 4 * kbmax * sizeof(int));
-    work.npairs = (int *)take((size_t)work.max_panels * sizeof(int));
+    work.pairs = (int *)take(pairs_b);
+    work.npairs = (int *)take(npairs_b);
+    work.seq_next = 0;
     work_base = base;
     SCIRS_HIP_CHECK(hipHostMalloc((void **)&host_flags, 64, hipHostMallocDefault));
     kb_f64 = env_int("SCIRS_GPU_KB", 128);
@@ -175,10 +171,7 @@ int Context::read_flags(hipStream_t s, int *info, double *info_val)
     memcpy(&v, (char *)host_flags + 8, 8);
     if (info_val) *info_val = v;
     if (abort_flag) {
-        // clear barrier state for the next call
-        (void)hipMemsetAsync(work.bar, 0, 12, s);
-        (void)hipStreamSynchronize(s);
-        scirs_set_error("a cooperative panel kernel timed out at its grid barrier");
+        scirs_set_error("a cooperative panel kernel timed out waiting for its exchange");
         return SCIRS_ERR_TIMEOUT;
     }
     return SCIRS_OK;
@@ -265,6 +258,20 @@ extern "C" int scirs_gpu_init(int device)
 }
 
 // ------------------------------------------------------------------ LU core
+// X <- U^-1 L^-1 X (X already row-permuted): 256-row block substitution, or the
+// 64-row launch-per-block solver when SCIRS_GPU_TRSV=blocked.
+template <typename T>
+static int lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s)
+{
+    Context &c = ctx();
+    const char *mode = getenv("SCIRS_GPU_TRSV");
+    if (mode && strcmp(mode, "blocked") == 0)
+        return scirs_getrs_device<T>(n, nrhs, dLU, ldlu, dX, ldx, c.work.info, s);
+    int rc = scirs_trsv_big<T>(n, nrhs, dLU, ldlu, dX, ldx, 1, 1, c.work.info, s);
+    if (rc) return rc;
+    return scirs_trsv_big<T>(n, nrhs, dLU, ldlu, dX, ldx, 0, 0, c.work.info, s);
+}
+
 template <typename T>
 static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm, const T *dB, long ldb, T *dX,
                               long ldx, hipStream_t s, scirs_solve_info *info)
@@ -283,7 +290,7 @@ static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm
     SCIRS_HIP_CHECK(hipEventRecord(e1, s));
     if (nrhs > 0) {
         if ((rc = scirs_permute_rows<T>(n, nrhs, dPerm, dB, ldb, dX, ldx, s))) return rc;
-        if ((rc = scirs_getrs_device<T>(n, nrhs, dLU, ldlu, dX, ldx, c.work.info, s))) return rc;
+        if ((rc = lu_substitute<T>(n, nrhs, dLU, ldlu, dX, ldx, s))) return rc;
     }
     SCIRS_HIP_CHECK(hipEventRecord(e2, s));
     int st = 0;

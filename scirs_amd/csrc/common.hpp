@@ -5,18 +5,9 @@
 #include <stdint.h>
 #include <float.h>
 
-#define SCIRS_WAVE 64
+#include "../../include/scirs_gpu.h"   // status codes SCIRS_OK, SCIRS_ERR_*
 
-// Status codes shared by every entry point (mirrored in include/scirs_gpu.h).
-enum {
-    SCIRS_OK = 0,
-    SCIRS_ERR_ARG = -1,
-    SCIRS_ERR_HIP = -2,
-    SCIRS_ERR_NOMEM = -3,
-    SCIRS_ERR_TIMEOUT = -4,
-    SCIRS_ERR_NOT_CONVERGED = -5,
-    SCIRS_ERR_UNSUPPORTED = -6,
-};
+#define SCIRS_WAVE 64
 
 template <typename T> struct num_limits;
 template <> struct num_limits<double> { static __host__ __device__ constexpr double eps() { return DBL_EPSILON; } };
@@ -53,37 +44,32 @@ __device__ __forceinline__ T wave_max(T v)
     return v;
 }
 
-// Grid-wide barrier for cooperative launches.  `bar` is a counter that every
-// workgroup increments once per barrier; barrier number t (1-based) completes
-// when the counter reaches t * nblocks.  The wait is bounded: after
-// `spin_limit` polls the workgroup raises *abort and leaves, so a kernel can
-// never wait forever.  All threads fence at agent scope on both sides so that
-// global writes made before the barrier by any thread of any workgroup are
-// visible to every thread after it.
-__device__ __forceinline__ bool grid_barrier(unsigned *bar, unsigned target, int *abort,
-                                             unsigned spin_limit)
+// Copy an nrows x ncols block of a row-major global matrix into LDS (row pitch
+// `pitch`), with every thread issuing UNROLL loads before it stores any, so the
+// copy pays the memory latency once per UNROLL elements instead of once per
+// element.  Elements outside [0, vrows) x [0, vcols) of the source are
+// written as zero.  Call from all threads of the workgroup.
+template <typename T, int UNROLL = 8>
+__device__ __forceinline__ void g2s_block(T *lds, int pitch, const T *g, long ld, int nrows, int ncols, int vrows,
+                                          int vcols)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    __shared__ int s_ok;
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
-            if (++spins > spin_limit) {
-                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+    const int total = nrows * ncols;
+    const int nt = blockDim.x;
+    for (int base = threadIdx.x; base < total; base += UNROLL * nt) {
+        T v[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int idx = base + u * nt;
+            const int r = idx / ncols, c = idx - r * ncols;
+            v[u] = (idx < total && r < vrows && c < vcols) ? g[(long)r * ld + c] : (T)0;
         }
-        s_ok = ok;
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int idx = base + u * nt;
+            const int r = idx / ncols, c = idx - r * ncols;
+            if (idx < total) lds[r * pitch + c] = v[u];
+        }
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return s_ok != 0;
 }
 
 // Host-side error helper.

@@ -54,7 +54,7 @@ struct LowpArgs {
 };
 
 template <int MODE, bool TRANS_B>
-__global__ __launch_bounds__(NTHREADS) void lowp_gemm_kernel(LowpArgs<MODE> g)
+__global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g)
 {
     if (g.info && *g.info != 0) return;
     int tm, tn;
@@ -157,6 +157,22 @@ __global__ __launch_bounds__(NTHREADS) void lowp_gemm_kernel(LowpArgs<MODE> g)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
+    // C is read once: issue its loads now so their latency overlaps the K loop.
+    // Lane l, register q of a 16x16 block holds row 4*(l/16)+q, column l%16.
+    const int lq = (lane >> 4) * 4;
+    f4 cpre[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = col0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = row0 + wm * 64 + i * 16 + lq + q;
+                cpre[i][j][q] = (r < g.M && c < g.N) ? g.C[(long)r * g.ldc + c] : 0.0f;
+            }
+        }
+
     const int nslab = (g.K + BK - 1) / BK;
     load_slab(0);
     store_slab(0);
@@ -201,7 +217,6 @@ __global__ __launch_bounds__(NTHREADS) void lowp_gemm_kernel(LowpArgs<MODE> g)
         __syncthreads();
     }
 
-    const int lq = (lane >> 4) * 4;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int c = col0 + wn * 64 + j * 16 + lr;
@@ -213,10 +228,7 @@ __global__ __launch_bounds__(NTHREADS) void lowp_gemm_kernel(LowpArgs<MODE> g)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = row0 + wm * 64 + i * 16 + lq + q;
-                if (r < g.M) {
-                    float *p = g.C + (long)r * g.ldc + c;
-                    *p = *p - acc[i][j][q] * sc;
-                }
+                if (r < g.M) g.C[(long)r * g.ldc + c] = cpre[i][j][q] - acc[i][j][q] * sc;
             }
         }
     }

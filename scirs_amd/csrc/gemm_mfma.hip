@@ -50,7 +50,7 @@ struct GemmArgs {
 };
 
 template <typename T, bool TRANS_B>
-__global__ __launch_bounds__(NTHREADS) void gemm_sub_kernel(GemmArgs<T> g)
+__global__ __launch_bounds__(NTHREADS, 2) void gemm_sub_kernel(GemmArgs<T> g)
 {
     if (g.info && *g.info != 0) return;
     int tm, tn;

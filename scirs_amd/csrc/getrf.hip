@@ -22,227 +22,14 @@
 // is what the normwise backward-error parity test allows.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "panel.hpp"
+#include "prof.hpp"
 #include <algorithm>
 
 namespace {
 
 constexpr int NT = 256;  // threads per workgroup everywhere in this file
 
-template <typename T>
-struct PanelArgs {
-    T *A; long lda;
-    int k0, m, kb, R, G;
-    int64_t *ipiv_unused;
-    T *cval; int *cidx; T *crow; T *rowj;
-    unsigned *bar; unsigned *done; int *abort;
-    int *pairs; int *npairs;
-    int *info; double *info_val;
-    unsigned spin_limit;
-};
-
-template <typename T>
-__global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
-{
-    if (*a.info != 0) return;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int kb = a.kb;
-    const int kbp = kb + 1;                       // LDS pitch (breaks bank aliasing)
-    T *tile = reinterpret_cast<T *>(smem_raw);    // R x kbp
-    T *prow = tile + (size_t)a.R * kbp;           // kb
-    T *lcol = prow + kb;                          // R multipliers of the current column
-    int *top_label = reinterpret_cast<int *>(lcol + a.R);  // kb   (workgroup 0 only)
-    int *out_pos = top_label + kb;                // kb
-    int *out_label = out_pos + kb;                // kb
-    __shared__ T red_v[NT / 64];
-    __shared__ int red_i[NT / 64];
-    __shared__ int s_p, s_g, s_nout, s_stop;
-    __shared__ T s_pv;
-
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int g = blockIdx.x;
-    const int row0 = g * a.R;                     // first panel-relative row of this chunk
-    const int Rg = max(0, min(a.R, a.m - row0));  // rows held here
-    T *Ap = a.A + (long)a.k0 * a.lda + a.k0;      // panel origin
-
-    for (int r = wave; r < Rg; r += NT / 64)
-        for (int c = lane; c < kb; c += 64)
-            tile[r * kbp + c] = Ap[(long)(row0 + r) * a.lda + c];
-    if (g == 0) {
-        for (int q = t; q < kb; q += NT) top_label[q] = q;
-        if (t == 0) s_nout = 0;
-    }
-    if (t == 0) s_stop = 0;
-    __syncthreads();
-
-    const T eps = num_limits<T>::eps();
-    int buf = 0;
-    for (int j = 0; j < kb; ++j) {
-        // ---- 1. local arg-max over rows with panel index >= j
-        const int rstart = max(0, j - row0);
-        T bv = (T)0; int bi = -1;
-        for (int r = rstart + t; r < Rg; r += NT) {
-            T v = fabs(tile[r * kbp + j]);
-            int i = row0 + r;
-            if (bi < 0 || v > bv) { bv = v; bi = i; }   // rows visited in increasing order
-        }
-        wave_argmax(bv, bi);
-        if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
-        __syncthreads();
-        if (t == 0) {
-            T v = red_v[0]; int i = red_i[0];
-            for (int w = 1; w < NT / 64; ++w) {
-                int oi = red_i[w]; T ov = red_v[w];
-                if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) { v = ov; i = oi; }
-            }
-            s_p = i; s_pv = v;
-        }
-        __syncthreads();
-        // ---- 2. publish the candidate row, and row j when this chunk holds it
-        T *cv = a.cval + buf * a.G;
-        int *ci = a.cidx + buf * a.G;
-        T *cr = a.crow + ((size_t)buf * a.G + g) * kb;
-        T *rj = a.rowj + (size_t)buf * kb;
-        const int my_best = s_p;
-        if (t == 0) { cv[g] = s_pv; ci[g] = my_best; }
-        if (my_best >= 0)
-            for (int c = t; c < kb; c += NT) cr[c] = tile[(my_best - row0) * kbp + c];
-        if (j >= row0 && j < row0 + Rg)
-            for (int c = t; c < kb; c += NT) rj[c] = tile[(j - row0) * kbp + c];
-        // ---- 3. one grid barrier per column
-        if (!grid_barrier(a.bar, (unsigned)(j + 1) * a.G, a.abort, a.spin_limit)) break;
-        // ---- 4. every workgroup reduces the same candidate list
-        {
-            T v = (T)0; int i = -1, wg = -1;
-            for (int q = t; q < a.G; q += NT) {
-                int qi = ci[q]; T qv = cv[q];
-                if (qi >= 0 && (i < 0 || qv > v || (qv == v && qi < i))) { v = qv; i = qi; wg = q; }
-            }
-            // arg-max carrying the owning workgroup: pack (row, wg) — rows are unique
-            T vv = v; int ii = i;
-            wave_argmax(vv, ii);
-            // recover wg of the winning row: the lane that owned it broadcasts
-            unsigned long long ball = __ballot(i == ii && i >= 0);
-            int src = ball ? __ffsll(ball) - 1 : 0;
-            int wgw = __shfl(wg, src, 64);
-            if (lane == 0) { red_v[wave] = vv; red_i[wave] = ii; }
-            __shared__ int red_g[NT / 64];
-            if (lane == 0) red_g[wave] = wgw;
-            __syncthreads();
-            if (t == 0) {
-                T bv2 = red_v[0]; int bi2 = red_i[0], bg2 = red_g[0];
-                for (int w = 1; w < NT / 64; ++w) {
-                    int oi = red_i[w]; T ov = red_v[w];
-                    if (oi >= 0 && (bi2 < 0 || ov > bv2 || (ov == bv2 && oi < bi2))) {
-                        bv2 = ov; bi2 = oi; bg2 = red_g[w];
-                    }
-                }
-                s_p = bi2; s_g = bg2; s_pv = bv2;
-                if (!(bv2 >= eps)) {                 // |pivot| < eps: singular (lapack.rs:106-120)
-                    s_stop = 1;
-                    if (g == 0) {
-                        *a.info = a.k0 + j + 1;
-                        *a.info_val = (double)bv2;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        if (s_stop) break;
-        const int p = s_p;
-        const T *wrow = a.crow + ((size_t)buf * a.G + s_g) * kb;
-        for (int c = t; c < kb; c += NT) prow[c] = wrow[c];
-        __syncthreads();
-        // ---- 5. row interchange inside the panel
-        if (p != j) {
-            if (j >= row0 && j < row0 + Rg)
-                for (int c = t; c < kb; c += NT) tile[(j - row0) * kbp + c] = prow[c];
-            if (p >= row0 && p < row0 + Rg)
-                for (int c = t; c < kb; c += NT) tile[(p - row0) * kbp + c] = rj[c];
-        }
-        if (g == 0 && wave == 0) {
-            // bookkeeping of the net permutation (panel-relative positions)
-            if (p != j) {
-                if (p < kb) {
-                    if (lane == 0) {
-                        int tmp = top_label[j]; top_label[j] = top_label[p]; top_label[p] = tmp;
-                    }
-                } else {
-                    int nout = s_nout, found = -1;
-                    for (int base = 0; base < nout && found < 0; base += 64) {
-                        int s = base + lane;
-                        unsigned long long b = __ballot(s < nout && out_pos[s] == p);
-                        if (b) found = base + __ffsll(b) - 1;
-                    }
-                    if (lane == 0) {
-                        if (found < 0) {
-                            found = nout;
-                            out_pos[found] = p;
-                            out_label[found] = p;
-                            s_nout = nout + 1;
-                        }
-                        int tmp = top_label[j]; top_label[j] = out_label[found]; out_label[found] = tmp;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // ---- 6. multipliers (true division, lapack.rs:138) and rank-1 update
-        const T pv = prow[j];
-        const int r1 = max(0, j + 1 - row0);
-        for (int r = r1 + t; r < Rg; r += NT) {
-            T l = tile[r * kbp + j] / pv;
-            tile[r * kbp + j] = l;
-            lcol[r] = l;
-        }
-        __syncthreads();
-        const int ncol = kb - j - 1;
-        if (ncol > 0 && Rg > r1) {
-            // 2-D split: consecutive threads walk consecutive columns of one row
-            const int nrow = Rg - r1;
-            const long total = (long)nrow * ncol;
-            for (long w = t; w < total; w += NT) {
-                int r = r1 + (int)(w / ncol);
-                int c = j + 1 + (int)(w % ncol);
-                tile[r * kbp + c] -= lcol[r] * prow[c];
-            }
-        }
-        __syncthreads();
-        buf ^= 1;
-    }
-
-    // write the factored panel back
-    for (int r = wave; r < Rg; r += NT / 64)
-        for (int c = lane; c < kb; c += 64)
-            Ap[(long)(row0 + r) * a.lda + c] = tile[r * kbp + c];
-
-    if (g == 0) {
-        // gather list (dst <- src) of the net permutation, absolute rows
-        __syncthreads();
-        if (t == 0) {
-            int np = 0;
-            for (int q = 0; q < kb; ++q)
-                if (top_label[q] != q) {
-                    a.pairs[2 * np] = a.k0 + q; a.pairs[2 * np + 1] = a.k0 + top_label[q]; ++np;
-                }
-            for (int s = 0; s < s_nout; ++s)
-                if (out_label[s] != out_pos[s]) {
-                    a.pairs[2 * np] = a.k0 + out_pos[s]; a.pairs[2 * np + 1] = a.k0 + out_label[s]; ++np;
-                }
-            *a.npairs = np;
-        }
-    }
-
-    // the last workgroup out resets the barrier state for the next launch
-    __syncthreads();
-    if (t == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        unsigned prev = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == (unsigned)a.G - 1) {
-            __hip_atomic_store(a.bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(a.done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
 
 // Apply a gather list to the columns outside [k0, k0+kb): each workgroup owns 64
 // columns, loads every source row of its columns first, then stores.
@@ -260,8 +47,20 @@ __global__ __launch_bounds__(NT) void laswp_kernel(T *A, long lda, int ncols_tot
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *vals = reinterpret_cast<T *>(smem_raw);   // np x 64
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int q = wave; q < np; q += NT / 64)
-        if (ok) vals[q * 64 + lane] = A[(long)pairs[2 * q + 1] * lda + c];
+    constexpr int U = 8;   // loads in flight per thread before the LDS stores
+    for (int q0 = wave; q0 < np; q0 += U * (NT / 64)) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u * (NT / 64);
+            v[u] = (ok && q < np) ? A[(long)pairs[2 * q + 1] * lda + c] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u * (NT / 64);
+            if (q < np) vals[q * 64 + lane] = v[u];
+        }
+    }
     __syncthreads();
     for (int q = wave; q < np; q += NT / 64)
         if (ok) A[(long)pairs[2 * q] * lda + c] = vals[q * 64 + lane];
@@ -283,12 +82,29 @@ __global__ __launch_bounds__(NT) void trsm_llnu_kernel(T *A, long lda, int k0, i
     const int c = cbase + lane;
     const bool ok = c < ncols_total;
     const T *L = A + (long)k0 * lda + k0;
-    for (int i = 1 + rg; i < kb; i += NT / 64) {
-        T *dst = Ls + (size_t)i * (i - 1) / 2;
-        for (int j = lane; j < i; j += 64) dst[j] = L[(long)i * lda + j];
+    // strict lower triangle of L11, packed by rows (row i at i*(i-1)/2), with
+    // 8 loads in flight per thread
+    {
+        const int total = kb * (kb - 1) / 2;
+        constexpr int U = 8;
+        for (int b = threadIdx.x; b < total; b += U * NT) {
+            T v[U]; int ii[U], jj[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = b + u * NT;
+                // invert idx = i*(i-1)/2 + j, 0 <= j < i
+                int i = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)idx)) * 0.5f);
+                while (i * (i - 1) / 2 > idx) --i;
+                while ((i + 1) * i / 2 <= idx) ++i;
+                ii[u] = i; jj[u] = idx - i * (i - 1) / 2;
+                v[u] = idx < total ? L[(long)i * lda + jj[u]] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (b + u * NT < total) Ls[(size_t)ii[u] * (ii[u] - 1) / 2 + jj[u]] = v[u];
+        }
     }
-    for (int i = rg; i < kb; i += NT / 64)
-        Bs[i * 64 + lane] = ok ? A[(long)(k0 + i) * lda + c] : (T)0;
+    g2s_block<T>(Bs, 64, A + (long)k0 * lda + cbase, lda, kb, 64, kb, ncols_total - cbase);
     __syncthreads();
     for (int ib = 0; ib < kb; ib += 8) {
         const int ie = min(ib + 8, kb);
@@ -318,20 +134,6 @@ __global__ __launch_bounds__(NT) void trsm_llnu_kernel(T *A, long lda, int k0, i
         for (int i = rg; i < kb; i += NT / 64) A[(long)(k0 + i) * lda + c] = Bs[i * 64 + lane];
 }
 
-size_t panel_lds_bytes(int R, int kb, size_t esz)
-{
-    return ((size_t)R * (kb + 1) + kb + R) * esz + 3 * (size_t)kb * sizeof(int);
-}
-
-template <typename T>
-int max_lds_rows(int kb)
-{
-    const size_t budget = 160 * 1024 - 2048;   // leave room for the static reduction buffers
-    size_t per_row = (size_t)(kb + 2) * sizeof(T);
-    size_t fixed = (size_t)kb * sizeof(T) + 3 * (size_t)kb * sizeof(int);
-    return (int)((budget - fixed) / per_row);
-}
-
 } // namespace
 
 template <typename T>
@@ -345,40 +147,46 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         scirs_set_error("getrf: %d panels exceed the workspace (%d)", npanels, w.max_panels);
         return SCIRS_ERR_ARG;
     }
-    const int rmax = max_lds_rows<T>(kb);
+    const int rmax = panel::max_rows<T>(kb);
     for (int p = 0; p < npanels; ++p) {
         const int k0 = p * kb;
         const int kbw = std::min(kb, kmax - k0);
         const int mrows = m - k0;
-        int G = (mrows + rmax - 1) / rmax;
-        // spread rows over more workgroups when there is room: fewer rows per
-        // workgroup shortens every column step
-        const int target = std::max(8, w.panel_rows);
-        G = std::max(G, std::min(w.max_g, (mrows + target - 1) / target));
-        G = std::min(G, w.max_g);
-        const int R = (mrows + G - 1) / G;
+        // rows per workgroup: as many as the target allows, never more than LDS holds
+        const int target = std::max(8, std::min(rmax, w.panel_rows));
+        int G = std::min(w.max_g, (mrows + target - 1) / target);
+        G = std::max(G, 1);
+        int R = (mrows + G - 1) / G;
         if (R > rmax) {
-            scirs_set_error("getrf: panel of %d rows does not fit %d co-resident workgroups", mrows, w.max_g);
-            return SCIRS_ERR_UNSUPPORTED;
+            G = (mrows + rmax - 1) / rmax;
+            if (G > w.max_g) {
+                scirs_set_error("getrf: panel of %d rows does not fit %d co-resident workgroups", mrows, w.max_g);
+                return SCIRS_ERR_UNSUPPORTED;
+            }
+            R = (mrows + G - 1) / G;
         }
         G = (mrows + R - 1) / R;
-        PanelArgs<T> pa;
+        panel::PanelArgs<T> pa;
         pa.A = A; pa.lda = lda; pa.k0 = k0; pa.m = mrows; pa.kb = kbw; pa.R = R; pa.G = G;
-        pa.ipiv_unused = nullptr;
-        pa.cval = (T *)w.cval; pa.cidx = w.cidx; pa.crow = (T *)w.crow; pa.rowj = (T *)w.rowj;
-        pa.bar = w.bar; pa.done = w.done; pa.abort = w.abort;
+        pa.xw = w.xw; pa.xrow = w.xrow; pa.xrj = w.xrj;
+        pa.seq_base = w.seq_next;
+        w.seq_next += 512u;
         pa.pairs = w.pairs + (size_t)p * 4 * SCIRS_KB_MAX; pa.npairs = w.npairs + p;
-        pa.info = w.info; pa.info_val = w.info_val; pa.spin_limit = w.spin_limit;
+        pa.info = w.info; pa.info_val = w.info_val; pa.abort = w.abort; pa.spin_limit = w.spin_limit;
         void *args[] = {&pa};
-        const size_t lds = panel_lds_bytes(R, kbw, sizeof(T));
-        if (int rc = scirs_allow_lds((const void *)getf2_panel_kernel<T>, lds)) return rc;
-        SCIRS_HIP_CHECK(hipLaunchCooperativeKernel((const void *)getf2_panel_kernel<T>, dim3(G), dim3(NT),
-                                                   args, lds, stream));
+        const size_t lds = panel::lds_bytes(R, kbw, sizeof(T));
+        if (int rc = scirs_allow_lds((const void *)panel::getf2_panel_kernel<T>, lds)) return rc;
+        {
+            scirs::ProfScope ps(SCIRS_PROF_PANEL, (double)mrows * kbw * kbw, 2.0 * mrows * kbw * sizeof(T), stream);
+            SCIRS_HIP_CHECK(hipLaunchCooperativeKernel((const void *)panel::getf2_panel_kernel<T>, dim3(G),
+                                                       dim3(panel::NT), args, lds, stream));
+        }
         // row interchanges outside the panel
         const int nout_cols = n - kbw;
         if (nout_cols > 0) {
             const size_t lds_sw = (size_t)2 * kbw * 64 * sizeof(T);
             if (int rc = scirs_allow_lds((const void *)laswp_kernel<T>, lds_sw)) return rc;
+            scirs::ProfScope ps(SCIRS_PROF_LASWP, 0.0, 4.0 * kbw * nout_cols * sizeof(T), stream);
             hipLaunchKernelGGL(laswp_kernel<T>, dim3((nout_cols + 63) / 64), dim3(NT), lds_sw, stream,
                                A, lda, n, k0, kbw, pa.pairs, pa.npairs, (const int *)w.info);
             SCIRS_HIP_CHECK(hipGetLastError());
@@ -387,11 +195,18 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         if (nright > 0) {
             const size_t lds_t = ((size_t)kbw * (kbw - 1) / 2 + (size_t)kbw * 64) * sizeof(T);
             if (int rc = scirs_allow_lds((const void *)trsm_llnu_kernel<T>, lds_t)) return rc;
-            hipLaunchKernelGGL(trsm_llnu_kernel<T>, dim3((nright + 63) / 64), dim3(NT), lds_t, stream,
-                               A, lda, k0, kbw, n, (const int *)w.info);
-            SCIRS_HIP_CHECK(hipGetLastError());
+            {
+                scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * nright,
+                                    (0.5 * kbw * kbw + 2.0 * kbw * nright) * sizeof(T), stream);
+                hipLaunchKernelGGL(trsm_llnu_kernel<T>, dim3((nright + 63) / 64), dim3(NT), lds_t, stream,
+                                   A, lda, k0, kbw, n, (const int *)w.info);
+                SCIRS_HIP_CHECK(hipGetLastError());
+            }
             const int mbelow = m - k0 - kbw;
             if (mbelow > 0) {
+                scirs::ProfScope ps(SCIRS_PROF_UPDATE, 2.0 * mbelow * nright * kbw,
+                                    (2.0 * mbelow * nright + (double)mbelow * kbw + (double)kbw * nright) * sizeof(T),
+                                    stream);
                 int rc = scirs_trailing_update<T>(mbelow, nright, kbw,
                                                   A + (long)(k0 + kbw) * lda + k0, lda,
                                                   A + (long)k0 * lda + k0 + kbw, lda,

@@ -156,6 +156,15 @@ struct IrState {
     int kb;
 };
 
+// d32 <- U^-1 L^-1 d32 with the FP32 factors
+int lu_solve_f32(IrState &st, float *d32)
+{
+    Context &c = ctx();
+    int rc = scirs_trsv_big<float>(st.n, 1, st.LU, st.n, d32, 1, 1, 1, c.work.info, st.s);
+    if (rc) return rc;
+    return scirs_trsv_big<float>(st.n, 1, st.LU, st.n, d32, 1, 0, 0, c.work.info, st.s);
+}
+
 // apply z = (LU)^-1 P v in FP32 for one vector v (FP64), result FP64; scale keeps FP32 in range
 int precond_apply(IrState &st, const double *v, double vnorm, double *z)
 {
@@ -163,7 +172,7 @@ int precond_apply(IrState &st, const double *v, double vnorm, double *z)
     const double sc = pow2_above(vnorm);
     hipLaunchKernelGGL(permute_to_f32_kernel, dim3(nb(st.n)), dim3(256), 0, st.s, st.n, st.perm, v, 1L, 0, 1.0 / sc,
                        st.d32);
-    int rc = scirs_getrs_device<float>(st.n, 1, st.LU, st.n, st.d32, 1, c.work.info, st.s);
+    int rc = lu_solve_f32(st, st.d32);
     if (rc) return rc;
     hipLaunchKernelGGL(f32_to_f64_scaled_kernel, dim3(nb(st.n)), dim3(256), 0, st.s, st.n, st.d32, sc, z);
     SCIRS_HIP_CHECK(hipGetLastError());
@@ -197,7 +206,7 @@ int ir_step(IrState &st, int j, double rn)
     const double sc = pow2_above(rn);
     hipLaunchKernelGGL(permute_to_f32_kernel, dim3(nb(st.n)), dim3(256), 0, st.s, st.n, st.perm, st.R, (long)st.nrhs, j,
                        1.0 / sc, st.d32);
-    int rc = scirs_getrs_device<float>(st.n, 1, st.LU, st.n, st.d32, 1, c.work.info, st.s);
+    int rc = lu_solve_f32(st, st.d32);
     if (rc) return rc;
     hipLaunchKernelGGL(axpy_f32_kernel, dim3(nb(st.n)), dim3(256), 0, st.s, st.n, st.d32, sc, st.X, st.ldx, j, 0);
     SCIRS_HIP_CHECK(hipGetLastError());

@@ -11,13 +11,11 @@ int scirs_launch_dgemm_sub(int M, int N, int K, const double *A, long lda, const
 
 // Device scratch used by the factorisations (allocated once per context).
 struct ScirsWork {
-    void *cval;          // [2][MAX_G] candidate |max| (element type of the panel)
-    int *cidx;           // [2][MAX_G] candidate row (panel-relative), -1 = none
-    void *crow;          // [2][MAX_G][KB_MAX] candidate rows
-    void *rowj;          // [2][KB_MAX] current diagonal row
-    unsigned *bar;       // grid-barrier counter
-    unsigned *done;      // workgroups finished (resets bar)
-    int *abort;          // raised by a timed-out barrier
+    unsigned long long *xw;    // [2][MAX_G][4]        panel candidates (sequence-tagged words)
+    unsigned long long *xrow;  // [2][MAX_G][KB_MAX*2] panel candidate rows
+    unsigned long long *xrj;   // [2][KB_MAX*2]        panel row j
+    unsigned seq_next;         // sequence base of the next panel launch (host side)
+    int *abort;          // raised by a timed-out exchange
     int *pairs;          // [n_panels_max][2*KB_MAX][2] row gather lists (dst, src), absolute
     int *npairs;         // [n_panels_max]
     int *info;           // 1-based failing column / row, 0 = ok
@@ -61,6 +59,12 @@ int scirs_trsv_device(int n, int nrhs, const T *A, long lda, T *B, long ldb, int
 int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
                            int trans_b, float *C, long ldc, int lower, int mode, const int *info,
                            hipStream_t stream);
+
+// Triangular solve with 256-row diagonal blocks (two launches per block):
+// X <- op(A)^-1 X with op the lower (unit or not) or upper triangle of A.
+template <typename T>
+int scirs_trsv_big(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, const int *info,
+                   hipStream_t s);
 
 template <typename T>
 int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);

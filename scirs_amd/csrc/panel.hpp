@@ -1,0 +1,329 @@
+// Cooperative partial-pivot panel factorisation (included by getrf.hip).
+//
+// The panel's rows are split into contiguous chunks, one per workgroup, and
+// stay in LDS for the whole panel.  Per column j every workgroup publishes
+// its candidate (largest |a_ij| of its chunk, lowest row on ties) and that
+// row's full contents; every workgroup then reads all candidates and picks the
+// same winner — the strict '>' scan of scirs2-linalg/src/lapack.rs:95-104 —
+// swaps, divides (lapack.rs:138) and updates its own rows (lapack.rs:140-142).
+//
+// Exchange protocol: every 64-bit word written to the exchange buffers
+// carries the 32-bit sequence number of its column in the high half (a value
+// wider than 32 bits travels as several tagged words), so a reader knows a
+// word is current without any fence or atomic read-modify-write: relaxed
+// agent-scope atomic stores and loads only.  Readers poll until every word
+// they need carries the expected number; polls are bounded and raise *abort
+// when exceeded, so no wave can wait forever.
+//
+// Latency hiding: right after the pivot row of column j arrives, a workgroup
+// updates only column j+1, finds its next candidate, brings that row (and row
+// j+1 when it holds it) fully up to date, publishes, and only then performs
+// the bulk rank-1 update of its other rows, which overlaps the other
+// workgroups' publication latency.
+#pragma once
+#include "common.hpp"
+
+namespace panel {
+
+constexpr int NT = 256;
+typedef unsigned long long u64;
+
+__device__ __forceinline__ void st_agent(u64 *p, u64 v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 ld_agent(const u64 *p)
+{
+    return __hip_atomic_load(const_cast<u64 *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T> struct Tag;
+template <> struct Tag<float> {
+    static constexpr int W = 1;   // words per value
+    __device__ static void put(u64 *dst, unsigned seq, float v)
+    {
+        st_agent(dst, ((u64)seq << 32) | __float_as_uint(v));
+    }
+    __device__ static bool get(const u64 *src, unsigned seq, float &v)
+    {
+        const u64 w = ld_agent(src);
+        v = __uint_as_float((unsigned)w);
+        return (unsigned)(w >> 32) == seq;
+    }
+};
+template <> struct Tag<double> {
+    static constexpr int W = 2;
+    __device__ static void put(u64 *dst, unsigned seq, double v)
+    {
+        const u64 bits = (u64)__double_as_longlong(v);
+        st_agent(dst, ((u64)seq << 32) | (bits & 0xffffffffull));
+        st_agent(dst + 1, ((u64)seq << 32) | (bits >> 32));
+    }
+    __device__ static bool get(const u64 *src, unsigned seq, double &v)
+    {
+        const u64 lo = ld_agent(src), hi = ld_agent(src + 1);
+        v = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+        return (unsigned)(lo >> 32) == seq && (unsigned)(hi >> 32) == seq;
+    }
+};
+
+template <typename T>
+struct PanelArgs {
+    T *A; long lda;
+    int k0, m, kb, R, G;
+    u64 *xw;       // [2][G][4]   candidate: (seq|row), value words
+    u64 *xrow;     // [2][G][kb*W] candidate rows
+    u64 *xrj;      // [2][kb*W]   row j of the panel
+    unsigned seq_base;
+    int *pairs; int *npairs;
+    int *info; double *info_val; int *abort;
+    unsigned spin_limit;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
+{
+    if (*a.info != 0) return;
+    constexpr int W = Tag<T>::W;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int kb = a.kb, kbp = kb + 1;
+    T *tile = reinterpret_cast<T *>(smem_raw);        // R x kbp
+    T *prow = tile + (size_t)a.R * kbp;               // kb  pivot row
+    T *rjv = prow + kb;                               // kb  row j (for the owner of the pivot row)
+    T *lcol = rjv + kb;                               // R   multipliers
+    int *top_label = reinterpret_cast<int *>(lcol + a.R);   // kb (workgroup 0)
+    int *out_pos = top_label + kb;                    // kb
+    int *out_label = out_pos + kb;                    // kb
+    __shared__ T red_v[NT / 64];
+    __shared__ int red_i[NT / 64];
+    __shared__ int s_p, s_g, s_stop, s_c, s_nout, s_bad;
+    __shared__ T s_pv, s_cv;
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int g = blockIdx.x, G = a.G;
+    const int row0 = g * a.R;
+    const int Rg = max(0, min(a.R, a.m - row0));
+    T *Ap = a.A + (long)a.k0 * a.lda + a.k0;
+
+    g2s_block<T>(tile, kbp, Ap + (long)row0 * a.lda, a.lda, Rg, kb, Rg, kb);
+    if (g == 0) {
+        for (int q = t; q < kb; q += NT) top_label[q] = q;
+        if (t == 0) s_nout = 0;
+    }
+    if (t == 0) s_stop = 0;
+    __syncthreads();
+
+    // local arg-max of column `col` over local rows >= rstart -> s_c (local row, -1 none), s_cv
+    auto local_candidate = [&](int col, int rstart) {
+        T bv = (T)0; int bi = -1;
+        for (int r = rstart + t; r < Rg; r += NT) {
+            const T v = fabs(tile[r * kbp + col]);
+            if (bi < 0 || v > bv) { bv = v; bi = r; }
+        }
+        wave_argmax(bv, bi);
+        if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+        __syncthreads();
+        if (t == 0) {
+            T v = red_v[0]; int i = red_i[0];
+            for (int w = 1; w < NT / 64; ++w) {
+                const int oi = red_i[w]; const T ov = red_v[w];
+                if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) { v = ov; i = oi; }
+            }
+            s_c = i; s_cv = v;
+        }
+        __syncthreads();
+    };
+    // publish the candidate of column j (rows are fully up to date for column j)
+    auto publish = [&](int j) {
+        const int b = j & 1;
+        const unsigned seq = a.seq_base + (unsigned)j + 1u;
+        const int c = s_c;
+        u64 *xr = a.xrow + ((size_t)b * G + g) * (size_t)kb * W;
+        if (c >= 0)
+            for (int q = t; q < kb; q += NT) Tag<T>::put(xr + (size_t)q * W, seq, tile[c * kbp + q]);
+        if (j >= row0 && j < row0 + Rg) {
+            u64 *xj = a.xrj + (size_t)b * kb * W;
+            for (int q = t; q < kb; q += NT) Tag<T>::put(xj + (size_t)q * W, seq, tile[(j - row0) * kbp + q]);
+        }
+        if (t == 0) {
+            u64 *xw = a.xw + ((size_t)b * G + g) * 4;
+            st_agent(xw, ((u64)seq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
+            Tag<T>::put(xw + 1, seq, s_cv);
+        }
+    };
+
+    local_candidate(0, 0);
+    publish(0);
+
+    const T eps = num_limits<T>::eps();
+    for (int j = 0; j < kb; ++j) {
+        const int b = j & 1;
+        const unsigned seq = a.seq_base + (unsigned)j + 1u;
+        // ---- 1. every workgroup reduces the same candidate list
+        if (wave == 0) {
+            unsigned spins = 0;
+            T v = (T)0; int i = -1, wg = -1;
+            bool timed_out = false;
+            while (true) {
+                bool ok = true;
+                v = (T)0; i = -1; wg = -1;
+                for (int q = lane; q < G; q += 64) {
+                    const u64 *xw = a.xw + ((size_t)b * G + q) * 4;
+                    const u64 w0 = ld_agent(xw);
+                    T qv;
+                    const bool okv = Tag<T>::get(xw + 1, seq, qv);
+                    ok = ok && okv && (unsigned)(w0 >> 32) == seq;
+                    const int qi = (int)(unsigned)w0;
+                    if (qi >= 0 && (i < 0 || qv > v || (qv == v && qi < i))) { v = qv; i = qi; wg = q; }
+                }
+                if (__all(ok)) break;
+                if (++spins > a.spin_limit ||
+                    __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    timed_out = true;
+                    break;
+                }
+            }
+            T vv = v; int ii = i;
+            wave_argmax(vv, ii);
+            const unsigned long long ball = __ballot(i == ii && i >= 0);
+            const int src = ball ? __ffsll(ball) - 1 : 0;
+            const int wgw = __shfl(wg, src, 64);
+            if (lane == 0) {
+                s_p = ii; s_g = wgw; s_pv = vv;
+                if (timed_out) {
+                    __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_stop = 1;
+                } else if (!(vv >= eps)) {   // |pivot| < eps: singular (lapack.rs:106-120)
+                    s_stop = 1;
+                    if (g == 0) { *a.info = a.k0 + j + 1; *a.info_val = (double)vv; }
+                }
+            }
+        }
+        __syncthreads();
+        if (s_stop) break;
+        const int p = s_p, sg = s_g;
+        const bool own_j = (j >= row0 && j < row0 + Rg);
+        const bool own_p = (p >= row0 && p < row0 + Rg);
+        // ---- 2. fetch the pivot row (and row j when this workgroup holds row p)
+        {
+            const u64 *xr = a.xrow + ((size_t)b * G + sg) * (size_t)kb * W;
+            const u64 *xj = a.xrj + (size_t)b * kb * W;
+            const bool need_j = own_p && p != j;
+            unsigned spins = 0;
+            while (true) {
+                if (t == 0) s_bad = 0;
+                __syncthreads();
+                bool bad = false;
+                for (int q = t; q < kb; q += NT) {
+                    T v;
+                    if (!Tag<T>::get(xr + (size_t)q * W, seq, v)) bad = true;
+                    prow[q] = v;
+                    if (need_j) {
+                        if (!Tag<T>::get(xj + (size_t)q * W, seq, v)) bad = true;
+                        rjv[q] = v;
+                    }
+                }
+                if (bad) s_bad = 1;
+                __syncthreads();
+                if (!s_bad) break;
+                if (++spins > a.spin_limit) {
+                    if (t == 0) __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            if (s_bad) break;
+        }
+        // ---- 3. row interchange inside the panel and the permutation record
+        if (p != j) {
+            if (own_j)
+                for (int c = t; c < kb; c += NT) tile[(j - row0) * kbp + c] = prow[c];
+            if (own_p)
+                for (int c = t; c < kb; c += NT) tile[(p - row0) * kbp + c] = rjv[c];
+        }
+        if (g == 0 && wave == 0 && p != j) {
+            if (p < kb) {
+                if (lane == 0) { int tmp = top_label[j]; top_label[j] = top_label[p]; top_label[p] = tmp; }
+            } else {
+                const int nout = s_nout;
+                int found = -1;
+                for (int base = 0; base < nout && found < 0; base += 64) {
+                    const int s = base + lane;
+                    const unsigned long long bl = __ballot(s < nout && out_pos[s] == p);
+                    if (bl) found = base + __ffsll(bl) - 1;
+                }
+                if (lane == 0) {
+                    if (found < 0) { found = nout; out_pos[found] = p; out_label[found] = p; s_nout = nout + 1; }
+                    int tmp = top_label[j]; top_label[j] = out_label[found]; out_label[found] = tmp;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- 4. multipliers by true division (lapack.rs:138)
+        const T pivot = prow[j];
+        const int r1 = max(0, j + 1 - row0);
+        for (int r = r1 + t; r < Rg; r += NT) {
+            const T l = tile[r * kbp + j] / pivot;
+            tile[r * kbp + j] = l;
+            lcol[r] = l;
+        }
+        __syncthreads();
+        if (j + 1 >= kb) break;
+        // ---- 5. column j+1 first, next candidate, publish
+        for (int r = r1 + t; r < Rg; r += NT) tile[r * kbp + j + 1] -= lcol[r] * prow[j + 1];
+        __syncthreads();
+        local_candidate(j + 1, r1);
+        const int c = s_c;
+        const int jn = (j + 1 >= row0 && j + 1 < row0 + Rg) ? j + 1 - row0 : -1;
+        for (int q = j + 2 + t; q < kb; q += NT) {
+            if (c >= 0) tile[c * kbp + q] -= lcol[c] * prow[q];
+            if (jn >= 0 && jn != c) tile[jn * kbp + q] -= lcol[jn] * prow[q];
+        }
+        __syncthreads();
+        publish(j + 1);
+        // ---- 6. bulk rank-1 update of the remaining rows (overlaps the exchange)
+        if (j + 2 < kb) {
+            // TPR threads per row, consecutive threads on consecutive columns
+            constexpr int TPR = 8;
+            for (int r = r1 + t / TPR; r < Rg; r += NT / TPR) {
+                if (r == c || r == jn) continue;
+                const T l = lcol[r];
+                T *row = tile + r * kbp;
+                for (int q = j + 2 + (t % TPR); q < kb; q += TPR) row[q] -= l * prow[q];
+            }
+        }
+        __syncthreads();
+    }
+
+    for (int r = wave; r < Rg; r += NT / 64)
+        for (int c = lane; c < kb; c += 64) Ap[(long)(row0 + r) * a.lda + c] = tile[r * kbp + c];
+
+    if (g == 0) {
+        __syncthreads();
+        if (t == 0) {
+            int np = 0;
+            for (int q = 0; q < kb; ++q)
+                if (top_label[q] != q) { a.pairs[2 * np] = a.k0 + q; a.pairs[2 * np + 1] = a.k0 + top_label[q]; ++np; }
+            for (int s = 0; s < s_nout; ++s)
+                if (out_label[s] != out_pos[s]) {
+                    a.pairs[2 * np] = a.k0 + out_pos[s]; a.pairs[2 * np + 1] = a.k0 + out_label[s]; ++np;
+                }
+            *a.npairs = np;
+        }
+    }
+}
+
+inline size_t lds_bytes(int R, int kb, size_t esz)
+{
+    return ((size_t)R * (kb + 1) + 2 * (size_t)kb + R) * esz + 3 * (size_t)kb * sizeof(int);
+}
+
+template <typename T>
+inline int max_rows(int kb)
+{
+    const size_t budget = 160 * 1024 - 1024;   // static reduction buffers
+    const size_t per_row = (size_t)(kb + 2) * sizeof(T);
+    const size_t fixed = 2 * (size_t)kb * sizeof(T) + 3 * (size_t)kb * sizeof(int);
+    return (int)((budget - fixed) / per_row);
+}
+
+} // namespace panel

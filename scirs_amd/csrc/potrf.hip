@@ -23,8 +23,7 @@ __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int k
     const int p = kb + 1;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     T *D = A + (long)k0 * lda + k0;
-    for (int i = wave; i < kb; i += NT / 64)
-        for (int j = lane; j <= i; j += 64) S[i * p + j] = D[(long)i * lda + j];
+    g2s_block<T>(S, p, D, lda, kb, kb, kb, kb);   // (the strict upper part is never read)
     __shared__ int s_fail;
     if (t == 0) s_fail = 0;
     __syncthreads();
@@ -72,10 +71,23 @@ __global__ __launch_bounds__(NT) void trsm_rltn_kernel(T *A, long lda, int k0, i
         for (int j = lane; j <= i; j += 64) dst[j] = L[(long)i * lda + j];
     }
     const int rbase = r0 + blockIdx.x * 64;
-    for (int r = rg; r < 64; r += NT / 64) {
-        const int gr = rbase + r;
-        for (int j = lane; j < kb; j += 64)
-            Bs[j * 65 + r] = (gr < r0 + nrows) ? A[(long)gr * lda + k0 + j] : (T)0;
+    {
+        // the 64 x kb row strip, transposed into Bs[j*65 + r], 8 loads in flight per thread
+        const int vrows = min(64, r0 + nrows - rbase);
+        constexpr int U = 8;
+        for (int b = threadIdx.x; b < 64 * kb; b += U * NT) {
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = b + u * NT, r = idx / kb, j = idx - r * kb;
+                v[u] = (idx < 64 * kb && r < vrows) ? A[(long)(rbase + r) * lda + k0 + j] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = b + u * NT, r = idx / kb, j = idx - r * kb;
+                if (idx < 64 * kb) Bs[j * 65 + r] = v[u];
+            }
+        }
     }
     __syncthreads();
     for (int jb = 0; jb < kb; jb += 8) {

@@ -3,6 +3,7 @@
 // small vector updates of the refinement loops.  All row-major.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "prof.hpp"
 
 namespace {
 
@@ -137,6 +138,7 @@ int scirs_residual(int n, int nrhs, const double *A, long lda, const double *X, 
 {
     const int waves = (n + 3) / 4;
     const int blocks = (waves * 64 + NT - 1) / NT;
+    scirs::ProfScope ps(SCIRS_PROF_RESIDUAL, 2.0 * n * n * nrhs, 8.0 * n * n + 24.0 * n * nrhs, s);
     hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(NT), 0, s, n, nrhs, A, lda, X, ldx, B, ldb, R, ldr);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
@@ -160,6 +162,7 @@ int scirs_convert_rowsum(int n, int m, const double *A, long lda, float *A32, lo
                          hipStream_t s)
 {
     const int blocks = (n * 64 + NT - 1) / NT;
+    scirs::ProfScope ps(SCIRS_PROF_CONVERT, 0.0, (8.0 + (A32 ? 4.0 : 0.0)) * n * m, s);
     hipLaunchKernelGGL(convert_rowsum_kernel, dim3(blocks), dim3(NT), 0, s, n, m, A, lda, A32, lda32, rowsum);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;

@@ -185,3 +185,16 @@ def backward_error_dev(A, X, B):
                                           _stream_ptr(), ctypes.byref(out))
     _check("backward_error_dev", rc, None)
     return out.value
+
+
+def prof_begin():
+    N.load().scirs_gpu_prof_begin()
+
+
+def prof_end():
+    """Per kernel class: launches, total_ms, flops, bytes (event timing)."""
+    arr = (N.ProfStats * len(N.PROF_CLASSES))()
+    rc = N.load().scirs_gpu_prof_end(arr)
+    _check("prof_end", rc, None)
+    return {name: {"launches": int(a.launches), "total_ms": a.total_ms, "flops": a.flops, "bytes": a.bytes}
+            for name, a in zip(N.PROF_CLASSES, arr)}
