@@ -1,0 +1,546 @@
+// scirs2_linalg.hpp — C++17 mirror of the scirs2-linalg dense-solve API on top of
+// the C ABI in scirs_gpu.h.  It is the host side a binding keeps: the same
+// function names, argument meaning, validation order, error variants and error
+// texts as the Rust functions it stands in for, with the O(n^3) work sent to the
+// GPU library.
+//
+//   solve            scirs2-linalg/src/solve.rs:54-111
+//   solve_triangular scirs2-linalg/src/solve.rs:139-202
+//   solve_multiple   scirs2-linalg/src/solve.rs:366-424
+//   lu               scirs2-linalg/src/decomposition.rs:108-183
+//   cholesky         scirs2-linalg/src/decomposition.rs:44-81
+//   inv, det         scirs2-linalg/src/basic.rs:117-186, 29-93
+//   validation       scirs2-linalg/src/validation.rs:22-329
+//   LinalgError      scirs2-linalg/src/error.rs:8-150
+//
+// Header-only; link with libscirs_gpu.so.  Element types: double and float.
+#ifndef SCIRS2_LINALG_HPP
+#define SCIRS2_LINALG_HPP
+
+#include "scirs_gpu.h"
+
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <optional>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+namespace scirs2_linalg {
+
+// ------------------------------------------------------------------ errors
+// Variants and Display prefixes of LinalgError (error.rs:8-66).
+class LinalgError {
+  public:
+    enum class Kind {
+        ComputationError, DomainError, ConvergenceError, DimensionError, ShapeError, IndexError,
+        SingularMatrixError, NonPositiveDefiniteError, NotImplementedError, ImplementationError,
+        ValueError, InvalidInputError, InvalidInput, NumericalError
+    };
+    LinalgError(Kind k, std::string msg) : kind_(k), msg_(std::move(msg)) {}
+    Kind kind() const { return kind_; }
+    const std::string &message() const { return msg_; }
+    // the Display text of the Rust enum
+    std::string to_string() const
+    {
+        static const char *prefix[] = {
+            "Computation error: ", "Domain error: ", "Convergence error: ", "Dimension mismatch error: ",
+            "Shape error: ", "Index out of bounds: ", "Singular matrix error: ",
+            "Non-positive definite matrix error: ", "Not implemented: ", "Implementation error: ",
+            "Value error: ", "Invalid input error: ", "Invalid input: ", "Numerical error: "};
+        return std::string(prefix[(int)kind_]) + msg_;
+    }
+
+  private:
+    Kind kind_;
+    std::string msg_;
+};
+
+inline std::string shape_str(std::size_t r, std::size_t c)
+{
+    return std::to_string(r) + "\xC3\x97" + std::to_string(c);   // "r×c"
+}
+
+// {:.2e} of Rust for the condition-number line
+inline std::string fmt_e2(double v)
+{
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.2e", v);
+    // Rust prints the exponent without sign padding: 1.00e16, 2.50e-3
+    std::string s(buf);
+    auto e = s.find('e');
+    if (e == std::string::npos) return s;
+    std::string mant = s.substr(0, e), ex = s.substr(e + 1);
+    bool neg = !ex.empty() && ex[0] == '-';
+    if (!ex.empty() && (ex[0] == '+' || ex[0] == '-')) ex = ex.substr(1);
+    while (ex.size() > 1 && ex[0] == '0') ex = ex.substr(1);
+    return mant + "e" + (neg ? "-" : "") + ex;
+}
+
+// error.rs:73-110
+inline LinalgError singular_matrix_with_suggestions(const std::string &operation, std::pair<std::size_t, std::size_t> shape,
+                                                    std::optional<double> condition_number)
+{
+    std::string base = "Matrix is singular during " + operation + " operation";
+    std::string shape_info = "Matrix shape: " + shape_str(shape.first, shape.second);
+    std::vector<std::string> s = {
+        "Consider the following regularization approaches:",
+        "1. Ridge regularization: Add \xCE\xBBI to the matrix (small positive \xCE\xBB)",
+        "2. Pseudo-inverse: Use SVD-based pseudo-inverse for rank-deficient matrices",
+        "3. Truncated SVD: Remove small singular values below a threshold"};
+    if (condition_number) {
+        s.push_back("4. Condition number: " + fmt_e2(*condition_number) + " (>1e12 indicates ill-conditioning)");
+        if (*condition_number > 1e12) s.push_back("5. Use extended precision arithmetic for better numerical stability");
+    }
+    s.push_back("6. Check input data for linear dependencies or scaling issues");
+    s.push_back("7. Use iterative refinement for improved accuracy");
+    std::string joined;
+    for (std::size_t i = 0; i < s.size(); ++i) joined += (i ? "\n" : "") + s[i];
+    return LinalgError(LinalgError::Kind::SingularMatrixError, base + "\n" + shape_info + "\n" + joined);
+}
+
+// error.rs:112-150
+inline LinalgError non_positive_definite_with_suggestions(const std::string &operation,
+                                                          std::pair<std::size_t, std::size_t> shape,
+                                                          std::optional<std::size_t> negative_eigenvalues)
+{
+    std::string base = "Matrix is not positive definite during " + operation + " operation";
+    std::string shape_info = "Matrix shape: " + shape_str(shape.first, shape.second);
+    std::vector<std::string> s = {
+        "Consider the following regularization approaches:",
+        "1. Diagonal regularization: Add \xCE\xBBI where \xCE\xBB > |most negative eigenvalue|",
+        "2. Modified Cholesky: Use algorithms that ensure positive definiteness",
+        "3. Eigenvalue clipping: Replace negative eigenvalues with small positive values",
+        "4. Use LDL decomposition instead of Cholesky for indefinite matrices"};
+    if (negative_eigenvalues)
+        s.push_back("5. Found " + std::to_string(*negative_eigenvalues) +
+                    " negative eigenvalue(s) - consider spectral regularization");
+    s.push_back("6. Check if matrix is symmetric (required for Cholesky)");
+    s.push_back("7. Use pivoted Cholesky for rank-deficient positive semidefinite matrices");
+    s.push_back("8. Consider using QR or LU decomposition for non-symmetric matrices");
+    std::string joined;
+    for (std::size_t i = 0; i < s.size(); ++i) joined += (i ? "\n" : "") + s[i];
+    return LinalgError(LinalgError::Kind::NonPositiveDefiniteError, base + "\n" + shape_info + "\n" + joined);
+}
+
+// LinalgResult<T> (error.rs:202)
+template <class T>
+class Result {
+  public:
+    Result(T v) : ok_(true), val_(std::move(v)), err_(LinalgError::Kind::ComputationError, "") {}
+    Result(LinalgError e) : ok_(false), val_(), err_(std::move(e)) {}
+    bool is_ok() const { return ok_; }
+    bool is_err() const { return !ok_; }
+    const T &value() const { return val_; }
+    T &value() { return val_; }
+    const LinalgError &error() const { return err_; }
+
+  private:
+    bool ok_;
+    T val_;
+    LinalgError err_;
+};
+
+// ------------------------------------------------------------------ arrays
+// Strided views with the meaning of ndarray's ArrayView1/ArrayView2.
+template <class F>
+struct ArrayView1 {
+    const F *data = nullptr;
+    std::size_t len = 0;
+    std::ptrdiff_t stride = 1;
+    F operator[](std::size_t i) const { return data[(std::ptrdiff_t)i * stride]; }
+    bool is_empty() const { return len == 0; }
+};
+
+template <class F>
+struct ArrayView2 {
+    const F *data = nullptr;
+    std::size_t rows = 0, cols = 0;
+    std::ptrdiff_t row_stride = 0, col_stride = 1;
+    F operator()(std::size_t i, std::size_t j) const
+    {
+        return data[(std::ptrdiff_t)i * row_stride + (std::ptrdiff_t)j * col_stride];
+    }
+    std::size_t nrows() const { return rows; }
+    std::size_t ncols() const { return cols; }
+    bool is_empty() const { return rows == 0 || cols == 0; }
+    std::pair<std::size_t, std::size_t> dim() const { return {rows, cols}; }
+};
+
+template <class F>
+struct Array2 {
+    std::size_t rows = 0, cols = 0;
+    std::vector<F> data;   // row-major
+    Array2() = default;
+    Array2(std::size_t r, std::size_t c) : rows(r), cols(c), data(r * c, F(0)) {}
+    F &operator()(std::size_t i, std::size_t j) { return data[i * cols + j]; }
+    F operator()(std::size_t i, std::size_t j) const { return data[i * cols + j]; }
+    ArrayView2<F> view() const { return {data.data(), rows, cols, (std::ptrdiff_t)cols, 1}; }
+};
+
+template <class F>
+using Array1 = std::vector<F>;
+
+template <class F>
+ArrayView1<F> view1(const std::vector<F> &v)
+{
+    return {v.data(), v.size(), 1};
+}
+
+// ------------------------------------------------------------------ validation (validation.rs)
+template <class F>
+std::optional<LinalgError> validate_not_empty_matrix(const ArrayView2<F> &m, const std::string &op)
+{
+    if (m.is_empty()) return LinalgError(LinalgError::Kind::ShapeError, op + " failed: Input matrix cannot be empty");
+    return std::nullopt;
+}
+template <class F>
+std::optional<LinalgError> validate_not_empty_vector(const ArrayView1<F> &v, const std::string &op)
+{
+    if (v.is_empty()) return LinalgError(LinalgError::Kind::ShapeError, op + " failed: Input vector cannot be empty");
+    return std::nullopt;
+}
+// validation.rs:69-81 (the Rust format string carries literal "\n" sequences)
+template <class F>
+std::optional<LinalgError> validate_square_matrix(const ArrayView2<F> &m, const std::string &op)
+{
+    if (m.nrows() != m.ncols())
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           op + " failed: Matrix must be square\\nMatrix shape: " + shape_str(m.nrows(), m.ncols()) +
+                               "\\nExpected: Square matrix (n\xC3\x97n)");
+    return std::nullopt;
+}
+template <class F>
+std::optional<LinalgError> validate_matrix_vector_dimensions(const ArrayView2<F> &m, const ArrayView1<F> &v,
+                                                             const std::string &op)
+{
+    if (m.nrows() != v.len)
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           op + " failed: Matrix and vector dimensions must match\\nMatrix shape: " +
+                               shape_str(m.nrows(), m.ncols()) + "\\nVector shape: " + std::to_string(v.len) +
+                               "\\nExpected: Vector length = " + std::to_string(m.nrows()));
+    return std::nullopt;
+}
+template <class F>
+std::optional<LinalgError> validate_matrix_matrix_dimensions(const ArrayView2<F> &a, const ArrayView2<F> &b,
+                                                             const std::string &op, bool require_same_rows)
+{
+    if (require_same_rows && a.nrows() != b.nrows())
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           op + " failed: Matrix rows must match\\nFirst matrix shape: " + shape_str(a.nrows(), a.ncols()) +
+                               "\\nSecond matrix shape: " + shape_str(b.nrows(), b.ncols()) +
+                               "\\nExpected: Same number of rows");
+    return std::nullopt;
+}
+template <class F>
+std::optional<LinalgError> validate_finite_matrix(const ArrayView2<F> &m, const std::string &op)
+{
+    for (std::size_t i = 0; i < m.nrows(); ++i)
+        for (std::size_t j = 0; j < m.ncols(); ++j)
+            if (!std::isfinite(m(i, j)))
+                return LinalgError(LinalgError::Kind::InvalidInputError, op + " failed: Matrix contains non-finite values");
+    return std::nullopt;
+}
+template <class F>
+std::optional<LinalgError> validate_finite_vector(const ArrayView1<F> &v, const std::string &op)
+{
+    for (std::size_t i = 0; i < v.len; ++i)
+        if (!std::isfinite(v[i]))
+            return LinalgError(LinalgError::Kind::InvalidInputError, op + " failed: Vector contains non-finite values");
+    return std::nullopt;
+}
+// validation.rs:213-228
+template <class F>
+std::optional<LinalgError> validate_linear_system(const ArrayView2<F> &m, const ArrayView1<F> &v, const std::string &op)
+{
+    if (auto e = validate_not_empty_matrix(m, op)) return e;
+    if (auto e = validate_not_empty_vector(v, op)) return e;
+    if (auto e = validate_square_matrix(m, op)) return e;
+    if (auto e = validate_matrix_vector_dimensions(m, v, op)) return e;
+    if (auto e = validate_finite_matrix(m, op)) return e;
+    if (auto e = validate_finite_vector(v, op)) return e;
+    return std::nullopt;
+}
+// validation.rs:280-294
+template <class F>
+std::optional<LinalgError> validate_decomposition(const ArrayView2<F> &m, const std::string &op, bool require_square)
+{
+    if (auto e = validate_not_empty_matrix(m, op)) return e;
+    if (require_square)
+        if (auto e = validate_square_matrix(m, op)) return e;
+    if (auto e = validate_finite_matrix(m, op)) return e;
+    return std::nullopt;
+}
+// validation.rs:314-329
+template <class F>
+std::optional<LinalgError> validate_multiple_linear_systems(const ArrayView2<F> &a, const ArrayView2<F> &b,
+                                                            const std::string &op)
+{
+    if (auto e = validate_not_empty_matrix(a, op)) return e;
+    if (auto e = validate_not_empty_matrix(b, op)) return e;
+    if (auto e = validate_square_matrix(a, op)) return e;
+    if (auto e = validate_matrix_matrix_dimensions(a, b, op, true)) return e;
+    if (auto e = validate_finite_matrix(a, op)) return e;
+    if (auto e = validate_finite_matrix(b, op)) return e;
+    return std::nullopt;
+}
+
+// ------------------------------------------------------------------ GPU dispatch
+namespace detail {
+template <class F>
+constexpr bool supported = std::is_same<F, double>::value || std::is_same<F, float>::value;
+
+// row-major contiguous copy of a strided view (ndarray's as_standard_layout)
+template <class F>
+std::vector<F> standard(const ArrayView2<F> &a)
+{
+    std::vector<F> out(a.rows * a.cols);
+    for (std::size_t i = 0; i < a.rows; ++i)
+        for (std::size_t j = 0; j < a.cols; ++j) out[i * a.cols + j] = a(i, j);
+    return out;
+}
+
+inline LinalgError gpu_failure(int rc)
+{
+    return LinalgError(LinalgError::Kind::ComputationError,
+                       std::string("GPU solve failed (") + std::to_string(rc) + "): " + scirs_gpu_last_error());
+}
+
+inline void set_workers(std::optional<std::size_t> workers)
+{
+    // solve.rs:79-81: the only effect of `workers` is this environment variable
+    if (workers) setenv("OMP_NUM_THREADS", std::to_string(*workers).c_str(), 1);
+}
+
+template <class F>
+int getrf(int m, int n, F *a, int64_t *perm, scirs_solve_info *info)
+{
+    if constexpr (std::is_same<F, double>::value) return scirs_gpu_dgetrf(m, n, a, n, perm, info);
+    else return scirs_gpu_sgetrf(m, n, a, n, perm, info);
+}
+template <class F>
+int gesv(int n, int nrhs, const F *a, const F *b, F *x, scirs_solve_info *info)
+{
+    if constexpr (std::is_same<F, double>::value) return scirs_gpu_dgesv(n, nrhs, a, n, b, nrhs, x, nrhs, info);
+    else return scirs_gpu_sgesv(n, nrhs, a, n, b, nrhs, x, nrhs, info);
+}
+template <class F>
+int potrf(int n, F *a, scirs_solve_info *info)
+{
+    if constexpr (std::is_same<F, double>::value) return scirs_gpu_dpotrf(n, a, n, info);
+    else return scirs_gpu_spotrf(n, a, n, info);
+}
+} // namespace detail
+
+template <class F>
+Result<Array2<F>> solve_multiple(const ArrayView2<F> &a, const ArrayView2<F> &b, std::optional<std::size_t> workers);
+template <class F>
+Result<Array2<F>> inv(const ArrayView2<F> &a, std::optional<std::size_t> workers);
+
+// decomposition.rs:108-183: (P, L, U) with P A = L U
+template <class F>
+Result<std::tuple<Array2<F>, Array2<F>, Array2<F>>> lu(const ArrayView2<F> &a, std::optional<std::size_t> workers)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    if (a.is_empty())
+        return LinalgError(LinalgError::Kind::ShapeError, "LU decomposition failed: Input matrix cannot be empty");
+    for (std::size_t i = 0; i < a.rows; ++i)
+        for (std::size_t j = 0; j < a.cols; ++j)
+            if (!std::isfinite(a(i, j)))
+                return LinalgError(LinalgError::Kind::InvalidInputError,
+                                   "LU decomposition failed: Matrix contains non-finite values");
+    detail::set_workers(workers);
+    const std::size_t n = a.rows, m = a.cols, k = n < m ? n : m;
+    std::vector<F> packed = detail::standard(a);
+    std::vector<int64_t> piv(n);
+    scirs_solve_info info;
+    int rc = detail::getrf<F>((int)n, (int)m, packed.data(), piv.data(), &info);
+    if (rc > 0) return singular_matrix_with_suggestions("LU decomposition", a.dim(), std::nullopt);
+    if (rc < 0) return detail::gpu_failure(rc);
+    Array2<F> p(n, n), l(n, k), u(k, m);
+    for (std::size_t i = 0; i < n; ++i) p(i, (std::size_t)piv[i]) = F(1);
+    for (std::size_t i = 0; i < n; ++i) {
+        for (std::size_t j = 0; j < (i < m ? i : m); ++j) l(i, j) = packed[i * m + j];
+        if (i < m) l(i, i) = F(1);
+    }
+    for (std::size_t i = 0; i < k; ++i)
+        for (std::size_t j = i; j < m; ++j) u(i, j) = packed[i * m + j];
+    return std::make_tuple(std::move(p), std::move(l), std::move(u));
+}
+
+// decomposition.rs:44-81
+template <class F>
+Result<Array2<F>> cholesky(const ArrayView2<F> &a, std::optional<std::size_t> workers)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    if (auto e = validate_decomposition(a, "Cholesky decomposition", true)) return *e;
+    detail::set_workers(workers);
+    const std::size_t n = a.rows;
+    Array2<F> l(n, n);
+    l.data = detail::standard(a);
+    scirs_solve_info info;
+    int rc = detail::potrf<F>((int)n, l.data.data(), &info);
+    if (rc > 0) return non_positive_definite_with_suggestions("Cholesky decomposition", a.dim(), std::nullopt);
+    if (rc < 0) return detail::gpu_failure(rc);
+    return l;
+}
+
+// solve.rs:139-202
+template <class F>
+Result<Array1<F>> solve_triangular(const ArrayView2<F> &a, const ArrayView1<F> &b, bool lower, bool unit_diagonal)
+{
+    const std::string op = "Triangular system solve";
+    if (auto e = validate_not_empty_matrix(a, op)) return *e;
+    if (auto e = validate_not_empty_vector(b, op)) return *e;
+    if (auto e = validate_square_matrix(a, op)) return *e;
+    if (auto e = validate_matrix_vector_dimensions(a, b, op)) return *e;
+    if (auto e = validate_finite_matrix(a, op)) return *e;
+    if (auto e = validate_finite_vector(b, op)) return *e;
+    const std::size_t n = a.rows;
+    std::vector<double> ad(n * n), bd(n), xd(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        bd[i] = (double)b[i];
+        for (std::size_t j = 0; j < n; ++j) ad[i * n + j] = (double)a(i, j);
+    }
+    // The reference stops at the first diagonal with |a_ii| < F::epsilon() in
+    // substitution order (solve.rs:166-173, 186-193); only the error comes back,
+    // so the test runs here, before any device work.
+    if (!unit_diagonal)
+        for (std::size_t k = 0; k < n; ++k) {
+            std::size_t i = lower ? k : n - 1 - k;
+            if (std::fabs(a(i, i)) < std::numeric_limits<F>::epsilon())
+                return singular_matrix_with_suggestions(lower ? "triangular system solve (forward substitution)"
+                                                              : "triangular system solve (back substitution)",
+                                                        a.dim(), 1e16);
+        }
+    scirs_solve_info info;
+    int rc = scirs_gpu_dtrsv((int)n, 1, ad.data(), (int64_t)n, bd.data(), 1, xd.data(), 1, lower, unit_diagonal, &info);
+    if (rc > 0)
+        return singular_matrix_with_suggestions(lower ? "triangular system solve (forward substitution)"
+                                                      : "triangular system solve (back substitution)",
+                                                a.dim(), 1e16);
+    if (rc < 0) return detail::gpu_failure(rc);
+    Array1<F> x(n);
+    for (std::size_t i = 0; i < n; ++i) x[i] = (F)xd[i];
+    return x;
+}
+
+// solve.rs:54-111
+template <class F>
+Result<Array1<F>> solve(const ArrayView2<F> &a, const ArrayView1<F> &b, std::optional<std::size_t> workers)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    if (auto e = validate_linear_system(a, b, "Linear system solve")) return *e;
+    const std::size_t n = a.rows;
+    if (n <= 4) {
+        // solve.rs:65-76: small systems go through the explicit inverse
+        auto ai = inv(a, std::nullopt);
+        if (ai.is_err()) return ai.error();
+        Array1<F> x(n, F(0));
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = 0; j < n; ++j) x[i] += ai.value()(i, j) * b[j];
+        return x;
+    }
+    detail::set_workers(workers);
+    std::vector<F> ad = detail::standard(a), bd(n), xd(n);
+    for (std::size_t i = 0; i < n; ++i) bd[i] = b[i];
+    scirs_solve_info info;
+    int rc = detail::gesv<F>((int)n, 1, ad.data(), bd.data(), xd.data(), &info);
+    if (rc > 0) return singular_matrix_with_suggestions("linear system solve", a.dim(), std::nullopt);
+    if (rc < 0) return detail::gpu_failure(rc);
+    return xd;
+}
+
+// solve.rs:366-424
+template <class F>
+Result<Array2<F>> solve_multiple(const ArrayView2<F> &a, const ArrayView2<F> &b, std::optional<std::size_t> workers)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    if (auto e = validate_multiple_linear_systems(a, b, "Multiple linear systems solve")) return *e;
+    detail::set_workers(workers);
+    const std::size_t n = a.rows, nrhs = b.cols;
+    std::vector<F> ad = detail::standard(a), bd = detail::standard(b);
+    Array2<F> x(a.cols, nrhs);
+    scirs_solve_info info;
+    int rc = detail::gesv<F>((int)n, (int)nrhs, ad.data(), bd.data(), x.data.data(), &info);
+    if (rc > 0) return singular_matrix_with_suggestions("multiple linear systems solve", a.dim(), std::nullopt);
+    if (rc < 0) return detail::gpu_failure(rc);
+    return x;
+}
+
+// basic.rs:29-93 (the n > 3 branch through lu)
+template <class F>
+Result<F> det(const ArrayView2<F> &a, std::optional<std::size_t> workers)
+{
+    if (a.nrows() != a.ncols())
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           "Determinant computation failed: Matrix must be square\nMatrix shape: " +
+                               shape_str(a.nrows(), a.ncols()) + "\nExpected: Square matrix (n\xC3\x97n)");
+    switch (a.nrows()) {
+    case 0: return F(1);
+    case 1: return a(0, 0);
+    case 2: return a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0);
+    case 3:
+        return a(0, 0) * (a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1)) - a(0, 1) * (a(1, 0) * a(2, 2) - a(1, 2) * a(2, 0)) +
+               a(0, 2) * (a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0));
+    default: break;
+    }
+    auto r = lu(a, workers);
+    if (r.is_err()) {
+        if (r.error().kind() == LinalgError::Kind::SingularMatrixError) return F(0);
+        return r.error();
+    }
+    const auto &p = std::get<0>(r.value());
+    const auto &u = std::get<2>(r.value());
+    F det_u = F(1);
+    for (std::size_t i = 0; i < u.rows; ++i) det_u *= u(i, i);
+    std::size_t swaps = 0;
+    for (std::size_t i = 0; i < p.rows; ++i)
+        for (std::size_t j = 0; j < i; ++j)
+            if (p(i, j) == F(1)) ++swaps;
+    return swaps % 2 == 0 ? det_u : -det_u;
+}
+
+// basic.rs:117-186
+template <class F>
+Result<Array2<F>> inv(const ArrayView2<F> &a, std::optional<std::size_t> workers)
+{
+    if (a.nrows() != a.ncols())
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           "Matrix inverse computation failed: Matrix must be square\nMatrix shape: " +
+                               shape_str(a.nrows(), a.ncols()) + "\nExpected: Square matrix (n\xC3\x97n)");
+    if (a.nrows() == 2) {
+        auto d = det(a, workers);
+        if (d.is_err()) return d.error();
+        const F det_val = d.value();
+        if (std::fabs(det_val) < std::numeric_limits<F>::epsilon()) {
+            const F norm_a = std::sqrt(a(0, 0) * a(0, 0) + a(0, 1) * a(0, 1) + a(1, 0) * a(1, 0) + a(1, 1) * a(1, 1));
+            std::optional<double> cond;
+            if (std::fabs(det_val) > F(0)) cond = (double)(norm_a / std::fabs(det_val));
+            return singular_matrix_with_suggestions("matrix inverse", a.dim(), cond);
+        }
+        const F inv_det = F(1) / det_val;
+        Array2<F> r(2, 2);
+        r(0, 0) = a(1, 1) * inv_det;
+        r(0, 1) = -a(0, 1) * inv_det;
+        r(1, 0) = -a(1, 0) * inv_det;
+        r(1, 1) = a(0, 0) * inv_det;
+        return r;
+    }
+    const std::size_t n = a.nrows();
+    Array2<F> eye(n, n);
+    for (std::size_t i = 0; i < n; ++i) eye(i, i) = F(1);
+    auto r = solve_multiple(a, eye.view(), workers);
+    if (r.is_err() && r.error().kind() == LinalgError::Kind::SingularMatrixError)
+        return singular_matrix_with_suggestions("matrix inverse via solve", a.dim(), std::nullopt);
+    return r;
+}
+
+} // namespace scirs2_linalg
+
+#endif // SCIRS2_LINALG_HPP

@@ -130,6 +130,32 @@ int scirs_gpu_dgesv_ir_dev(int n, int nrhs, const double *dA, int64_t lda, const
                            double *dX, int64_t ldx, int mode, int max_iter, int max_gmres, double tol,
                            void *stream, scirs_solve_info *info);
 
+/* ---- batched uneven solves ------------------------------------------------------ */
+/* No reference counterpart: scirs2-linalg/src/batch/mod.rs has no batched solve.
+ * Each system i follows the reference's per-system semantics — `solve`
+ * (solve.rs:54-111) when spd[i] == 0, the cholesky + substitution composition
+ * (decomposition.rs:44, solve.rs:139) when spd[i] != 0 — computed as an FP32
+ * factorisation plus FP64 refinement to a normwise backward error <= tol_scale *
+ * n_i * 2^-53 (FP64 factorisation when FP32 breaks down or stalls), all systems in
+ * one persistent, hipGraph-replayed launch.
+ * Host arrays: n, nrhs (1 <= n <= 256, 1 <= nrhs <= 16), spd (may be NULL), offA,
+ * offB (element offsets of A_i, row-major n_i x n_i, and of B_i / X_i, row-major
+ * n_i x nrhs_i), status (out: 0, or the breakdown column / row), bwd (out, may be
+ * NULL).  Device arrays: dA, dB, dX (packed). */
+typedef struct scirs_batch_info {
+    int32_t failed;              /* systems with status != 0                         */
+    int32_t max_iters;           /* most refinement steps taken by any system        */
+    double max_backward_error;   /* over the systems with status 0                   */
+    double time_ms;              /* device time of the batched launch                */
+    double total_ms;             /* wall time of the call                            */
+    int32_t wave_max;            /* systems with n <= wave_max ran one per wave      */
+    int32_t graph_replayed;      /* 1 when the launch replayed a cached hipGraph     */
+} scirs_batch_info;
+int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *nrhs, const unsigned char *spd,
+                                const double *dA, const int64_t *offA, const double *dB, const int64_t *offB,
+                                double *dX, int *status, double *bwd, int max_iter, double tol_scale, void *stream,
+                                scirs_batch_info *info);
+
 /* ---- diagnostics -------------------------------------------------------------- */
 /* Kernel-class timing with HIP events on the launching stream (no reference
  * counterpart; the reference's profiling layer, scirs2-core/src/profiling.rs,

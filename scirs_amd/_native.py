@@ -51,6 +51,18 @@ class ProfStats(ctypes.Structure):
     ]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("failed", ctypes.c_int32),
+        ("max_iters", ctypes.c_int32),
+        ("max_backward_error", ctypes.c_double),
+        ("time_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("wave_max", ctypes.c_int32),
+        ("graph_replayed", ctypes.c_int32),
+    ]
+
+
 PROF_CLASSES = ["panel", "laswp", "trsm", "update", "trsv", "residual", "convert", "other"]
 
 _P = ctypes.c_void_p
@@ -74,6 +86,8 @@ SIGNATURES = {
     "scirs_gpu_dgesv_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P, _P, _INFO], _I),
     "scirs_gpu_dposv_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P, _INFO], _I),
     "scirs_gpu_dgesv_ir_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _I, _D, _P, _INFO], _I),
+    "scirs_gpu_dgesv_batched_dev": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _D, _P,
+                                     ctypes.POINTER(BatchInfo)], _I),
     "scirs_gpu_prof_begin": ([], _I),
     "scirs_gpu_prof_end": ([ctypes.POINTER(ProfStats)], _I),
     "scirs_gpu_backward_error_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, ctypes.POINTER(ctypes.c_double)], _I),

@@ -142,6 +142,7 @@ int Context::init(int dev)
     work.abort = (int *)(flags + 8);
     work.info = (int *)(flags + 12);
     work.info_val = (double *)(flags + 16);
+    work.ticket = (int *)(flags + 32);
     work.pairs = (int *)take(pairs_b);
     work.npairs = (int *)take(npairs_b);
     work.seq_next = 0;
@@ -171,7 +172,7 @@ int Context::read_flags(hipStream_t s, int *info, double *info_val)
     memcpy(&v, (char *)host_flags + 8, 8);
     if (info_val) *info_val = v;
     if (abort_flag) {
-        scirs_set_error("a cooperative panel kernel timed out waiting for its exchange");
+        scirs_set_error("a cooperative kernel (panel or chained triangular solve) timed out waiting for its exchange");
         return SCIRS_ERR_TIMEOUT;
     }
     return SCIRS_OK;
@@ -257,20 +258,43 @@ extern "C" int scirs_gpu_init(int device)
     return ctx().init(device < 0 ? 0 : device);
 }
 
+int scirs::Context::ensure_xtri(int n)
+{
+    const size_t words = (size_t)std::max(n, 1) * 4 * 2;
+    if (words <= work.xtri_words && xtri_mem) return SCIRS_OK;
+    int rc = ensure(&xtri_mem, &xtri_bytes, words * sizeof(unsigned long long));
+    if (rc) return rc;
+    // tag 0 is never a live sequence number, so zeroed words never match
+    SCIRS_HIP_CHECK(hipMemset(xtri_mem, 0, words * sizeof(unsigned long long)));
+    work.xtri = (unsigned long long *)xtri_mem;
+    work.xtri_words = words;
+    return SCIRS_OK;
+}
+
 // ------------------------------------------------------------------ LU core
-// X <- U^-1 L^-1 X (X already row-permuted): 256-row block substitution, or the
-// 64-row launch-per-block solver when SCIRS_GPU_TRSV=blocked.
+// X <- U^-1 L^-1 X (X already row-permuted): the chained one-launch solver by
+// default; SCIRS_GPU_TRSV=big selects the 256-row two-launches-per-block
+// solver, SCIRS_GPU_TRSV=blocked the 64-row one.
 template <typename T>
-static int lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s)
+int scirs::lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s)
 {
     Context &c = ctx();
     const char *mode = getenv("SCIRS_GPU_TRSV");
     if (mode && strcmp(mode, "blocked") == 0)
         return scirs_getrs_device<T>(n, nrhs, dLU, ldlu, dX, ldx, c.work.info, s);
-    int rc = scirs_trsv_big<T>(n, nrhs, dLU, ldlu, dX, ldx, 1, 1, c.work.info, s);
+    int rc;
+    if (mode && strcmp(mode, "big") == 0) {
+        rc = scirs_trsv_big<T>(n, nrhs, dLU, ldlu, dX, ldx, 1, 1, c.work.info, s);
+        if (rc) return rc;
+        return scirs_trsv_big<T>(n, nrhs, dLU, ldlu, dX, ldx, 0, 0, c.work.info, s);
+    }
+    if ((rc = c.ensure_xtri(n))) return rc;
+    rc = scirs_trsv_chain<T>(n, nrhs, dLU, ldlu, dX, ldx, 1, 1, c.work, s);
     if (rc) return rc;
-    return scirs_trsv_big<T>(n, nrhs, dLU, ldlu, dX, ldx, 0, 0, c.work.info, s);
+    return scirs_trsv_chain<T>(n, nrhs, dLU, ldlu, dX, ldx, 0, 0, c.work, s);
 }
+template int scirs::lu_substitute<float>(int, int, const float *, long, float *, long, hipStream_t);
+template int scirs::lu_substitute<double>(int, int, const double *, long, double *, long, hipStream_t);
 
 template <typename T>
 static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm, const T *dB, long ldb, T *dX,
@@ -588,7 +612,12 @@ extern "C" int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, co
                                      hipMemcpyHostToDevice, s));
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(dX, nrhs * sizeof(double), b, ldb * sizeof(double), nrhs * sizeof(double), n,
                                      hipMemcpyHostToDevice, s));
-    if ((rc = scirs_trsv_device<double>(n, nrhs, dA, n, dX, nrhs, lower, unit_diagonal, nullptr, s))) return finish(info, rc, t0);
+    if ((rc = c.ensure_xtri(n))) return finish(info, rc, t0);
+    if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
+    if ((rc = scirs_trsv_chain<double>(n, nrhs, dA, n, dX, nrhs, lower, unit_diagonal, c.work, s)))
+        return finish(info, rc, t0);
+    int st = 0;
+    if ((rc = c.read_flags(s, &st, nullptr))) return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(double), dX, nrhs * sizeof(double), nrhs * sizeof(double), n,
                                      hipMemcpyDeviceToHost, s));
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));

@@ -26,12 +26,19 @@ struct Context {
     void *scratch_bwd = nullptr; size_t scratch_bwd_bytes = 0;
     void *scratch_ir = nullptr; size_t scratch_ir_bytes = 0;
     void *scratch_ir2 = nullptr; size_t scratch_ir2_bytes = 0;
+    void *xtri_mem = nullptr; size_t xtri_bytes = 0;
 
     int init(int dev);
     int ensure(void **p, size_t *have, size_t bytes);
     int reset_flags(hipStream_t s);
     int read_flags(hipStream_t s, int *info, double *info_val);
+    // exchange buffer of the chained triangular solve for systems of order n
+    int ensure_xtri(int n);
 };
+
+// X <- U^-1 L^-1 X with the packed LU factors (X already row-permuted).
+template <typename T>
+int lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s);
 
 Context &ctx();
 double now_ms();

@@ -159,10 +159,7 @@ struct IrState {
 // d32 <- U^-1 L^-1 d32 with the FP32 factors
 int lu_solve_f32(IrState &st, float *d32)
 {
-    Context &c = ctx();
-    int rc = scirs_trsv_big<float>(st.n, 1, st.LU, st.n, d32, 1, 1, 1, c.work.info, st.s);
-    if (rc) return rc;
-    return scirs_trsv_big<float>(st.n, 1, st.LU, st.n, d32, 1, 0, 0, c.work.info, st.s);
+    return scirs::lu_substitute<float>(st.n, 1, st.LU, st.n, d32, 1, st.s);
 }
 
 // apply z = (LU)^-1 P v in FP32 for one vector v (FP64), result FP64; scale keeps FP32 in range

@@ -25,6 +25,9 @@ struct ScirsWork {
     unsigned spin_limit;
     int lowp_mode;       // trailing-update arithmetic of the FP32 factorisation (SCIRS_LOWP_*)
     int panel_rows;      // target rows per workgroup in the cooperative panel
+    unsigned long long *xtri;  // chained triangular solve: sequence-tagged x values
+    size_t xtri_words;
+    int *ticket;         // chained triangular solve: block-row ticket counter
 };
 
 enum {
@@ -65,6 +68,12 @@ int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const 
 template <typename T>
 int scirs_trsv_big(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, const int *info,
                    hipStream_t s);
+
+// Chained triangular solve in one launch (64-row block rows handed over through
+// w.xtri, which must hold n * 4 * 2 words): X <- op(A)^-1 X as scirs_trsv_big.
+template <typename T>
+int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
+                     hipStream_t s);
 
 template <typename T>
 int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);

@@ -15,17 +15,21 @@
 // they need carries the expected number; polls are bounded and raise *abort
 // when exceeded, so no wave can wait forever.
 //
-// Latency hiding: right after the pivot row of column j arrives, a workgroup
-// updates only column j+1, finds its next candidate, brings that row (and row
-// j+1 when it holds it) fully up to date, publishes, and only then performs
-// the bulk rank-1 update of its other rows, which overlaps the other
-// workgroups' publication latency.
+// Latency hiding: (1) the poll of the candidate list also reads the row of
+// the best candidate seen so far, so when the last candidate arrives the pivot
+// row is usually already in registers — one memory round trip per column
+// instead of two; (2) right after the pivot row arrives, a workgroup computes
+// the multipliers, updates column j+1 and finds its next candidate in one
+// pass, brings that row (and row j+1 when it holds it) fully up to date and
+// publishes; (3) the bulk rank-1 update of its other rows runs on waves 1-3
+// while wave 0 already polls the next column.
 #pragma once
 #include "common.hpp"
 
 namespace panel {
 
 constexpr int NT = 256;
+constexpr int SCIRS_PANEL_KB_MAX = 256;   // widest panel (the launcher rejects wider)
 typedef unsigned long long u64;
 
 __device__ __forceinline__ void st_agent(u64 *p, u64 v)
@@ -85,18 +89,19 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
 {
     if (*a.info != 0) return;
     constexpr int W = Tag<T>::W;
+    constexpr int SPEC = SCIRS_PANEL_KB_MAX / 64;   // pivot-row values per lane of wave 0
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int kb = a.kb, kbp = kb + 1;
     T *tile = reinterpret_cast<T *>(smem_raw);        // R x kbp
-    T *prow = tile + (size_t)a.R * kbp;               // kb  pivot row
-    T *rjv = prow + kb;                               // kb  row j (for the owner of the pivot row)
+    T *prow2 = tile + (size_t)a.R * kbp;              // [2][kb] pivot row, by column parity
+    T *rjv = prow2 + 2 * kb;                          // kb  row j (for the owner of the pivot row)
     T *lcol = rjv + kb;                               // R   multipliers
     int *top_label = reinterpret_cast<int *>(lcol + a.R);   // kb (workgroup 0)
     int *out_pos = top_label + kb;                    // kb
     int *out_label = out_pos + kb;                    // kb
     __shared__ T red_v[NT / 64];
     __shared__ int red_i[NT / 64];
-    __shared__ int s_p, s_g, s_stop, s_c, s_nout, s_bad;
+    __shared__ int s_p, s_g, s_stop, s_c, s_nout, s_bad, s_fetched;
     __shared__ T s_pv, s_cv;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         }
         __syncthreads();
     };
-    // publish the candidate of column j (rows are fully up to date for column j)
+    // publish the candidate of column j (its row is fully up to date for column j)
     auto publish = [&](int j) {
         const int b = j & 1;
         const unsigned seq = a.seq_base + (unsigned)j + 1u;
@@ -159,14 +164,20 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
     for (int j = 0; j < kb; ++j) {
         const int b = j & 1;
         const unsigned seq = a.seq_base + (unsigned)j + 1u;
-        // ---- 1. every workgroup reduces the same candidate list
+        T *prow = prow2 + b * kb;
+        // ---- 1. wave 0 reduces the candidate list (the same in every workgroup) and,
+        //         in the same polls, fetches the current best candidate's row and row j,
+        //         so the pivot row usually arrives with the last candidate.  Waves 1-3
+        //         are meanwhile finishing the previous column's bulk update.
         if (wave == 0) {
             unsigned spins = 0;
-            T v = (T)0; int i = -1, wg = -1;
-            bool timed_out = false;
+            T vv = (T)0; int ii = -1, wgw = -1;
+            bool timed_out = false, row_ok = false;
+            T sp[SPEC], sj[SPEC];
+            const u64 *xj = a.xrj + (size_t)b * kb * W;
             while (true) {
                 bool ok = true;
-                v = (T)0; i = -1; wg = -1;
+                T v = (T)0; int i = -1, wg = -1;
                 for (int q = lane; q < G; q += 64) {
                     const u64 *xw = a.xw + ((size_t)b * G + q) * 4;
                     const u64 w0 = ld_agent(xw);
@@ -176,20 +187,39 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                     const int qi = (int)(unsigned)w0;
                     if (qi >= 0 && (i < 0 || qv > v || (qv == v && qi < i))) { v = qv; i = qi; wg = q; }
                 }
-                if (__all(ok)) break;
+                vv = v; ii = i;
+                wave_argmax(vv, ii);
+                const unsigned long long ball = __ballot(i == ii && i >= 0);
+                const int src = ball ? __ffsll(ball) - 1 : 0;
+                wgw = __shfl(wg, src, 64);
+                bool rok = wgw >= 0;
+                if (wgw >= 0) {
+                    const u64 *xr = a.xrow + ((size_t)b * G + wgw) * (size_t)kb * W;
+#pragma unroll
+                    for (int u = 0; u < SPEC; ++u) {
+                        const int q = lane + 64 * u;
+                        if (q < kb) {
+                            rok = Tag<T>::get(xr + (size_t)q * W, seq, sp[u]) && rok;
+                            rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
+                        }
+                    }
+                }
+                if (__all(ok)) { row_ok = __all(rok); break; }
                 if (++spins > a.spin_limit ||
                     __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     timed_out = true;
                     break;
                 }
             }
-            T vv = v; int ii = i;
-            wave_argmax(vv, ii);
-            const unsigned long long ball = __ballot(i == ii && i >= 0);
-            const int src = ball ? __ffsll(ball) - 1 : 0;
-            const int wgw = __shfl(wg, src, 64);
+            if (row_ok) {
+#pragma unroll
+                for (int u = 0; u < SPEC; ++u) {
+                    const int q = lane + 64 * u;
+                    if (q < kb) { prow[q] = sp[u]; rjv[q] = sj[u]; }
+                }
+            }
             if (lane == 0) {
-                s_p = ii; s_g = wgw; s_pv = vv;
+                s_p = ii; s_g = wgw; s_pv = vv; s_fetched = row_ok ? 1 : 0;
                 if (timed_out) {
                     __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     s_stop = 1;
@@ -204,11 +234,10 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         const int p = s_p, sg = s_g;
         const bool own_j = (j >= row0 && j < row0 + Rg);
         const bool own_p = (p >= row0 && p < row0 + Rg);
-        // ---- 2. fetch the pivot row (and row j when this workgroup holds row p)
-        {
+        // ---- 2. the pivot row (and row j) when the speculative fetch missed
+        if (!s_fetched) {
             const u64 *xr = a.xrow + ((size_t)b * G + sg) * (size_t)kb * W;
             const u64 *xj = a.xrj + (size_t)b * kb * W;
-            const bool need_j = own_p && p != j;
             unsigned spins = 0;
             while (true) {
                 if (t == 0) s_bad = 0;
@@ -218,10 +247,8 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                     T v;
                     if (!Tag<T>::get(xr + (size_t)q * W, seq, v)) bad = true;
                     prow[q] = v;
-                    if (need_j) {
-                        if (!Tag<T>::get(xj + (size_t)q * W, seq, v)) bad = true;
-                        rjv[q] = v;
-                    }
+                    if (!Tag<T>::get(xj + (size_t)q * W, seq, v)) bad = true;
+                    rjv[q] = v;
                 }
                 if (bad) s_bad = 1;
                 __syncthreads();
@@ -258,20 +285,37 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
             }
         }
         __syncthreads();
-        // ---- 4. multipliers by true division (lapack.rs:138)
+        // ---- 4. multipliers by true division (lapack.rs:138), column j+1, and this
+        //         thread's arg-max of column j+1, in one pass over the rows
         const T pivot = prow[j];
         const int r1 = max(0, j + 1 - row0);
+        const bool more = j + 1 < kb;
+        const T u1 = more ? prow[j + 1] : (T)0;
+        T bv = (T)0; int bi = -1;
         for (int r = r1 + t; r < Rg; r += NT) {
             const T l = tile[r * kbp + j] / pivot;
             tile[r * kbp + j] = l;
             lcol[r] = l;
+            if (more) {
+                const T x = tile[r * kbp + j + 1] - l * u1;
+                tile[r * kbp + j + 1] = x;
+                if (bi < 0 || fabs(x) > bv) { bv = fabs(x); bi = r; }
+            }
+        }
+        if (!more) break;
+        wave_argmax(bv, bi);
+        if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+        __syncthreads();
+        if (t == 0) {
+            T v = red_v[0]; int i = red_i[0];
+            for (int w = 1; w < NT / 64; ++w) {
+                const int oi = red_i[w]; const T ov = red_v[w];
+                if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) { v = ov; i = oi; }
+            }
+            s_c = i; s_cv = v;
         }
         __syncthreads();
-        if (j + 1 >= kb) break;
-        // ---- 5. column j+1 first, next candidate, publish
-        for (int r = r1 + t; r < Rg; r += NT) tile[r * kbp + j + 1] -= lcol[r] * prow[j + 1];
-        __syncthreads();
-        local_candidate(j + 1, r1);
+        // ---- 5. the next candidate row and row j+1 brought fully up to date, then published
         const int c = s_c;
         const int jn = (j + 1 >= row0 && j + 1 < row0 + Rg) ? j + 1 - row0 : -1;
         for (int q = j + 2 + t; q < kb; q += NT) {
@@ -280,19 +324,22 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         }
         __syncthreads();
         publish(j + 1);
-        // ---- 6. bulk rank-1 update of the remaining rows (overlaps the exchange)
-        if (j + 2 < kb) {
-            // TPR threads per row, consecutive threads on consecutive columns
-            constexpr int TPR = 8;
-            for (int r = r1 + t / TPR; r < Rg; r += NT / TPR) {
+        // ---- 6. bulk rank-1 update of the other rows by waves 1-3, overlapping the
+        //         exchange; wave 0 goes straight to the next column's poll.  The next
+        //         column writes the other parity of prow2, and its first barrier
+        //         orders this update before any use of the tile.
+        if (wave > 0 && j + 2 < kb) {
+            constexpr int TPR = 8;   // threads per row, consecutive threads on consecutive columns
+            const int tt = t - 64;
+            for (int r = r1 + tt / TPR; r < Rg; r += (NT - 64) / TPR) {
                 if (r == c || r == jn) continue;
                 const T l = lcol[r];
                 T *row = tile + r * kbp;
-                for (int q = j + 2 + (t % TPR); q < kb; q += TPR) row[q] -= l * prow[q];
+                for (int q = j + 2 + (tt % TPR); q < kb; q += TPR) row[q] -= l * prow[q];
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     for (int r = wave; r < Rg; r += NT / 64)
         for (int c = lane; c < kb; c += 64) Ap[(long)(row0 + r) * a.lda + c] = tile[r * kbp + c];
@@ -314,7 +361,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
 
 inline size_t lds_bytes(int R, int kb, size_t esz)
 {
-    return ((size_t)R * (kb + 1) + 2 * (size_t)kb + R) * esz + 3 * (size_t)kb * sizeof(int);
+    return ((size_t)R * (kb + 1) + 3 * (size_t)kb + R) * esz + 3 * (size_t)kb * sizeof(int);
 }
 
 template <typename T>
@@ -322,7 +369,7 @@ inline int max_rows(int kb)
 {
     const size_t budget = 160 * 1024 - 1024;   // static reduction buffers
     const size_t per_row = (size_t)(kb + 2) * sizeof(T);
-    const size_t fixed = 2 * (size_t)kb * sizeof(T) + 3 * (size_t)kb * sizeof(int);
+    const size_t fixed = 3 * (size_t)kb * sizeof(T) + 3 * (size_t)kb * sizeof(int);
     return (int)((budget - fixed) / per_row);
 }
 

@@ -1,0 +1,293 @@
+// Chained triangular solve: X <- T^-1 X for T the unit-lower or upper triangle
+// of a row-major matrix, in ONE launch.
+//
+// Same arithmetic as the substitution loops of scirs2-linalg/src/solve.rs:159-199
+// (true division by the diagonal in FP64), with the k-sum split into 64-column
+// blocks.  The matrix is cut into 64-row block rows; block row p is solved by one
+// workgroup once the x values of every block it depends on are known.
+//
+//   * Workgroups take block rows in dependency order from an atomic ticket, so
+//     every block a workgroup waits for is held by a workgroup that is already
+//     running (or done): no deadlock whatever the grid size or dispatch order.
+//   * A finished block publishes its x values as sequence-tagged 64-bit words
+//     (relaxed agent-scope atomics; a value is current when its word carries
+//     this launch's number), which later block rows poll: no fences, no flags,
+//     one memory round trip per hand-over.  Polls are bounded and raise *abort.
+//   * Waves 1-3 stream the off-diagonal tiles of their block row from HBM as
+//     the x blocks arrive (each lane: 4 consecutive columns of 16 rows, so a
+//     load instruction covers whole 256-byte row segments) and keep per-lane
+//     partial sums.  Wave 0 stages the diagonal block and the LAST off-diagonal
+//     tile in LDS, and after a single barrier does the only work on the
+//     critical path: that last tile's product (one row per lane, x broadcast by
+//     readlane) and the 64-step substitution.
+//
+// Algorithmic traffic per solve: the triangle once (n^2/2 elements) plus the
+// right-hand sides; the chain is bounded by ~n/64 hand-overs.
+#include "api_internal.hpp"
+#include "panel.hpp"
+#include "prof.hpp"
+
+namespace {
+
+using panel::Tag;
+using panel::u64;
+using panel::ld_agent;
+using panel::st_agent;
+
+constexpr int TB = 64;        // rows per block row
+constexpr int NTC = 256;      // 4 waves
+constexpr int LDP = TB + 1;   // LDS pitch of staged tiles
+
+template <typename T>
+__device__ __forceinline__ T readlane(T v, int k);
+template <>
+__device__ __forceinline__ float readlane<float>(float v, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+template <>
+__device__ __forceinline__ double readlane<double>(double v, int k)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Poll until every active lane holds the QB tagged values at src (stride W
+// words per value).  Gives up (returns false) after spin_limit polls or when
+// another workgroup raised *abort.
+template <typename T, int QB>
+__device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB], unsigned spin_limit, int *abort)
+{
+    constexpr int W = Tag<T>::W;
+    unsigned spins = 0;
+    while (true) {
+        bool ok = true;
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < QB; ++q) ok = Tag<T>::get(src + q * W, seq, v[q]) && ok;
+        } else {
+#pragma unroll
+            for (int q = 0; q < QB; ++q) v[q] = (T)0;
+        }
+        if (__all(ok)) return true;
+        ++spins;
+        if (spins > spin_limit) {
+            if ((threadIdx.x & 63) == 0) __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        if ((spins & 255u) == 0 &&
+            __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template <typename T, int QB, bool LOWER, bool UNIT>
+__global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A, long lda, T *X, long ldx, int n,
+                                                         int nr, u64 *xch, unsigned seq, int *ticket, int *abort,
+                                                         unsigned spin_limit, const int *info)
+{
+    if (info && *info != 0) return;
+    constexpr int W = Tag<T>::W;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *Ld = reinterpret_cast<T *>(smem_raw);   // [TB][LDP] diagonal block
+    T *Lt = Ld + TB * LDP;                      // [TB][LDP] last off-diagonal tile
+    T *part = Lt + TB * LDP;                    // [3][TB][QB] partial sums of waves 1-3
+    __shared__ int s_ticket;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t == 0) s_ticket = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int b = s_ticket;                      // position in dependency order
+    const int nblk = (n + TB - 1) / TB;
+    const int p = LOWER ? b : nblk - 1 - b;      // block row solved here
+    const int r0 = p * TB, bs = min(TB, n - r0);
+    bool failed = false;
+
+    if (wave > 0) {
+        // ---- off-diagonal tiles of dependencies 0 .. b-2, round robin over waves 1-3
+        const int g = lane >> 4, c = (lane & 15) * 4;
+        T acc[16][QB];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+#pragma unroll
+            for (int q = 0; q < QB; ++q) acc[m][q] = (T)0;
+        for (int j = wave - 1; j < b - 1; j += 3) {
+            const int cb = LOWER ? j : nblk - 1 - j;
+            const int c0 = cb * TB;
+            T a[16][4];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int row = 4 * m + g;
+                const T *src = A + (long)(r0 + row) * lda + c0 + c;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) a[m][e] = (row < bs && c0 + c + e < n) ? src[e] : (T)0;
+            }
+            T xv[4][QB];
+            bool ok = true;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int col = c0 + c + e;
+                ok = ok && poll_vals<T, QB>(xch + (size_t)col * QB * W, col < n, seq, xv[e], spin_limit, abort);
+            }
+            if (!ok) { failed = true; break; }
+#pragma unroll
+            for (int m = 0; m < 16; ++m)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int q = 0; q < QB; ++q) acc[m][q] += a[m][e] * xv[e][q];
+        }
+        // reduce over the 16 lanes that share a row; lane (m) of each group keeps row 4m+g
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                T v = acc[m][q];
+#pragma unroll
+                for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+                if ((lane & 15) == m) part[((wave - 1) * TB + 4 * m + g) * QB + q] = v;
+            }
+    } else {
+        // ---- wave 0: stage the diagonal block and the last dependency's tile (coalesced rows)
+        for (int i = 0; i < TB; ++i)
+            Ld[i * LDP + lane] = (i < bs && lane < bs) ? A[(long)(r0 + i) * lda + r0 + lane] : (T)0;
+        if (b > 0) {
+            const int cb = LOWER ? b - 1 : nblk - b;
+            const int c0 = cb * TB, cs = min(TB, n - c0);
+            for (int i = 0; i < TB; ++i)
+                Lt[i * LDP + lane] = (i < bs && lane < cs) ? A[(long)(r0 + i) * lda + c0 + lane] : (T)0;
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+
+    // ---- critical path (wave 0, lane = row of the block)
+    T s[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        s[q] = (lane < bs && q < nr) ? X[(long)(r0 + lane) * ldx + q] : (T)0;
+        if (b > 1) s[q] -= part[lane * QB + q] + part[(TB + lane) * QB + q] + part[(2 * TB + lane) * QB + q];
+    }
+    if (b > 0) {
+        const int cb = LOWER ? b - 1 : nblk - b;
+        const int c0 = cb * TB;
+        T xk[QB];
+        if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort))
+            failed = true;
+        if (failed) return;
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            const T l = Lt[lane * LDP + k];
+#pragma unroll
+            for (int q = 0; q < QB; ++q) s[q] -= l * readlane(xk[q], k);
+        }
+    }
+    if (failed) return;
+    if (LOWER) {
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            if (k < bs) {
+                const T l = Ld[lane * LDP + k];
+                const T d = Ld[k * LDP + k];
+#pragma unroll
+                for (int q = 0; q < QB; ++q) {
+                    T x = readlane(s[q], k);
+                    if (!UNIT) x = x / d;
+                    if (lane == k) s[q] = x;
+                    else if (lane > k) s[q] -= l * x;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = TB - 1; k >= 0; --k) {
+            if (k < bs) {
+                const T l = Ld[lane * LDP + k];
+                const T d = Ld[k * LDP + k];
+#pragma unroll
+                for (int q = 0; q < QB; ++q) {
+                    T x = readlane(s[q], k);
+                    if (!UNIT) x = x / d;
+                    if (lane == k) s[q] = x;
+                    else if (lane < k) s[q] -= l * x;
+                }
+            }
+        }
+    }
+    if (lane < bs) {
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            if (q < nr) X[(long)(r0 + lane) * ldx + q] = s[q];
+            Tag<T>::put(xch + ((size_t)(r0 + lane) * QB + q) * W, seq, s[q]);
+        }
+    }
+}
+
+template <typename T, int QB, bool LOWER, bool UNIT>
+int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork &w, hipStream_t s)
+{
+    const auto kern = trsv_chain_kernel<T, QB, LOWER, UNIT>;
+    const size_t lds = (2 * (size_t)TB * LDP + 3 * (size_t)TB * QB) * sizeof(T);
+    if (lds > 64 * 1024) {
+        int rc = scirs_allow_lds((const void *)kern, lds);
+        if (rc) return rc;
+    }
+    int rc = scirs_fill_int(w.ticket, 1, 0, s);
+    if (rc) return rc;
+    const unsigned seq = w.seq_next + 1u;
+    w.seq_next += 512u;
+    const int nblk = (n + TB - 1) / TB;
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(NTC), lds, s, A, lda, X, ldx, n, nr, w.xtri, seq, w.ticket, w.abort,
+                       w.spin_limit, w.info);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+
+template <typename T, int QB>
+int chain_dispatch(int n, int nr, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
+                   hipStream_t s)
+{
+    if (lower && unit) return launch_chain<T, QB, true, true>(n, nr, A, lda, X, ldx, w, s);
+    if (lower) return launch_chain<T, QB, true, false>(n, nr, A, lda, X, ldx, w, s);
+    if (unit) return launch_chain<T, QB, false, true>(n, nr, A, lda, X, ldx, w, s);
+    return launch_chain<T, QB, false, false>(n, nr, A, lda, X, ldx, w, s);
+}
+
+} // namespace
+
+template <typename T>
+int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
+                     hipStream_t s)
+{
+    if (n <= 0 || nrhs <= 0) return SCIRS_OK;
+    constexpr int QMAX = sizeof(T) == 8 ? 2 : 4;
+    if ((size_t)n * QMAX * Tag<T>::W > w.xtri_words) {
+        scirs_set_error("chained triangular solve: exchange buffer holds %zu words, n = %d needs %zu", w.xtri_words,
+                        n, (size_t)n * QMAX * Tag<T>::W);
+        return SCIRS_ERR_ARG;
+    }
+    scirs::ProfScope prof(SCIRS_PROF_TRSV, (double)n * n * nrhs, (0.5 * n * n + 2.0 * n * nrhs) * sizeof(T), s);
+    for (int rb = 0; rb < nrhs;) {
+        const int left = nrhs - rb;
+        int rc;
+        if (left == 1) {
+            rc = chain_dispatch<T, 1>(n, 1, A, lda, X + rb, ldx, lower, unit, w, s);
+            rb += 1;
+        } else if (QMAX == 2 || left == 2) {
+            rc = chain_dispatch<T, 2>(n, 2, A, lda, X + rb, ldx, lower, unit, w, s);
+            rb += 2;
+        } else {
+            const int nr = left < 4 ? left : 4;
+            rc = chain_dispatch<T, QMAX>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
+            rb += nr;
+        }
+        if (rc) return rc;
+    }
+    return SCIRS_OK;
+}
+
+template int scirs_trsv_chain<float>(int, int, const float *, long, float *, long, int, int, ScirsWork &, hipStream_t);
+template int scirs_trsv_chain<double>(int, int, const double *, long, double *, long, int, int, ScirsWork &,
+                                      hipStream_t);

@@ -198,3 +198,51 @@ def prof_end():
     _check("prof_end", rc, None)
     return {name: {"launches": int(a.launches), "total_ms": a.total_ms, "flops": a.flops, "bytes": a.bytes}
             for name, a in zip(N.PROF_CLASSES, arr)}
+
+
+# ---------------------------------------------------------------- batched
+def pack_batch(As, Bs):
+    """Host layout of a batch: packed FP64 buffers and per-system offsets."""
+    n = np.array([a.shape[0] for a in As], dtype=np.int32)
+    nrhs = np.array([b.reshape(b.shape[0], -1).shape[1] for b in Bs], dtype=np.int32)
+    offA = np.zeros(len(As), dtype=np.int64)
+    offB = np.zeros(len(As), dtype=np.int64)
+    offA[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    offB[1:] = np.cumsum(n.astype(np.int64) * nrhs)[:-1]
+    A = np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in As])
+    B = np.concatenate([np.ascontiguousarray(b, dtype=np.float64).reshape(b.shape[0], -1).ravel() for b in Bs])
+    return n, nrhs, offA, offB, A, B
+
+
+def gesv_batched_dev(n, nrhs, offA, offB, dA, dB, dX, spd=None, max_iter=0, tol_scale=0.0):
+    """Batched solve on device buffers (torch CUDA tensors dA, dB, dX); the
+    size / offset arrays are host numpy arrays.  Returns (status, bwd, info)."""
+    lib = N.load()
+    batch = len(n)
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+    offA = np.ascontiguousarray(offA, dtype=np.int64)
+    offB = np.ascontiguousarray(offB, dtype=np.int64)
+    spd_arr = None if spd is None else np.ascontiguousarray(spd, dtype=np.uint8)
+    status = np.zeros(batch, dtype=np.int32)
+    bwd = np.zeros(batch, dtype=np.float64)
+    info = N.BatchInfo()
+    rc = lib.scirs_gpu_dgesv_batched_dev(batch, _ptr(n), _ptr(nrhs), _ptr(spd_arr) if spd_arr is not None else None,
+                                         _tp(dA), _ptr(offA), _tp(dB), _ptr(offB), _tp(dX), _ptr(status),
+                                         _ptr(bwd), int(max_iter), float(tol_scale), _stream_ptr(),
+                                         ctypes.byref(info))
+    _check("gesv_batched_dev", rc, None)
+    return status, bwd, info
+
+
+def gesv_batched(As, Bs, spd=None, max_iter=0, tol_scale=0.0):
+    """Host convenience wrapper: lists of numpy systems in, list of solutions out."""
+    import torch
+    n, nrhs, offA, offB, A, B = pack_batch(As, Bs)
+    dA = torch.from_numpy(A).cuda()
+    dB = torch.from_numpy(B).cuda()
+    dX = torch.zeros_like(dB)
+    status, bwd, info = gesv_batched_dev(n, nrhs, offA, offB, dA, dB, dX, spd, max_iter, tol_scale)
+    X = dX.cpu().numpy()
+    xs = [X[offB[i]:offB[i] + n[i] * nrhs[i]].reshape(n[i], nrhs[i]) for i in range(len(As))]
+    return xs, status, bwd, info

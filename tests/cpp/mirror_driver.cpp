@@ -1,0 +1,132 @@
+// Test driver for include/scirs2_linalg.hpp: reads cases from stdin, calls the
+// C++ mirror of the scirs2-linalg API, and prints one result line per case.
+//
+// Input, whitespace separated, one case after another:
+//   <op> <type> <flag1> <flag2> <ar> <ac> <ar*ac values> <br> <bc> <br*bc values>
+//   op   : solve | solve_multiple | lu | cholesky | solve_triangular | det | inv
+//   type : f64 | f32
+//   flags: lower / unit_diagonal for solve_triangular, ignored otherwise
+//   b    : the right-hand side (br = length, bc = 1 for a vector; 0 0 when unused)
+//   values may be nan / inf / -inf
+// Output per case:
+//   OK <count> <arrays...>   each array as "<rows> <cols> <values>" (%.17g)
+//   ERR <Kind> <Display text with each newline written as <LF>>
+#include "scirs2_linalg.hpp"
+
+#include <cstdio>
+#include <iostream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace scirs2_linalg;
+
+static const char *kind_name(LinalgError::Kind k)
+{
+    switch (k) {
+    case LinalgError::Kind::ShapeError: return "ShapeError";
+    case LinalgError::Kind::InvalidInputError: return "InvalidInputError";
+    case LinalgError::Kind::SingularMatrixError: return "SingularMatrixError";
+    case LinalgError::Kind::NonPositiveDefiniteError: return "NonPositiveDefiniteError";
+    case LinalgError::Kind::ComputationError: return "ComputationError";
+    default: return "Other";
+    }
+}
+
+static void print_err(const LinalgError &e)
+{
+    std::string t = e.to_string(), o;
+    for (char ch : t) {
+        if (ch == '\n') o += "<LF>";
+        else o += ch;
+    }
+    std::printf("ERR %s %s\n", kind_name(e.kind()), o.c_str());
+}
+
+template <class F>
+static void put(std::size_t r, std::size_t c, const F *v)
+{
+    std::printf(" %zu %zu", r, c);
+    for (std::size_t i = 0; i < r * c; ++i) std::printf(" %.17g", (double)v[i]);
+}
+
+static double read_val(std::istream &in)
+{
+    std::string s;
+    in >> s;
+    return std::stod(s);   // accepts nan, inf, -inf
+}
+
+template <class F>
+static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size_t ac, const std::vector<double> &av,
+                std::size_t br, std::size_t bc, const std::vector<double> &bv)
+{
+    Array2<F> a(ar, ac), b(br, bc);
+    for (std::size_t i = 0; i < av.size(); ++i) a.data[i] = (F)av[i];
+    for (std::size_t i = 0; i < bv.size(); ++i) b.data[i] = (F)bv[i];
+    std::vector<F> bvec(b.data.begin(), b.data.end());
+    const std::optional<std::size_t> workers;
+    if (op == "solve") {
+        auto r = solve<F>(a.view(), view1(bvec), workers);
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().size(), 1, r.value().data());
+    } else if (op == "solve_triangular") {
+        auto r = solve_triangular<F>(a.view(), view1(bvec), f1 != 0, f2 != 0);
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().size(), 1, r.value().data());
+    } else if (op == "solve_multiple") {
+        auto r = solve_multiple<F>(a.view(), b.view(), workers);
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().rows, r.value().cols, r.value().data.data());
+    } else if (op == "lu") {
+        auto r = lu<F>(a.view(), workers);
+        if (r.is_err()) return print_err(r.error());
+        const auto &[p, l, u] = r.value();
+        std::printf("OK 3");
+        put(p.rows, p.cols, p.data.data());
+        put(l.rows, l.cols, l.data.data());
+        put(u.rows, u.cols, u.data.data());
+    } else if (op == "cholesky") {
+        auto r = cholesky<F>(a.view(), workers);
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().rows, r.value().cols, r.value().data.data());
+    } else if (op == "det") {
+        auto r = det<F>(a.view(), workers);
+        if (r.is_err()) return print_err(r.error());
+        F v = r.value();
+        std::printf("OK 1");
+        put(1, 1, &v);
+    } else if (op == "inv") {
+        auto r = inv<F>(a.view(), workers);
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().rows, r.value().cols, r.value().data.data());
+    } else {
+        std::printf("ERR Other unknown op %s\n", op.c_str());
+        return;
+    }
+    std::printf("\n");
+}
+
+int main()
+{
+    std::string op, type;
+    while (std::cin >> op >> type) {
+        int f1, f2;
+        std::size_t ar, ac, br, bc;
+        std::cin >> f1 >> f2 >> ar >> ac;
+        std::vector<double> av(ar * ac);
+        for (auto &v : av) v = read_val(std::cin);
+        std::cin >> br >> bc;
+        std::vector<double> bv(br * bc);
+        for (auto &v : bv) v = read_val(std::cin);
+        if (type == "f32") run<float>(op, f1, f2, ar, ac, av, br, bc, bv);
+        else run<double>(op, f1, f2, ar, ac, av, br, bc, bv);
+        std::fflush(stdout);
+    }
+    return 0;
+}

@@ -245,6 +245,27 @@ def test_trsv_matches_oracle(L, lower, unit):
     assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-12
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 1000])
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 7])
+def test_trsv_chain_sizes_and_rhs(L, n, nrhs):
+    """The one-launch chained solver: partial last block row (n % 64 != 0),
+    right-hand sides split into chunks of 2 (FP64), every triangle kind.
+    Bar: per column, |x - x_oracle| <= 1e-12 max|x_oracle| (well-conditioned
+    triangles: unit-ish diagonal, off-diagonal entries ~ 1/sqrt(n))."""
+    rng = np.random.default_rng(1000 * n + nrhs)
+    for lower in (True, False):
+        for unit in (True, False):
+            a = rng.standard_normal((n, n)) / np.sqrt(n)
+            a = np.tril(a) if lower else np.triu(a)
+            np.fill_diagonal(a, 1.0 + rng.random(n))
+            b = rng.standard_normal((n, nrhs))
+            print(f"trsv n={n} nrhs={nrhs} lower={lower} unit={unit}", flush=True)
+            x, _ = L.trsv(a, b, lower, unit)
+            for j in range(nrhs):
+                xo = oracle.solve_triangular(a, np.ascontiguousarray(b[:, j]), lower, unit)
+                assert np.abs(x[:, j] - xo).max() <= 1e-12 * np.abs(xo).max(), (lower, unit, j)
+
+
 def test_trsv_singular_diagonal(L):
     a = np.array([[1.0, 0.0, 0.0], [2.0, 1e-20, 0.0], [1.0, 1.0, 1.0]])
     with pytest.raises(L.SingularError) as eg:
