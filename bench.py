@@ -40,9 +40,54 @@ def parse():
     ap.add_argument("--mode", default=os.environ.get("SCIRS_BENCH_MODE", "bf16x3"), choices=sorted(MODES))
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 LU comparison line")
-    ap.add_argument("--cpu-n", type=int, default=2048, help="size of the CPU baseline sample")
+    ap.add_argument("--cpu-n", type=int, default=2560, help="size of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "batched"],
+                    help="mixed: the headline n=16384 mixed-precision solve; batched: 20,000 uneven "
+                         "systems in one launch against a per-matrix launch loop")
+    ap.add_argument("--batch", type=int, default=20000)
+    ap.add_argument("--loop-sample", type=int, default=1000,
+                    help="systems solved one launch sequence at a time for the per-matrix baseline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (scales rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size: reads and writes exactly mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "dispatches": 3}), flush=True)
+
+
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel_class):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3 PMC
+    summary (profiles/pmc_traffic.txt: lines 'class: fetch_bytes write_bytes'),
+    or None when that class was not measured."""
+    path = os.path.join(PROFILES, "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line.startswith("#") or ":" not in line:
+                continue
+            name, vals = line.split(":", 1)
+            if name.strip() == kernel_class:
+                parts = vals.split()
+                if len(parts) >= 2:
+                    return float(parts[0]) + float(parts[1])
+    return None
 
 
 def make_system(n, kappa, seed, torch):
@@ -93,12 +138,128 @@ def cpu_baseline(n_sample, seed):
     }
 
 
+def make_batch(count, seed, torch):
+    """count systems, n ~ U{8..256}; even index: A = U diag(logspace(0,3)) V^T,
+    odd index: SPD G G^T/n + I; 1 or 4 right-hand sides N(0,1).  Generated on the
+    GPU size group by size group and packed back to back (row-major)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = rng.integers(8, 257, size=count).astype(np.int32)
+    nrhs = np.where(rng.random(count) < 0.5, 1, 4).astype(np.int32)
+    spd = (np.arange(count) % 2).astype(np.uint8)
+    offA = np.zeros(count, dtype=np.int64)
+    offB = np.zeros(count, dtype=np.int64)
+    offA[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    offB[1:] = np.cumsum(n.astype(np.int64) * nrhs)[:-1]
+    totA = int((n.astype(np.int64) ** 2).sum())
+    totB = int((n.astype(np.int64) * nrhs).sum())
+    dA = torch.empty(totA, dtype=torch.float64, device="cuda")
+    dB = torch.empty(totB, dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    for size in np.unique(n):
+        idx = np.nonzero(n == size)[0]
+        for kind in (0, 1):
+            sel = idx[spd[idx] == kind]
+            if len(sel) == 0:
+                continue
+            m = int(size)
+            if kind == 0:
+                u, _ = torch.linalg.qr(torch.randn(len(sel), m, m, dtype=torch.float64, device="cuda", generator=g))
+                v, _ = torch.linalg.qr(torch.randn(len(sel), m, m, dtype=torch.float64, device="cuda", generator=g))
+                sig = torch.logspace(0.0, 3.0, m, dtype=torch.float64, device="cuda")
+                a = torch.bmm(u * sig, v.transpose(1, 2).contiguous())
+            else:
+                gm = torch.randn(len(sel), m, m, dtype=torch.float64, device="cuda", generator=g)
+                a = torch.bmm(gm, gm.transpose(1, 2).contiguous()) / m + torch.eye(m, dtype=torch.float64,
+                                                                                  device="cuda")
+            for k, i in enumerate(sel):
+                dA[offA[i]:offA[i] + m * m] = a[k].reshape(-1)
+    dB.copy_(torch.randn(totB, dtype=torch.float64, device="cuda", generator=g))
+    torch.cuda.synchronize()
+    return n, nrhs, spd, offA, offB, dA, dB
+
+
+def bench_batched(args):
+    """20,000 uneven systems in one persistent launch vs. a per-matrix launch loop."""
+    import numpy as np
+    import torch
+    from scirs_amd import linalg as L
+    n, nrhs, spd, offA, offB, dA, dB = make_batch(args.batch, args.seed, torch)
+    dX = torch.zeros_like(dB)
+    n64 = n.astype(np.float64)
+    flops = float(np.where(spd == 1, n64 ** 3 / 3.0, 2.0 * n64 ** 3 / 3.0).sum() + (2.0 * n64 ** 2 * nrhs).sum())
+    bytes_alg = float((16.0 * n64 ** 2 + 16.0 * n64 * nrhs).sum())
+
+    def one():
+        return L.gesv_batched_dev(n, nrhs, offA, offB, dA, dB, dX, spd)
+
+    for _ in range(max(1, args.warmup)):
+        one()
+    torch.cuda.synchronize()
+    t, outs = time_steps(one, args.steps, 0, torch)
+    status, bwd, info = outs[-1]
+    ok = status == 0
+    # per-matrix baseline: the same systems through the single-system entry points
+    # (LU solve or Cholesky solve, one launch sequence per system)
+    sample = min(args.loop_sample, args.batch)
+    views = []
+    for i in range(sample):
+        m, r = int(n[i]), int(nrhs[i])
+        views.append((dA[offA[i]:offA[i] + m * m].view(m, m), dB[offB[i]:offB[i] + m * r].view(m, r),
+                      torch.empty(m, r, dtype=torch.float64, device="cuda"), bool(spd[i])))
+
+    def loop():
+        for a_i, b_i, x_i, s_i in views:
+            if s_i:
+                L.posv_dev(a_i, b_i, x_i)
+            else:
+                L.gesv_dev(a_i, b_i, x_i)
+
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loop()
+    torch.cuda.synchronize()
+    t_loop = (time.perf_counter() - t0) / sample * args.batch
+    roof_bw = bytes_alg / (info.time_ms * 1e-3) / 1e9
+    line = {
+        "metric": "batched uneven solves/s (FP32 factor + FP64 refinement, one persistent launch)",
+        "value": args.batch / t,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.batch} systems, n~U[8,256], kappa<=1e3 (general) / G G^T/n+I (SPD, "
+                               f"half), 1 or 4 rhs", "seed": args.seed},
+        "tflops": flops / t / 1e12,
+        "failed": int((~ok).sum()),
+        "max_backward_error": float(bwd[ok].max()) if ok.any() else None,
+        "max_refinement_steps": int(info.max_iters),
+        "wave_max": int(info.wave_max),
+        "graph_replayed": int(info.graph_replayed),
+        "device_ms": float(info.time_ms),
+        "per_matrix_loop": {"solves_per_s": args.batch / t_loop, "sample": sample,
+                            "note": "gesv_dev / posv_dev per system, timed on the first "
+                                    f"{sample} systems and scaled to {args.batch}"},
+        "speedup_vs_loop": t_loop / t,
+        "roofline": {"bound": "hbm", "kernel": "batched_solve_kernel", "achieved": roof_bw,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": roof_bw / PEAK_HBM_GBS, "traffic": None,
+                     "algorithmic_bytes": bytes_alg},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve engine does not shard "
               "across GPUs. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "batched":
+        bench_batched(args)
+        return
     import torch
     from scirs_amd import linalg as L
 
@@ -162,7 +323,8 @@ def main():
         achieved = d["bytes"] / d["launches"] / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(dom)
+    roof["algorithmic_bytes_per_launch"] = d["bytes"] / max(1, d["launches"])
     roof["avg_launch_ms"] = avg_ms
     roof["launches_per_step"] = d["launches"] / args.steps
 

@@ -116,6 +116,11 @@ int Context::init(int dev)
         return SCIRS_ERR_UNSUPPORTED;
     }
     SCIRS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // side stream of the look-ahead LU (default priority, like every stream here)
+    SCIRS_HIP_CHECK(hipStreamCreateWithFlags(&work.side_stream, hipStreamNonBlocking));
+    SCIRS_HIP_CHECK(hipEventCreateWithFlags(&work.ev_trsm, hipEventDisableTiming));
+    SCIRS_HIP_CHECK(hipEventCreateWithFlags(&work.ev_upd, hipEventDisableTiming));
+    SCIRS_HIP_CHECK(hipEventCreateWithFlags(&work.ev_start, hipEventDisableTiming));
     // factorisation workspace
     const int max_g = SCIRS_MAX_G;
     const int kbmax = SCIRS_KB_MAX;
@@ -124,12 +129,14 @@ int Context::init(int dev)
     work.spin_limit = (unsigned)env_int("SCIRS_GPU_SPIN_LIMIT", 1 << 22);
     work.lowp_mode = SCIRS_LOWP_FP32;   // plain FP32 entry points compute in FP32; the mixed solve switches it
     work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 128);
+    work.panel_w0 = env_int("SCIRS_GPU_PANEL_W0", 0);
+    work.lookahead = env_int("SCIRS_GPU_LOOKAHEAD", 1);
     size_t bytes = 0;
     const size_t xw_b = 2 * (size_t)max_g * 4 * 8, xrow_b = 2 * (size_t)max_g * kbmax * 2 * 8,
                  xrj_b = 2 * (size_t)kbmax * 2 * 8, flags_b = 64 * sizeof(int),
                  pairs_b = (size_t)work.max_panels * 4 * kbmax * sizeof(int),
-                 npairs_b = (size_t)work.max_panels * sizeof(int);
-    for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b}) bytes += (b + 255) & ~(size_t)255;
+                 npairs_b = (size_t)work.max_panels * sizeof(int), linv_b = 128 * 128 * sizeof(double);
+    for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b, linv_b}) bytes += (b + 255) & ~(size_t)255;
     char *base = nullptr;
     SCIRS_HIP_CHECK(hipMalloc((void **)&base, bytes));
     SCIRS_HIP_CHECK(hipMemset(base, 0, bytes));
@@ -145,6 +152,7 @@ int Context::init(int dev)
     work.ticket = (int *)(flags + 32);
     work.pairs = (int *)take(pairs_b);
     work.npairs = (int *)take(npairs_b);
+    work.linv = take(linv_b);
     work.seq_next = 0;
     work_base = base;
     SCIRS_HIP_CHECK(hipHostMalloc((void **)&host_flags, 64, hipHostMallocDefault));

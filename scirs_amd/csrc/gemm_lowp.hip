@@ -53,7 +53,7 @@ struct LowpArgs {
     int a_shift;         // FP16: A is multiplied by 2^a_shift before rounding
 };
 
-template <int MODE, bool TRANS_B>
+template <int MODE, bool TRANS_B, bool VEC>
 __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g)
 {
     if (g.info && *g.info != 0) return;
@@ -77,27 +77,29 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
     float ra[16], rb[16];
     const float ascale = (MODE == SCIRS_LOWP_FP16) ? __builtin_ldexpf(1.0f, g.a_shift) : 1.0f;
 
-    auto load_slab = [&](int k0) {
-        const int gr = row0 + a_r;
+    // 16 consecutive floats of one row: four 16-byte loads when the row segment
+    // is in range and aligned (VEC: every leading dimension a multiple of 4)
+    auto load16 = [&](float (&dst)[16], const float *src, bool row_ok, int first, int limit) {
+        if (VEC && row_ok && first + 15 < limit) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int gk = k0 + a_k + i;
-            ra[i] = (gr < g.M && gk < g.K) ? g.A[(long)gr * g.lda + gk] : 0.0f;
-        }
-        if (!TRANS_B) {
-            const int gk = k0 + b_k;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int gc = col0 + b_c + i;
-                rb[i] = (gk < g.K && gc < g.N) ? g.B[(long)gk * g.ldb + gc] : 0.0f;
+            for (int v = 0; v < 4; ++v) {
+                const f4 x = *reinterpret_cast<const f4 *>(src + 4 * v);
+                dst[4 * v] = x[0]; dst[4 * v + 1] = x[1]; dst[4 * v + 2] = x[2]; dst[4 * v + 3] = x[3];
             }
         } else {
-            const int gc = col0 + a_r;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int gk = k0 + a_k + i;
-                rb[i] = (gc < g.N && gk < g.K) ? g.B[(long)gc * g.ldb + gk] : 0.0f;
-            }
+            for (int i = 0; i < 16; ++i) dst[i] = (row_ok && first + i < limit) ? src[i] : 0.0f;
+        }
+    };
+    auto load_slab = [&](int k0) {
+        const int gr = row0 + a_r;
+        load16(ra, g.A + (long)gr * g.lda + k0 + a_k, gr < g.M, k0 + a_k, g.K);
+        if (!TRANS_B) {
+            const int gk = k0 + b_k;
+            load16(rb, g.B + (long)gk * g.ldb + col0 + b_c, gk < g.K, col0 + b_c, g.N);
+        } else {
+            const int gc = col0 + a_r;
+            load16(rb, g.B + (long)gc * g.ldb + k0 + a_k, gc < g.N, k0 + a_k, g.K);
         }
     };
     auto cvt = [&](float x, u16 &hi, u16 &lo) {
@@ -157,21 +159,27 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // C is read once: issue its loads now so their latency overlaps the K loop.
-    // Lane l, register q of a 16x16 block holds row 4*(l/16)+q, column l%16.
-    const int lq = (lane >> 4) * 4;
+    // The matrix-core products are formed as (op(B)^T A^T) so that lane l of a
+    // 16 x 16 block holds row l%16 and the FOUR CONSECUTIVE columns 4*(l/16)+q:
+    // C moves as 16-byte vectors.  C is read once; its loads are issued now so
+    // their latency overlaps the K loop.
+    const int lc = (lane >> 4) * 4;
     f4 cpre[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+        const int r = row0 + wm * 64 + i * 16 + (lane & 15);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int c = col0 + wn * 64 + j * 16 + (lane & 15);
+            const int c = col0 + wn * 64 + j * 16 + lc;
+            const float *src = g.C + (long)r * g.ldc + c;
+            if (VEC && r < g.M && c + 3 < g.N) {
+                cpre[i][j] = *reinterpret_cast<const f4 *>(src);
+            } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = row0 + wm * 64 + i * 16 + lq + q;
-                cpre[i][j][q] = (r < g.M && c < g.N) ? g.C[(long)r * g.ldc + c] : 0.0f;
+                for (int q = 0; q < 4; ++q) cpre[i][j][q] = (r < g.M && c + q < g.N) ? src[q] : 0.0f;
             }
         }
+    }
 
     const int nslab = (g.K + BK - 1) / BK;
     load_slab(0);
@@ -199,17 +207,18 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
+                // operands swapped: op(B) fragment as the left factor, A as the right
                 if (MODE == SCIRS_LOWP_FP16) {
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                        __builtin_bit_cast(f16x8, ah[i]), __builtin_bit_cast(f16x8, bh[j]), acc[i][j], 0, 0, 0);
+                        __builtin_bit_cast(f16x8, bh[j]), __builtin_bit_cast(f16x8, ah[i]), acc[i][j], 0, 0, 0);
                 } else {
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf16x8, ah[i]), __builtin_bit_cast(bf16x8, bh[j]), acc[i][j], 0, 0, 0);
+                        __builtin_bit_cast(bf16x8, bh[j]), __builtin_bit_cast(bf16x8, ah[i]), acc[i][j], 0, 0, 0);
                     if (MODE == SCIRS_LOWP_BF16X3) {
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, ah[i]), __builtin_bit_cast(bf16x8, bl[j]), acc[i][j], 0, 0, 0);
+                            __builtin_bit_cast(bf16x8, bl[j]), __builtin_bit_cast(bf16x8, ah[i]), acc[i][j], 0, 0, 0);
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, al[i]), __builtin_bit_cast(bf16x8, bh[j]), acc[i][j], 0, 0, 0);
+                            __builtin_bit_cast(bf16x8, bh[j]), __builtin_bit_cast(bf16x8, al[i]), acc[i][j], 0, 0, 0);
                     }
                 }
             }
@@ -218,17 +227,27 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
     }
 
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int c = col0 + wn * 64 + j * 16 + lr;
-        if (c >= g.N) continue;
-        float sc = 1.0f;
-        if (MODE == SCIRS_LOWP_FP16) sc = __builtin_ldexpf(1.0f, (g.colexp ? g.colexp[c] : 0) - g.a_shift);
+    for (int i = 0; i < 4; ++i) {
+        const int r = row0 + wm * 64 + i * 16 + (lane & 15);
+        if (r >= g.M) continue;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int j = 0; j < 4; ++j) {
+            const int c = col0 + wn * 64 + j * 16 + lc;
+            f4 out;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int r = row0 + wm * 64 + i * 16 + lq + q;
-                if (r < g.M) g.C[(long)r * g.ldc + c] = cpre[i][j][q] - acc[i][j][q] * sc;
+                float sc = 1.0f;
+                if (MODE == SCIRS_LOWP_FP16)
+                    sc = __builtin_ldexpf(1.0f, ((g.colexp && c + q < g.N) ? g.colexp[c + q] : 0) - g.a_shift);
+                out[q] = cpre[i][j][q] - acc[i][j][q] * sc;
+            }
+            float *dst = g.C + (long)r * g.ldc + c;
+            if (VEC && c + 3 < g.N) {
+                *reinterpret_cast<f4 *>(dst) = out;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (c + q < g.N) dst[q] = out[q];
             }
         }
     }
@@ -246,10 +265,15 @@ int launch_lowp(int M, int N, int K, const float *A, long lda, const float *B, l
     g.tiles_n = (N + BN - 1) / BN;
     g.lower = lower; g.info = info; g.colexp = colexp; g.a_shift = a_shift;
     const int nwg = (g.tiles_m * g.tiles_n + 7) / 8 * 8;
-    if (trans_b)
-        hipLaunchKernelGGL((lowp_gemm_kernel<MODE, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
-    else
-        hipLaunchKernelGGL((lowp_gemm_kernel<MODE, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    auto al16 = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
+    const bool vec = lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 && al16(A) && al16(B) && al16(C);
+    if (trans_b) {
+        if (vec) hipLaunchKernelGGL((lowp_gemm_kernel<MODE, true, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+        else hipLaunchKernelGGL((lowp_gemm_kernel<MODE, true, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    } else {
+        if (vec) hipLaunchKernelGGL((lowp_gemm_kernel<MODE, false, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+        else hipLaunchKernelGGL((lowp_gemm_kernel<MODE, false, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    }
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }

@@ -13,8 +13,9 @@
 //      info = column + 1.
 //   2. laswp        — applies the panel's net row permutation to the columns
 //      left and right of the panel as one gather (all loads, then all stores).
-//   3. trsm         — U12 = L11^-1 A12, 64-column strips with L11 (packed) and
-//      the strip in LDS.
+//   3. trsm         — U12 = L11^-1 A12: FP64 by substitution in 64-column strips
+//      with L11 (packed) and the strip in LDS; FP32 by forming L11^-1 in one
+//      workgroup and multiplying the strips by it.
 //   4. dgemm        — A22 -= L21 U12 on the matrix cores (gemm_f64.hip).
 //
 // The arithmetic per element is the reference's (multiplier by true division,
@@ -24,7 +25,9 @@
 #include "kernels.hpp"
 #include "panel.hpp"
 #include "prof.hpp"
+#include "api_internal.hpp"
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -32,7 +35,8 @@ constexpr int NT = 256;  // threads per workgroup everywhere in this file
 
 
 // Apply a gather list to the columns outside [k0, k0+kb): each workgroup owns 64
-// columns, loads every source row of its columns first, then stores.
+// columns, stages the (dst, src) list in LDS, loads every source row of its
+// columns first (16 loads in flight per thread), then stores.
 template <typename T>
 __global__ __launch_bounds__(NT) void laswp_kernel(T *A, long lda, int ncols_total, int k0, int kb,
                                                    const int *pairs, const int *npairs, const int *info)
@@ -45,15 +49,18 @@ __global__ __launch_bounds__(NT) void laswp_kernel(T *A, long lda, int ncols_tot
     if (c >= k0) c += kb;
     const bool ok = c < ncols_total;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *vals = reinterpret_cast<T *>(smem_raw);   // np x 64
+    int *sp = reinterpret_cast<int *>(smem_raw);                   // 2 * np
+    T *vals = reinterpret_cast<T *>(smem_raw + (size_t)4 * SCIRS_KB_MAX * sizeof(int));   // np x 64
+    for (int q = threadIdx.x; q < 2 * np; q += NT) sp[q] = pairs[q];
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int U = 8;   // loads in flight per thread before the LDS stores
+    constexpr int U = 16;   // loads in flight per thread before the LDS stores
     for (int q0 = wave; q0 < np; q0 += U * (NT / 64)) {
         T v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int q = q0 + u * (NT / 64);
-            v[u] = (ok && q < np) ? A[(long)pairs[2 * q + 1] * lda + c] : (T)0;
+            v[u] = (ok && q < np) ? A[(long)sp[2 * q + 1] * lda + c] : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -63,7 +70,112 @@ __global__ __launch_bounds__(NT) void laswp_kernel(T *A, long lda, int ncols_tot
     }
     __syncthreads();
     for (int q = wave; q < np; q += NT / 64)
-        if (ok) A[(long)pairs[2 * q] * lda + c] = vals[q * 64 + lane];
+        if (ok) A[(long)sp[2 * q] * lda + c] = vals[q * 64 + lane];
+}
+
+// ---- FP32 TRSM through the inverse of L11 (the FP32 factorisation only
+// preconditions refinement, so forming L11^-1 is acceptable there).
+//
+// trtri: one workgroup inverts the unit-lower kb x kb block (kb <= 128, padded
+// with the identity to 128): 16 x 16 diagonal blocks by forward substitution
+// (one lane per column), then three doubling levels
+//     [A 0; B C]^-1 = [A^-1 0; -C^-1 B A^-1  C^-1]
+// with all products in LDS.  Result: Linv (128 x 128, row-major) in HBM.
+constexpr int TI = 128;          // padded order of the inverted block
+constexpr int TIP = TI + 1;
+
+__global__ __launch_bounds__(NT) void trtri_unit_lower_kernel(const float *A, long lda, int kb, float *Linv,
+                                                              const int *info)
+{
+    if (*info != 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *Ls = reinterpret_cast<float *>(smem_raw);   // TI x TIP  L (unit diagonal, zero above)
+    float *Y = Ls + TI * TIP;                          // TI x TIP  inverse under construction
+    float *Tm = Y + TI * TIP;                          // 64 * 64   products B A^-1 of one level
+    const int t = threadIdx.x;
+    for (int idx = t; idx < TI * TI; idx += NT) {
+        const int i = idx / TI, j = idx % TI;
+        float v;
+        if (j < i) v = (i < kb && j < kb) ? A[(long)i * lda + j] : 0.0f;
+        else v = (j == i) ? 1.0f : 0.0f;
+        Ls[i * TIP + j] = v;
+        Y[i * TIP + j] = 0.0f;
+    }
+    __syncthreads();
+    // 16 x 16 diagonal blocks: lane (t < 128) -> block t / 16, column t % 16
+    if (t < TI) {
+        const int b0 = (t / 16) * 16, c = t % 16;
+        float x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < i; ++k) s += Ls[(b0 + i) * TIP + b0 + k] * x[k];
+            x[i] = (i < c) ? 0.0f : ((i == c) ? 1.0f : -s);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i >= c) Y[(b0 + i) * TIP + b0 + c] = x[i];
+    }
+    __syncthreads();
+    for (int sz = 16; sz < TI; sz *= 2) {
+        const int npair = TI / (2 * sz);
+        // Tm[p][i][j] = sum_k B[i][k] Ainv[k][j], B = L[C rows, A cols]
+        for (int o = t; o < npair * sz * sz; o += NT) {
+            const int pidx = o / (sz * sz), rem = o % (sz * sz), i = rem / sz, j = rem % sz;
+            const int a0 = 2 * pidx * sz, c0 = a0 + sz;
+            float acc = 0.0f;
+            for (int k = j; k < sz; ++k) acc += Ls[(c0 + i) * TIP + a0 + k] * Y[(a0 + k) * TIP + a0 + j];
+            Tm[o] = acc;
+        }
+        __syncthreads();
+        // Y[C rows, A cols] = -Cinv Tm
+        for (int o = t; o < npair * sz * sz; o += NT) {
+            const int pidx = o / (sz * sz), rem = o % (sz * sz), i = rem / sz, j = rem % sz;
+            const int a0 = 2 * pidx * sz, c0 = a0 + sz;
+            const float *tp = Tm + pidx * sz * sz;
+            float acc = 0.0f;
+            for (int k = 0; k <= i; ++k) acc += Y[(c0 + i) * TIP + c0 + k] * tp[k * sz + j];
+            Y[(c0 + i) * TIP + a0 + j] = -acc;
+        }
+        __syncthreads();
+    }
+    for (int idx = t; idx < TI * TI; idx += NT) Linv[idx] = Y[(idx / TI) * TIP + idx % TI];
+}
+
+// U12 = Linv A12 for one 64-column strip per workgroup: Linv and the strip in
+// LDS, each thread 32 rows of one column.
+__global__ __launch_bounds__(NT) void trsm_apply_kernel(float *A, long lda, int k0, int kb, int ncols_total,
+                                                        const float *Linv, const int *info)
+{
+    if (*info != 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *Ls = reinterpret_cast<float *>(smem_raw);   // kb x TIP
+    float *Bs = Ls + TI * TIP;                         // kb x 65
+    const int t = threadIdx.x, lane = t & 63, rg = t >> 6;
+    const int cbase = k0 + kb + blockIdx.x * 64;
+    for (int idx = t; idx < kb * TI; idx += NT) Ls[(idx / TI) * TIP + idx % TI] = Linv[idx];
+    g2s_block<float>(Bs, 65, A + (long)k0 * lda + cbase, lda, kb, 64, kb, ncols_total - cbase);
+    __syncthreads();
+    float acc[TI / 4];
+#pragma unroll
+    for (int i = 0; i < TI / 4; ++i) acc[i] = 0.0f;
+    for (int k = 0; k < kb; ++k) {
+        const float bk = Bs[k * 65 + lane];
+#pragma unroll
+        for (int i = 0; i < TI / 4; ++i) {
+            const int r = rg + 4 * i;
+            if (r < kb) acc[i] += Ls[r * TIP + k] * bk;
+        }
+    }
+    const int c = cbase + lane;
+    if (c < ncols_total) {
+#pragma unroll
+        for (int i = 0; i < TI / 4; ++i) {
+            const int r = rg + 4 * i;
+            if (r < kb) A[(long)(k0 + r) * lda + c] = acc[i];
+        }
+    }
 }
 
 // U12 = L11^-1 A12 for the kb x (N - k0 - kb) block right of the panel.
@@ -147,6 +259,17 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         scirs_set_error("getrf: %d panels exceed the workspace (%d)", npanels, w.max_panels);
         return SCIRS_ERR_ARG;
     }
+    // Every panel's interchange count starts at zero: panels that never run (the
+    // factorisation stopped at a singular column) must not leave the lists of an
+    // earlier factorisation behind for scirs_perm_from_pairs.
+    if (int rc = scirs_fill_int(w.npairs, npanels, 0, stream)) return rc;
+    const bool lookahead = w.lookahead && w.side_stream && npanels > 1;
+    if (lookahead) {
+        // the side stream starts after everything already queued on the caller's stream
+        SCIRS_HIP_CHECK(hipEventRecord(w.ev_start, stream));
+        SCIRS_HIP_CHECK(hipStreamWaitEvent(w.side_stream, w.ev_start, 0));
+        SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, stream));
+    }
     const int rmax = panel::max_rows<T>(kb);
     for (int p = 0; p < npanels; ++p) {
         const int k0 = p * kb;
@@ -173,18 +296,32 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         w.seq_next += 512u;
         pa.pairs = w.pairs + (size_t)p * 4 * SCIRS_KB_MAX; pa.npairs = w.npairs + p;
         pa.info = w.info; pa.info_val = w.info_val; pa.abort = w.abort; pa.spin_limit = w.spin_limit;
-        void *args[] = {&pa};
         const size_t lds = panel::lds_bytes(R, kbw, sizeof(T));
-        if (int rc = scirs_allow_lds((const void *)panel::getf2_panel_kernel<T>, lds)) return rc;
+        const void *pk = w.panel_w0 ? (const void *)panel::getf2_panel_kernel<T, true>
+                                    : (const void *)panel::getf2_panel_kernel<T, false>;
+        if (int rc = scirs_allow_lds(pk, lds)) return rc;
         {
             scirs::ProfScope ps(SCIRS_PROF_PANEL, (double)mrows * kbw * kbw, 2.0 * mrows * kbw * sizeof(T), stream);
-            SCIRS_HIP_CHECK(hipLaunchCooperativeKernel((const void *)panel::getf2_panel_kernel<T>, dim3(G),
-                                                       dim3(panel::NT), args, lds, stream));
+            if (lookahead) {
+                // a normal launch, so the panel can share the device with the side
+                // stream's update: its G <= #CUs workgroups all become resident once
+                // that update's workgroups retire (they wait for nothing)
+                if (w.panel_w0)
+                    hipLaunchKernelGGL((panel::getf2_panel_kernel<T, true>), dim3(G), dim3(panel::NT), lds, stream, pa);
+                else
+                    hipLaunchKernelGGL((panel::getf2_panel_kernel<T, false>), dim3(G), dim3(panel::NT), lds, stream, pa);
+                SCIRS_HIP_CHECK(hipGetLastError());
+            } else {
+                void *args[] = {&pa};
+                SCIRS_HIP_CHECK(hipLaunchCooperativeKernel(pk, dim3(G), dim3(panel::NT), args, lds, stream));
+            }
         }
+        // the rest of the previous panel's update must land before the row interchanges
+        if (lookahead && p > 0) SCIRS_HIP_CHECK(hipStreamWaitEvent(stream, w.ev_upd, 0));
         // row interchanges outside the panel
         const int nout_cols = n - kbw;
         if (nout_cols > 0) {
-            const size_t lds_sw = (size_t)2 * kbw * 64 * sizeof(T);
+            const size_t lds_sw = (size_t)4 * SCIRS_KB_MAX * sizeof(int) + (size_t)2 * kbw * 64 * sizeof(T);
             if (int rc = scirs_allow_lds((const void *)laswp_kernel<T>, lds_sw)) return rc;
             scirs::ProfScope ps(SCIRS_PROF_LASWP, 0.0, 4.0 * kbw * nout_cols * sizeof(T), stream);
             hipLaunchKernelGGL(laswp_kernel<T>, dim3((nout_cols + 63) / 64), dim3(NT), lds_sw, stream,
@@ -193,9 +330,23 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         }
         const int nright = n - k0 - kbw;
         if (nright > 0) {
-            const size_t lds_t = ((size_t)kbw * (kbw - 1) / 2 + (size_t)kbw * 64) * sizeof(T);
-            if (int rc = scirs_allow_lds((const void *)trsm_llnu_kernel<T>, lds_t)) return rc;
-            {
+            if (std::is_same<T, float>::value && kbw <= TI && w.linv) {
+                // FP32: L11^-1 once, then a parallel multiply
+                scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * nright,
+                                    (0.5 * kbw * kbw + 2.0 * kbw * nright) * sizeof(T), stream);
+                const size_t lds_i = ((size_t)2 * TI * TIP + 64 * 64) * sizeof(float);
+                if (int rc = scirs_allow_lds((const void *)trtri_unit_lower_kernel, lds_i)) return rc;
+                hipLaunchKernelGGL(trtri_unit_lower_kernel, dim3(1), dim3(NT), lds_i, stream,
+                                   (const float *)(A + (long)k0 * lda + k0), lda, kbw, (float *)w.linv,
+                                   (const int *)w.info);
+                const size_t lds_a = ((size_t)TI * TIP + (size_t)TI * 65) * sizeof(float);
+                if (int rc = scirs_allow_lds((const void *)trsm_apply_kernel, lds_a)) return rc;
+                hipLaunchKernelGGL(trsm_apply_kernel, dim3((nright + 63) / 64), dim3(NT), lds_a, stream,
+                                   (float *)A, lda, k0, kbw, n, (const float *)w.linv, (const int *)w.info);
+                SCIRS_HIP_CHECK(hipGetLastError());
+            } else {
+                const size_t lds_t = ((size_t)kbw * (kbw - 1) / 2 + (size_t)kbw * 64) * sizeof(T);
+                if (int rc = scirs_allow_lds((const void *)trsm_llnu_kernel<T>, lds_t)) return rc;
                 scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * nright,
                                     (0.5 * kbw * kbw + 2.0 * kbw * nright) * sizeof(T), stream);
                 hipLaunchKernelGGL(trsm_llnu_kernel<T>, dim3((nright + 63) / 64), dim3(NT), lds_t, stream,
@@ -204,17 +355,44 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
             }
             const int mbelow = m - k0 - kbw;
             if (mbelow > 0) {
-                scirs::ProfScope ps(SCIRS_PROF_UPDATE, 2.0 * mbelow * nright * kbw,
-                                    (2.0 * mbelow * nright + (double)mbelow * kbw + (double)kbw * nright) * sizeof(T),
+                // look-ahead: the next panel's columns now (main stream), the rest on
+                // the side stream, overlapping the next panel's factorisation
+                const int nnext = lookahead ? std::min(nright, kb) : nright;
+                const int nrest = nright - nnext;
+                if (nrest > 0) {
+                    SCIRS_HIP_CHECK(hipEventRecord(w.ev_trsm, stream));
+                    SCIRS_HIP_CHECK(hipStreamWaitEvent(w.side_stream, w.ev_trsm, 0));
+                    scirs::ProfScope ps(SCIRS_PROF_UPDATE, 2.0 * mbelow * nrest * kbw,
+                                        (2.0 * mbelow * nrest + (double)mbelow * kbw + (double)kbw * nrest) *
+                                            sizeof(T), w.side_stream);
+                    int rc = scirs_trailing_update<T>(mbelow, nrest, kbw,
+                                                      A + (long)(k0 + kbw) * lda + k0, lda,
+                                                      A + (long)k0 * lda + k0 + kbw + nnext, lda,
+                                                      A + (long)(k0 + kbw) * lda + k0 + kbw + nnext, lda,
+                                                      w, w.side_stream);
+                    if (rc) return rc;
+                    SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, w.side_stream));
+                }
+                scirs::ProfScope ps(SCIRS_PROF_UPDATE, 2.0 * mbelow * nnext * kbw,
+                                    (2.0 * mbelow * nnext + (double)mbelow * kbw + (double)kbw * nnext) * sizeof(T),
                                     stream);
-                int rc = scirs_trailing_update<T>(mbelow, nright, kbw,
+                int rc = scirs_trailing_update<T>(mbelow, nnext, kbw,
                                                   A + (long)(k0 + kbw) * lda + k0, lda,
                                                   A + (long)k0 * lda + k0 + kbw, lda,
                                                   A + (long)(k0 + kbw) * lda + k0 + kbw, lda,
                                                   w, stream);
                 if (rc) return rc;
+                if (nrest == 0 && lookahead) {
+                    // nothing pending on the side stream for the next panel to wait for
+                    SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, stream));
+                }
             }
         }
+    }
+    if (lookahead) {
+        // the caller's stream sees every update
+        SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, w.side_stream));
+        SCIRS_HIP_CHECK(hipStreamWaitEvent(stream, w.ev_upd, 0));
     }
     return SCIRS_OK;
 }
@@ -231,11 +409,14 @@ __global__ __launch_bounds__(NT) void perm_from_pairs_kernel(int m, int npanels,
         const int np = npairs[p];
         int64_t v0 = 0, v1 = 0;
         const int q0 = threadIdx.x, q1 = threadIdx.x + NT;
-        if (q0 < np) v0 = perm[pp[2 * q0 + 1]];
-        if (q1 < np) v1 = perm[pp[2 * q1 + 1]];
+        // a pair outside [0, m) cannot come from this factorisation: ignored
+        const bool ok0 = q0 < np && (unsigned)pp[2 * q0] < (unsigned)m && (unsigned)pp[2 * q0 + 1] < (unsigned)m;
+        const bool ok1 = q1 < np && (unsigned)pp[2 * q1] < (unsigned)m && (unsigned)pp[2 * q1 + 1] < (unsigned)m;
+        if (ok0) v0 = perm[pp[2 * q0 + 1]];
+        if (ok1) v1 = perm[pp[2 * q1 + 1]];
         __syncthreads();
-        if (q0 < np) perm[pp[2 * q0]] = v0;
-        if (q1 < np) perm[pp[2 * q1]] = v1;
+        if (ok0) perm[pp[2 * q0]] = v0;
+        if (ok1) perm[pp[2 * q1]] = v1;
         __syncthreads();
     }
 }

@@ -28,6 +28,11 @@ struct ScirsWork {
     unsigned long long *xtri;  // chained triangular solve: sequence-tagged x values
     size_t xtri_words;
     int *ticket;         // chained triangular solve: block-row ticket counter
+    void *linv;          // 128 x 128 inverse of a panel's unit-lower block (FP32 TRSM)
+    int panel_w0;        // panel critical path on wave 0 alone (1) or on all waves (0)
+    int lookahead;       // overlap panel k+1 with the rest of update k on side_stream
+    hipStream_t side_stream;
+    hipEvent_t ev_trsm, ev_upd, ev_start;
 };
 
 enum {

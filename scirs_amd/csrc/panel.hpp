@@ -15,10 +15,10 @@
 // they need carries the expected number; polls are bounded and raise *abort
 // when exceeded, so no wave can wait forever.
 //
-// Latency hiding: (1) the poll of the candidate list also reads the row of
-// the best candidate seen so far, so when the last candidate arrives the pivot
-// row is usually already in registers — one memory round trip per column
-// instead of two; (2) right after the pivot row arrives, a workgroup computes
+// Latency hiding: (1) every poll of the candidate list also reads the row of
+// the best candidate of the previous poll (in the same round trip), so when
+// the last candidate arrives the pivot row is usually already in registers —
+// one memory round trip per column instead of two; (2) right after the pivot row arrives, a workgroup computes
 // the multipliers, updates column j+1 and finds its next candidate in one
 // pass, brings that row (and row j+1 when it holds it) fully up to date and
 // publishes; (3) the bulk rank-1 update of its other rows runs on waves 1-3
@@ -84,7 +84,10 @@ struct PanelArgs {
     unsigned spin_limit;
 };
 
-template <typename T>
+// W0 selects where the column's critical path runs: false = all four waves
+// (five barriers per column), true = wave 0 alone (rows strided over its 64
+// lanes; two barriers per column), waves 1-3 doing only the bulk update.
+template <typename T, bool W0>
 __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
 {
     if (*a.info != 0) return;
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
     int *out_label = out_pos + kb;                    // kb
     __shared__ T red_v[NT / 64];
     __shared__ int red_i[NT / 64];
-    __shared__ int s_p, s_g, s_stop, s_c, s_nout, s_bad, s_fetched;
+    __shared__ int s_p, s_g, s_stop, s_c, s_nout;
     __shared__ T s_pv, s_cv;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -175,9 +178,20 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
             bool timed_out = false, row_ok = false;
             T sp[SPEC], sj[SPEC];
             const u64 *xj = a.xrj + (size_t)b * kb * W;
+            int guess = -1;   // best workgroup of the previous poll: its row is read with this poll's records
             while (true) {
                 bool ok = true;
                 T v = (T)0; int i = -1, wg = -1;
+                bool rok = guess >= 0;
+                const u64 *xr = a.xrow + ((size_t)b * G + (guess >= 0 ? guess : 0)) * (size_t)kb * W;
+#pragma unroll
+                for (int u = 0; u < SPEC; ++u) {
+                    const int q = lane + 64 * u;
+                    if (q < kb) {
+                        if (guess >= 0) rok = Tag<T>::get(xr + (size_t)q * W, seq, sp[u]) && rok;
+                        rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
+                    }
+                }
                 for (int q = lane; q < G; q += 64) {
                     const u64 *xw = a.xw + ((size_t)b * G + q) * 4;
                     const u64 w0 = ld_agent(xw);
@@ -192,9 +206,19 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 const unsigned long long ball = __ballot(i == ii && i >= 0);
                 const int src = ball ? __ffsll(ball) - 1 : 0;
                 wgw = __shfl(wg, src, 64);
-                bool rok = wgw >= 0;
-                if (wgw >= 0) {
-                    const u64 *xr = a.xrow + ((size_t)b * G + wgw) * (size_t)kb * W;
+                if (__all(ok)) { row_ok = wgw >= 0 && wgw == guess && __all(rok); break; }
+                guess = wgw;
+                if (++spins > a.spin_limit ||
+                    __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    timed_out = true;
+                    break;
+                }
+            }
+            // the guess missed: read the winner's row (and row j) now
+            if (!row_ok && !timed_out && wgw >= 0 && vv >= eps) {
+                const u64 *xr = a.xrow + ((size_t)b * G + wgw) * (size_t)kb * W;
+                while (true) {
+                    bool rok = true;
 #pragma unroll
                     for (int u = 0; u < SPEC; ++u) {
                         const int q = lane + 64 * u;
@@ -203,12 +227,8 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                             rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
                         }
                     }
-                }
-                if (__all(ok)) { row_ok = __all(rok); break; }
-                if (++spins > a.spin_limit ||
-                    __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    timed_out = true;
-                    break;
+                    if (__all(rok)) { row_ok = true; break; }
+                    if (++spins > a.spin_limit) { timed_out = true; break; }
                 }
             }
             if (row_ok) {
@@ -219,7 +239,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 }
             }
             if (lane == 0) {
-                s_p = ii; s_g = wgw; s_pv = vv; s_fetched = row_ok ? 1 : 0;
+                s_p = ii; s_g = wgw; s_pv = vv;
                 if (timed_out) {
                     __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     s_stop = 1;
@@ -231,35 +251,93 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         }
         __syncthreads();
         if (s_stop) break;
-        const int p = s_p, sg = s_g;
+        if constexpr (W0) {
+        const int p = s_p;
         const bool own_j = (j >= row0 && j < row0 + Rg);
         const bool own_p = (p >= row0 && p < row0 + Rg);
-        // ---- 2. the pivot row (and row j) when the speculative fetch missed
-        if (!s_fetched) {
-            const u64 *xr = a.xrow + ((size_t)b * G + sg) * (size_t)kb * W;
-            const u64 *xj = a.xrj + (size_t)b * kb * W;
-            unsigned spins = 0;
-            while (true) {
-                if (t == 0) s_bad = 0;
-                __syncthreads();
-                bool bad = false;
-                for (int q = t; q < kb; q += NT) {
-                    T v;
-                    if (!Tag<T>::get(xr + (size_t)q * W, seq, v)) bad = true;
-                    prow[q] = v;
-                    if (!Tag<T>::get(xj + (size_t)q * W, seq, v)) bad = true;
-                    rjv[q] = v;
-                }
-                if (bad) s_bad = 1;
-                __syncthreads();
-                if (!s_bad) break;
-                if (++spins > a.spin_limit) {
-                    if (t == 0) __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
+        const int r1 = max(0, j + 1 - row0);
+        const bool more = j + 1 < kb;
+        if (wave == 0) {
+            // ---- 2. swap, permutation record, multipliers, column j+1, arg-max: one wave
+            if (p != j) {
+                if (own_j)
+                    for (int c = lane; c < kb; c += 64) tile[(j - row0) * kbp + c] = prow[c];
+                if (own_p)
+                    for (int c = lane; c < kb; c += 64) tile[(p - row0) * kbp + c] = rjv[c];
+                if (g == 0) {
+                    if (p < kb) {
+                        if (lane == 0) { int tmp = top_label[j]; top_label[j] = top_label[p]; top_label[p] = tmp; }
+                    } else {
+                        const int nout = s_nout;
+                        int found = -1;
+                        for (int base = 0; base < nout && found < 0; base += 64) {
+                            const int sidx = base + lane;
+                            const unsigned long long bl = __ballot(sidx < nout && out_pos[sidx] == p);
+                            if (bl) found = base + __ffsll(bl) - 1;
+                        }
+                        if (lane == 0) {
+                            if (found < 0) { found = nout; out_pos[found] = p; out_label[found] = p; s_nout = nout + 1; }
+                            int tmp = top_label[j]; top_label[j] = out_label[found]; out_label[found] = tmp;
+                        }
+                    }
                 }
             }
-            if (s_bad) break;
+            const T pivot = prow[j];   // true division (lapack.rs:138)
+            const T u1 = more ? prow[j + 1] : (T)0;
+            T bv = (T)0; int bi = -1;
+            for (int r = r1 + lane; r < Rg; r += 64) {
+                const T l = tile[r * kbp + j] / pivot;
+                tile[r * kbp + j] = l;
+                lcol[r] = l;
+                if (more) {
+                    const T x = tile[r * kbp + j + 1] - l * u1;
+                    tile[r * kbp + j + 1] = x;
+                    if (bi < 0 || fabs(x) > bv) { bv = fabs(x); bi = r; }
+                }
+            }
+            wave_argmax(bv, bi);
+            if (lane == 0) { s_c = bi; s_cv = bv; }
         }
+        __syncthreads();
+        if (!more) break;
+        const int c = s_c;
+        const int jn = (j + 1 >= row0 && j + 1 < row0 + Rg) ? j + 1 - row0 : -1;
+        if (wave == 0) {
+            // ---- 3. the next candidate row and row j+1 fully up to date, published
+            for (int q = j + 2 + lane; q < kb; q += 64) {
+                if (c >= 0) tile[c * kbp + q] -= lcol[c] * prow[q];
+                if (jn >= 0 && jn != c) tile[jn * kbp + q] -= lcol[jn] * prow[q];
+            }
+            const int jj = j + 1, bb = jj & 1;
+            const unsigned sq = a.seq_base + (unsigned)jj + 1u;
+            u64 *xr = a.xrow + ((size_t)bb * G + g) * (size_t)kb * W;
+            if (c >= 0)
+                for (int q = lane; q < kb; q += 64) Tag<T>::put(xr + (size_t)q * W, sq, tile[c * kbp + q]);
+            if (jn >= 0) {
+                u64 *xj = a.xrj + (size_t)bb * kb * W;
+                for (int q = lane; q < kb; q += 64) Tag<T>::put(xj + (size_t)q * W, sq, tile[jn * kbp + q]);
+            }
+            if (lane == 0) {
+                u64 *xw = a.xw + ((size_t)bb * G + g) * 4;
+                st_agent(xw, ((u64)sq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
+                Tag<T>::put(xw + 1, sq, s_cv);
+            }
+        } else if (j + 2 < kb) {
+            // ---- 4. bulk rank-1 update of the other rows (waves 1-3), overlapping
+            //         wave 0's publication and its next poll
+            constexpr int TPR = 8;
+            const int tt = t - 64;
+            for (int r = r1 + tt / TPR; r < Rg; r += (NT - 64) / TPR) {
+                if (r == c || r == jn) continue;
+                const T l = lcol[r];
+                T *row = tile + r * kbp;
+                for (int q = j + 2 + (tt % TPR); q < kb; q += TPR) row[q] -= l * prow[q];
+            }
+        }
+        } else {
+        const int p = s_p;
+        const bool own_j = (j >= row0 && j < row0 + Rg);
+        const bool own_p = (p >= row0 && p < row0 + Rg);
         // ---- 3. row interchange inside the panel and the permutation record
         if (p != j) {
             if (own_j)
@@ -339,6 +417,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
             }
         }
     }
+        }
     __syncthreads();
 
     for (int r = wave; r < Rg; r += NT / 64)

@@ -2,8 +2,8 @@
 // of a row-major matrix, in ONE launch.
 //
 // Same arithmetic as the substitution loops of scirs2-linalg/src/solve.rs:159-199
-// (true division by the diagonal in FP64), with the k-sum split into 64-column
-// blocks.  The matrix is cut into 64-row block rows; block row p is solved by one
+// (true division by the diagonal in FP64; multiplication by the reciprocal in
+// FP32), with the k-sum split into 64-column blocks.  The matrix is cut into 64-row block rows; block row p is solved by one
 // workgroup once the x values of every block it depends on are known.
 //
 //   * Workgroups take block rows in dependency order from an atomic ticket, so
@@ -185,16 +185,22 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         }
     }
     if (failed) return;
+    // FP64 divides by the diagonal as the reference does; FP32 multiplies by a
+    // reciprocal each lane computes up front, which keeps the division off the
+    // 64-step dependent chain (the FP32 factors only precondition refinement).
+    constexpr bool EXACT = sizeof(T) == 8;
+    const T rinv = (!UNIT && !EXACT && lane < bs) ? (T)1 / Ld[lane * LDP + lane] : (T)1;
     if (LOWER) {
 #pragma unroll
         for (int k = 0; k < TB; ++k) {
             if (k < bs) {
                 const T l = Ld[lane * LDP + k];
                 const T d = Ld[k * LDP + k];
+                const T rk = readlane(rinv, k);
 #pragma unroll
                 for (int q = 0; q < QB; ++q) {
                     T x = readlane(s[q], k);
-                    if (!UNIT) x = x / d;
+                    if (!UNIT) x = EXACT ? x / d : x * rk;
                     if (lane == k) s[q] = x;
                     else if (lane > k) s[q] -= l * x;
                 }
@@ -206,10 +212,11 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
             if (k < bs) {
                 const T l = Ld[lane * LDP + k];
                 const T d = Ld[k * LDP + k];
+                const T rk = readlane(rinv, k);
 #pragma unroll
                 for (int q = 0; q < QB; ++q) {
                     T x = readlane(s[q], k);
-                    if (!UNIT) x = x / d;
+                    if (!UNIT) x = EXACT ? x / d : x * rk;
                     if (lane == k) s[q] = x;
                     else if (lane < k) s[q] -= l * x;
                 }

@@ -139,6 +139,26 @@ def test_singular_column_matches_oracle(L):
     assert eg.value.rc == 1
 
 
+def test_singular_after_larger_factorisation(L):
+    """Regression: a factorisation that stops at a singular column must not pick
+    up the row-interchange lists an earlier, larger factorisation left for the
+    panels it never ran (they named rows beyond n)."""
+    rng = np.random.default_rng(21)
+    big = rng.standard_normal((777, 777))
+    L.gesv(big, rng.standard_normal(777))
+    a = rng.standard_normal((200, 200))
+    a[:, 57] = 0.0
+    with pytest.raises(oracle.OracleError) as eo:
+        oracle.lu_factor(a)
+    for call in (lambda: L.gesv(a, np.ones(200)), lambda: L.getrf(a)):
+        with pytest.raises(L.SingularError) as eg:
+            call()
+        assert eg.value.rc == eo.value.code == 58
+    # the library is still usable afterwards
+    x, _ = L.gesv(big[:100, :100], np.ones(100))
+    assert np.all(np.isfinite(x))
+
+
 def test_tie_rule_first_max(L):
     a = np.array([[1.0, 2.0, 0.0, 1.0, 0.0],
                   [0.5, 1.0, 3.0, 0.0, 1.0],
