@@ -9,7 +9,8 @@ low-precision LU + FP64 refinement to the FP64 backward-error target, or, for
 the comparison line, FP64 LU + substitution.  TFLOP/s uses the 2n^3/3
 convention.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--mode bf16x3|fp16|bf16|fp32]
+    python bench.py [--steps K] [--warmup W] [--n N] [--mode bf16x6|bf16x3|fp16|bf16|fp32]
+    python bench.py --workload batched        # 20,000 uneven systems in one launch
 
 Prints one JSON line.  One GPU only: --gpus above 1 exits with a message.
 """
@@ -26,8 +27,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_MFMA_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
 PEAK_HBM_GBS = 8000.0
-MODES = {"bf16": 1, "fp16": 2, "bf16x3": 3, "fp32": 4}
-MODE_PEAK = {"bf16": "bf16", "fp16": "fp16", "bf16x3": "bf16", "fp32": "f32"}
+MODES = {"bf16": 1, "fp16": 2, "bf16x3": 3, "fp32": 4, "bf16x6": 5}
+MODE_PEAK = {"bf16": "bf16", "fp16": "fp16", "bf16x3": "bf16", "fp32": "f32", "bf16x6": "bf16"}
 
 
 def parse():
@@ -37,7 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--kappa", type=float, default=1e4)
-    ap.add_argument("--mode", default=os.environ.get("SCIRS_BENCH_MODE", "bf16x3"), choices=sorted(MODES))
+    ap.add_argument("--mode", default=os.environ.get("SCIRS_BENCH_MODE", "bf16x6"), choices=sorted(MODES))
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 LU comparison line")
     ap.add_argument("--cpu-n", type=int, default=2560, help="size of the CPU baseline sample")

@@ -46,11 +46,13 @@ extern "C" {
 #define SCIRS_PATH_FP32_CHOL 6
 
 /* precision modes of the mixed-precision solve (trailing-update arithmetic) */
-#define SCIRS_MODE_AUTO 0
+#define SCIRS_MODE_AUTO 0           /* SCIRS_GPU_MIXED_MODE, else BF16X6        */
 #define SCIRS_MODE_BF16 1           /* BF16 x BF16 -> FP32 matrix cores         */
 #define SCIRS_MODE_FP16 2           /* FP16 with power-of-two scaling -> FP32   */
 #define SCIRS_MODE_BF16X3 3         /* split BF16 (hi+lo), three products       */
-#define SCIRS_MODE_FP32 4           /* FP32 matrix cores                        */
+#define SCIRS_MODE_FP32 4           /* FP32 arithmetic                          */
+#define SCIRS_MODE_BF16X6 5         /* split BF16 (hi+mid+lo), six products:
+                                       FP32-accurate on BF16 matrix cores       */
 
 typedef struct scirs_solve_info {
     int32_t status;          /* same as the return value                          */

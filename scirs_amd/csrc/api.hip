@@ -131,6 +131,7 @@ int Context::init(int dev)
     work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 128);
     work.panel_w0 = env_int("SCIRS_GPU_PANEL_W0", 0);
     work.lookahead = env_int("SCIRS_GPU_LOOKAHEAD", 1);
+    work.poll_sleep = env_int("SCIRS_GPU_POLL_SLEEP", 0);
     size_t bytes = 0;
     const size_t xw_b = 2 * (size_t)max_g * 4 * 8, xrow_b = 2 * (size_t)max_g * kbmax * 2 * 8,
                  xrj_b = 2 * (size_t)kbmax * 2 * 8, flags_b = 64 * sizeof(int),

@@ -10,8 +10,13 @@
 //   SCIRS_LOWP_BF16X3  x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
 //                      C -= Ahi Bhi + Ahi Blo + Alo Bhi  (about 16 significant
 //                      bits per operand, three matrix-core products)
+//   SCIRS_LOWP_BF16X6  x = hi + mid + lo (three BF16 terms, ~24 bits: the FP32
+//                      significand); the six products whose order is above
+//                      2^-24 (hh, hm, mh, hl, lh, mm), so the update is as
+//                      accurate as an FP32 one while running on BF16 cores
 // Tiling: 128 x 128 per 256-thread workgroup, 2 x 2 waves of 64 x 64, K slabs
-// of 32 (one MFMA k-step) double-buffered in LDS.
+// of 32 (one MFMA k-step) double-buffered in LDS (single-buffered for the
+// three-term split, whose staging is 1.5x larger).
 #include "common.hpp"
 #include "kernels.hpp"
 #include "tile_map.hpp"
@@ -63,9 +68,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
     const int row0 = tm * BM, col0 = tn * BN;
     if (g.lower && col0 > row0 + BM - 1) return;
 
-    constexpr int NSPLIT = (MODE == SCIRS_LOWP_BF16X3) ? 2 : 1;
-    __shared__ __align__(16) u16 As[2][NSPLIT][BM * PITCH];
-    __shared__ __align__(16) u16 Bs[2][NSPLIT][BN * PITCH];
+    constexpr int NSPLIT = (MODE == SCIRS_LOWP_BF16X6) ? 3 : (MODE == SCIRS_LOWP_BF16X3) ? 2 : 1;
+    constexpr int NBUF = NSPLIT == 3 ? 1 : 2;
+    __shared__ __align__(16) u16 As[NBUF][NSPLIT][BM * PITCH];
+    __shared__ __align__(16) u16 Bs[NBUF][NSPLIT][BN * PITCH];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -102,24 +108,31 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
             load16(rb, g.B + (long)gc * g.ldb + k0 + a_k, gc < g.N, k0 + a_k, g.K);
         }
     };
-    auto cvt = [&](float x, u16 &hi, u16 &lo) {
+    auto cvt = [&](float x, u16 &hi, u16 &lo, u16 &mid) {
         if (MODE == SCIRS_LOWP_FP16) {
             hi = f32_to_f16(x);
         } else {
             hi = f32_to_bf16_rne(x);
             if (MODE == SCIRS_LOWP_BF16X3) lo = f32_to_bf16_rne(x - bf16_to_f32(hi));
+            if (MODE == SCIRS_LOWP_BF16X6) {
+                const float r = x - bf16_to_f32(hi);
+                mid = f32_to_bf16_rne(r);
+                lo = f32_to_bf16_rne(r - bf16_to_f32(mid));
+            }
         }
     };
+    // split index 0 = hi, NSPLIT-1 = lo, 1 = mid (three-term split only)
+    auto put = [&](u16 (*dst)[BM * PITCH], int idx, const u16 &hi, const u16 &lo, const u16 &mid) {
+        dst[0][idx] = hi;
+        if (NSPLIT >= 2) dst[NSPLIT - 1][idx] = lo;
+        if (NSPLIT == 3) dst[1][idx] = mid;
+    };
     auto store_slab = [&](int buf) {
-        u16 hi[16], lo[16];
+        u16 hi[16], lo[16], mid[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) cvt(ra[i] * ascale, hi[i], lo[i]);
+        for (int i = 0; i < 16; ++i) cvt(ra[i] * ascale, hi[i], lo[i], mid[i]);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) As[buf][0][a_r * PITCH + a_k + i] = hi[i];
-        if (NSPLIT == 2) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) As[buf][NSPLIT - 1][a_r * PITCH + a_k + i] = lo[i];
-        }
+        for (int i = 0; i < 16; ++i) put(As[buf], a_r * PITCH + a_k + i, hi[i], lo[i], mid[i]);
         if (!TRANS_B) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
@@ -128,14 +141,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
                     const int gc = col0 + b_c + i;
                     x = __builtin_ldexpf(x, gc < g.N ? -g.colexp[gc] : 0);
                 }
-                cvt(x, hi[i], lo[i]);
+                cvt(x, hi[i], lo[i], mid[i]);
             }
 #pragma unroll
-            for (int i = 0; i < 16; ++i) Bs[buf][0][(b_c + i) * PITCH + b_k] = hi[i];
-            if (NSPLIT == 2) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) Bs[buf][NSPLIT - 1][(b_c + i) * PITCH + b_k] = lo[i];
-            }
+            for (int i = 0; i < 16; ++i) put(Bs[buf], (b_c + i) * PITCH + b_k, hi[i], lo[i], mid[i]);
         } else {
             float sc = 1.0f;
             if (MODE == SCIRS_LOWP_FP16 && g.colexp) {
@@ -143,13 +152,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
                 sc = __builtin_ldexpf(1.0f, gc < g.N ? -g.colexp[gc] : 0);
             }
 #pragma unroll
-            for (int i = 0; i < 16; ++i) cvt(rb[i] * sc, hi[i], lo[i]);
+            for (int i = 0; i < 16; ++i) cvt(rb[i] * sc, hi[i], lo[i], mid[i]);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) Bs[buf][0][a_r * PITCH + a_k + i] = hi[i];
-            if (NSPLIT == 2) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) Bs[buf][NSPLIT - 1][a_r * PITCH + a_k + i] = lo[i];
-            }
+            for (int i = 0; i < 16; ++i) put(Bs[buf], a_r * PITCH + a_k + i, hi[i], lo[i], mid[i]);
         }
     };
 
@@ -190,19 +195,28 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
     const int lk = (lane >> 4) * 8;  // first of the lane's 8 k values
 
     for (int s = 0; s < nslab; ++s) {
-        const int buf = s & 1;
+        const int buf = NBUF == 2 ? (s & 1) : 0;
         if (s + 1 < nslab) load_slab((s + 1) * BK);
         typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-        u16x8 ah[4], bh[4], al[4], bl[4];
+        u16x8 ah[4], bh[4], al[4], bl[4], am[4], bm[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ah[i] = *reinterpret_cast<const u16x8 *>(&As[buf][0][(wm * 64 + i * 16 + lr) * PITCH + lk]);
-            bh[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][0][(wn * 64 + i * 16 + lr) * PITCH + lk]);
-            if (NSPLIT == 2) {
-                al[i] = *reinterpret_cast<const u16x8 *>(&As[buf][NSPLIT - 1][(wm * 64 + i * 16 + lr) * PITCH + lk]);
-                bl[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][NSPLIT - 1][(wn * 64 + i * 16 + lr) * PITCH + lk]);
+            const int ra_ = (wm * 64 + i * 16 + lr) * PITCH + lk, rb_ = (wn * 64 + i * 16 + lr) * PITCH + lk;
+            ah[i] = *reinterpret_cast<const u16x8 *>(&As[buf][0][ra_]);
+            bh[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][0][rb_]);
+            if (NSPLIT >= 2) {
+                al[i] = *reinterpret_cast<const u16x8 *>(&As[buf][NSPLIT - 1][ra_]);
+                bl[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][NSPLIT - 1][rb_]);
+            }
+            if (NSPLIT == 3) {
+                am[i] = *reinterpret_cast<const u16x8 *>(&As[buf][1][ra_]);
+                bm[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][1][rb_]);
             }
         }
+        auto mma = [&](const u16x8 &x, const u16x8 &y, f4 &c) {
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, x), __builtin_bit_cast(bf16x8, y),
+                                                        c, 0, 0, 0);
+        };
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -211,18 +225,26 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
                 if (MODE == SCIRS_LOWP_FP16) {
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
                         __builtin_bit_cast(f16x8, bh[j]), __builtin_bit_cast(f16x8, ah[i]), acc[i][j], 0, 0, 0);
+                } else if (MODE == SCIRS_LOWP_BF16X6) {
+                    // smallest terms first
+                    mma(bm[j], am[i], acc[i][j]);
+                    mma(bl[j], ah[i], acc[i][j]);
+                    mma(bh[j], al[i], acc[i][j]);
+                    mma(bm[j], ah[i], acc[i][j]);
+                    mma(bh[j], am[i], acc[i][j]);
+                    mma(bh[j], ah[i], acc[i][j]);
                 } else {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf16x8, bh[j]), __builtin_bit_cast(bf16x8, ah[i]), acc[i][j], 0, 0, 0);
+                    mma(bh[j], ah[i], acc[i][j]);
                     if (MODE == SCIRS_LOWP_BF16X3) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, bl[j]), __builtin_bit_cast(bf16x8, ah[i]), acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, bh[j]), __builtin_bit_cast(bf16x8, al[i]), acc[i][j], 0, 0, 0);
+                        mma(bl[j], ah[i], acc[i][j]);
+                        mma(bh[j], al[i], acc[i][j]);
                     }
                 }
             }
-        if (s + 1 < nslab) store_slab(buf ^ 1);
+        if (s + 1 < nslab) {
+            if (NBUF == 1) __syncthreads();   // every wave is done with the single buffer
+            store_slab(NBUF == 2 ? (buf ^ 1) : 0);
+        }
         __syncthreads();
     }
 
@@ -296,6 +318,8 @@ int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const 
         return launch_lowp<SCIRS_LOWP_BF16>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 0, stream);
     case SCIRS_LOWP_BF16X3:
         return launch_lowp<SCIRS_LOWP_BF16X3>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 0, stream);
+    case SCIRS_LOWP_BF16X6:
+        return launch_lowp<SCIRS_LOWP_BF16X6>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 0, stream);
     case SCIRS_LOWP_FP16:
         // |l_ij| <= 1 under partial pivoting: 2^14 keeps A in range and lifts small multipliers
         return launch_lowp<SCIRS_LOWP_FP16>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 14, stream);

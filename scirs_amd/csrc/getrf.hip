@@ -77,104 +77,145 @@ __global__ __launch_bounds__(NT) void laswp_kernel(T *A, long lda, int ncols_tot
 // preconditions refinement, so forming L11^-1 is acceptable there).
 //
 // trtri: one workgroup inverts the unit-lower kb x kb block (kb <= 128, padded
-// with the identity to 128): 16 x 16 diagonal blocks by forward substitution
-// (one lane per column), then three doubling levels
+// with the identity to 128), in place in LDS: the 16 x 16 diagonal blocks by
+// forward substitution (one lane per column, in registers), then three doubling
+// levels
 //     [A 0; B C]^-1 = [A^-1 0; -C^-1 B A^-1  C^-1]
-// with all products in LDS.  Result: Linv (128 x 128, row-major) in HBM.
+// whose products run as 4 x 4 register tiles.  Result: L11^-1 stored
+// TRANSPOSED (LinvT[k][r] = L11^-1[r][k], 128 x 128) in HBM.
 constexpr int TI = 128;          // padded order of the inverted block
-constexpr int TIP = TI + 1;
+constexpr int TIP = TI + 1;      // LDS pitch of the block
+constexpr int TP4 = TI + 4;      // LDS pitch of the transposed inverse (16-byte rows)
 
-__global__ __launch_bounds__(NT) void trtri_unit_lower_kernel(const float *A, long lda, int kb, float *Linv,
+__global__ __launch_bounds__(NT) void trtri_unit_lower_kernel(const float *A, long lda, int kb, float *LinvT,
                                                               const int *info)
 {
     if (*info != 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float *Ls = reinterpret_cast<float *>(smem_raw);   // TI x TIP  L (unit diagonal, zero above)
-    float *Y = Ls + TI * TIP;                          // TI x TIP  inverse under construction
-    float *Tm = Y + TI * TIP;                          // 64 * 64   products B A^-1 of one level
+    float *Ls = reinterpret_cast<float *>(smem_raw);   // TI x TIP: L, overwritten by its inverse
+    float *Tm = Ls + TI * TIP;                         // 64 * 64: B A^-1 of one level
     const int t = threadIdx.x;
+    g2s_block<float, 16>(Ls, TIP, A, lda, TI, TI, kb, kb);
+    __syncthreads();
     for (int idx = t; idx < TI * TI; idx += NT) {
         const int i = idx / TI, j = idx % TI;
-        float v;
-        if (j < i) v = (i < kb && j < kb) ? A[(long)i * lda + j] : 0.0f;
-        else v = (j == i) ? 1.0f : 0.0f;
-        Ls[i * TIP + j] = v;
-        Y[i * TIP + j] = 0.0f;
+        if (j >= i) Ls[i * TIP + j] = (j == i) ? 1.0f : 0.0f;
     }
     __syncthreads();
-    // 16 x 16 diagonal blocks: lane (t < 128) -> block t / 16, column t % 16
+    // 16 x 16 diagonal blocks: lane t < 128 -> block t / 16, column t % 16
+    float x[16];
+    const int b0 = (t / 16) * 16, c16 = t % 16;
     if (t < TI) {
-        const int b0 = (t / 16) * 16, c = t % 16;
-        float x[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            float s = 0.0f;
+            float sum = 0.0f;
 #pragma unroll
-            for (int k = 0; k < i; ++k) s += Ls[(b0 + i) * TIP + b0 + k] * x[k];
-            x[i] = (i < c) ? 0.0f : ((i == c) ? 1.0f : -s);
+            for (int k = 0; k < i; ++k) sum += Ls[(b0 + i) * TIP + b0 + k] * x[k];
+            x[i] = (i < c16) ? 0.0f : ((i == c16) ? 1.0f : -sum);
         }
+    }
+    __syncthreads();
+    if (t < TI) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-            if (i >= c) Y[(b0 + i) * TIP + b0 + c] = x[i];
+            if (i > c16) Ls[(b0 + i) * TIP + b0 + c16] = x[i];
     }
     __syncthreads();
     for (int sz = 16; sz < TI; sz *= 2) {
-        const int npair = TI / (2 * sz);
-        // Tm[p][i][j] = sum_k B[i][k] Ainv[k][j], B = L[C rows, A cols]
-        for (int o = t; o < npair * sz * sz; o += NT) {
-            const int pidx = o / (sz * sz), rem = o % (sz * sz), i = rem / sz, j = rem % sz;
+        const int tpr = sz / 4, tiles = (TI / (2 * sz)) * tpr * tpr;
+        // Tm[p] = B_p A_p^-1 with B_p = L[C rows, A cols], A_p^-1 = Ls[A, A]
+        for (int tile = t; tile < tiles; tile += NT) {
+            const int pidx = tile / (tpr * tpr), rem = tile % (tpr * tpr), ti = rem / tpr, tj = rem % tpr;
             const int a0 = 2 * pidx * sz, c0 = a0 + sz;
-            float acc = 0.0f;
-            for (int k = j; k < sz; ++k) acc += Ls[(c0 + i) * TIP + a0 + k] * Y[(a0 + k) * TIP + a0 + j];
-            Tm[o] = acc;
+            float acc[4][4] = {};
+            for (int k = 0; k < sz; ++k) {
+                float bv[4], av[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    bv[u] = Ls[(c0 + 4 * ti + u) * TIP + a0 + k];
+                    av[u] = Ls[(a0 + k) * TIP + a0 + 4 * tj + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += bv[u] * av[v];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Tm[pidx * sz * sz + (4 * ti + u) * sz + 4 * tj + v] = acc[u][v];
         }
         __syncthreads();
-        // Y[C rows, A cols] = -Cinv Tm
-        for (int o = t; o < npair * sz * sz; o += NT) {
-            const int pidx = o / (sz * sz), rem = o % (sz * sz), i = rem / sz, j = rem % sz;
+        // L[C rows, A cols] <- -C_p^-1 Tm[p]   (B_p has been consumed)
+        for (int tile = t; tile < tiles; tile += NT) {
+            const int pidx = tile / (tpr * tpr), rem = tile % (tpr * tpr), ti = rem / tpr, tj = rem % tpr;
             const int a0 = 2 * pidx * sz, c0 = a0 + sz;
             const float *tp = Tm + pidx * sz * sz;
-            float acc = 0.0f;
-            for (int k = 0; k <= i; ++k) acc += Y[(c0 + i) * TIP + c0 + k] * tp[k * sz + j];
-            Y[(c0 + i) * TIP + a0 + j] = -acc;
+            float acc[4][4] = {};
+            for (int k = 0; k < sz; ++k) {
+                float cv[4], tv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cv[u] = Ls[(c0 + 4 * ti + u) * TIP + c0 + k];
+                    tv[u] = tp[k * sz + 4 * tj + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += cv[u] * tv[v];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Ls[(c0 + 4 * ti + u) * TIP + a0 + 4 * tj + v] = -acc[u][v];
         }
         __syncthreads();
     }
-    for (int idx = t; idx < TI * TI; idx += NT) Linv[idx] = Y[(idx / TI) * TIP + idx % TI];
+    // transposed store: LinvT[k * TI + r] = inverse[r][k]
+    for (int idx = t; idx < TI * TI; idx += NT) {
+        const int r = idx % TI, k = idx / TI;
+        LinvT[idx] = Ls[r * TIP + k];
+    }
 }
 
-// U12 = Linv A12 for one 64-column strip per workgroup: Linv and the strip in
-// LDS, each thread 32 rows of one column.
+// U12 = L11^-1 A12 for one 64-column strip per workgroup.  Wave w owns rows
+// 32w .. 32w+31 of the strip (lane = column); for each k it reads those 32
+// entries of column k of L11^-1 as eight 16-byte broadcast LDS reads.
 __global__ __launch_bounds__(NT) void trsm_apply_kernel(float *A, long lda, int k0, int kb, int ncols_total,
-                                                        const float *Linv, const int *info)
+                                                        const float *LinvT, const int *info)
 {
     if (*info != 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float *Ls = reinterpret_cast<float *>(smem_raw);   // kb x TIP
-    float *Bs = Ls + TI * TIP;                         // kb x 65
-    const int t = threadIdx.x, lane = t & 63, rg = t >> 6;
+    float *Lt = reinterpret_cast<float *>(smem_raw);   // TI x TP4: Lt[k][r] = L11^-1[r][k]
+    float *Bs = Lt + TI * TP4;                         // TI x 65 strip (rows >= kb zero)
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int cbase = k0 + kb + blockIdx.x * 64;
-    for (int idx = t; idx < kb * TI; idx += NT) Ls[(idx / TI) * TIP + idx % TI] = Linv[idx];
-    g2s_block<float>(Bs, 65, A + (long)k0 * lda + cbase, lda, kb, 64, kb, ncols_total - cbase);
+    g2s_block<float, 16>(Lt, TP4, LinvT, TI, TI, TI, TI, TI);
+    g2s_block<float, 8>(Bs, 65, A + (long)k0 * lda + cbase, lda, TI, 64, kb, ncols_total - cbase);
     __syncthreads();
-    float acc[TI / 4];
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const int rbase = wave * 32;
+    float acc[32];
 #pragma unroll
-    for (int i = 0; i < TI / 4; ++i) acc[i] = 0.0f;
+    for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
+#pragma unroll 2
     for (int k = 0; k < kb; ++k) {
         const float bk = Bs[k * 65 + lane];
+        const f4v *lp = reinterpret_cast<const f4v *>(Lt + k * TP4 + rbase);
 #pragma unroll
-        for (int i = 0; i < TI / 4; ++i) {
-            const int r = rg + 4 * i;
-            if (r < kb) acc[i] += Ls[r * TIP + k] * bk;
+        for (int v = 0; v < 8; ++v) {
+            const f4v l = lp[v];
+            acc[4 * v] += l[0] * bk;
+            acc[4 * v + 1] += l[1] * bk;
+            acc[4 * v + 2] += l[2] * bk;
+            acc[4 * v + 3] += l[3] * bk;
         }
     }
     const int c = cbase + lane;
     if (c < ncols_total) {
 #pragma unroll
-        for (int i = 0; i < TI / 4; ++i) {
-            const int r = rg + 4 * i;
-            if (r < kb) A[(long)(k0 + r) * lda + c] = acc[i];
-        }
+        for (int i = 0; i < 32; ++i)
+            if (rbase + i < kb) A[(long)(k0 + rbase + i) * lda + c] = acc[i];
     }
 }
 
@@ -296,6 +337,7 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         w.seq_next += 512u;
         pa.pairs = w.pairs + (size_t)p * 4 * SCIRS_KB_MAX; pa.npairs = w.npairs + p;
         pa.info = w.info; pa.info_val = w.info_val; pa.abort = w.abort; pa.spin_limit = w.spin_limit;
+        pa.poll_sleep = w.poll_sleep;
         const size_t lds = panel::lds_bytes(R, kbw, sizeof(T));
         const void *pk = w.panel_w0 ? (const void *)panel::getf2_panel_kernel<T, true>
                                     : (const void *)panel::getf2_panel_kernel<T, false>;
@@ -334,12 +376,12 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
                 // FP32: L11^-1 once, then a parallel multiply
                 scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * nright,
                                     (0.5 * kbw * kbw + 2.0 * kbw * nright) * sizeof(T), stream);
-                const size_t lds_i = ((size_t)2 * TI * TIP + 64 * 64) * sizeof(float);
+                const size_t lds_i = ((size_t)TI * TIP + 64 * 64) * sizeof(float);
                 if (int rc = scirs_allow_lds((const void *)trtri_unit_lower_kernel, lds_i)) return rc;
                 hipLaunchKernelGGL(trtri_unit_lower_kernel, dim3(1), dim3(NT), lds_i, stream,
                                    (const float *)(A + (long)k0 * lda + k0), lda, kbw, (float *)w.linv,
                                    (const int *)w.info);
-                const size_t lds_a = ((size_t)TI * TIP + (size_t)TI * 65) * sizeof(float);
+                const size_t lds_a = ((size_t)TI * TP4 + (size_t)TI * 65) * sizeof(float);
                 if (int rc = scirs_allow_lds((const void *)trsm_apply_kernel, lds_a)) return rc;
                 hipLaunchKernelGGL(trsm_apply_kernel, dim3((nright + 63) / 64), dim3(NT), lds_a, stream,
                                    (float *)A, lda, k0, kbw, n, (const float *)w.linv, (const int *)w.info);

@@ -307,6 +307,7 @@ int lowp_mode_of(int mode)
     case SCIRS_MODE_FP16: return SCIRS_LOWP_FP16;
     case SCIRS_MODE_BF16X3: return SCIRS_LOWP_BF16X3;
     case SCIRS_MODE_FP32: return SCIRS_LOWP_FP32;
+    case SCIRS_MODE_BF16X6: return SCIRS_LOWP_BF16X6;
     default: return -1;
     }
 }
@@ -322,7 +323,7 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     int rc;
     if (mode == SCIRS_MODE_AUTO) {
         const char *m = getenv("SCIRS_GPU_MIXED_MODE");
-        mode = m && *m ? atoi(m) : SCIRS_MODE_BF16;
+        mode = m && *m ? atoi(m) : SCIRS_MODE_BF16X6;
     }
     const int lowp = lowp_mode_of(mode);
     if (lowp < 0) {

@@ -31,6 +31,7 @@ struct ScirsWork {
     void *linv;          // 128 x 128 inverse of a panel's unit-lower block (FP32 TRSM)
     int panel_w0;        // panel critical path on wave 0 alone (1) or on all waves (0)
     int lookahead;       // overlap panel k+1 with the rest of update k on side_stream
+    int poll_sleep;      // back-off of the panel's candidate poll (s_sleep(2) repetitions)
     hipStream_t side_stream;
     hipEvent_t ev_trsm, ev_upd, ev_start;
 };
@@ -40,6 +41,7 @@ enum {
     SCIRS_LOWP_BF16 = 1,   // BF16 operands, FP32 accumulation (MFMA)
     SCIRS_LOWP_FP16 = 2,   // FP16 operands with per-block power-of-two scaling, FP32 accumulation
     SCIRS_LOWP_BF16X3 = 3, // FP32 split into hi+lo BF16 pairs, three MFMA products
+    SCIRS_LOWP_BF16X6 = 4, // FP32 split into hi+mid+lo BF16 terms, six MFMA products (FP32-accurate)
 };
 
 enum { SCIRS_KB_MAX = 256, SCIRS_MAX_G = 512 };

@@ -82,6 +82,7 @@ struct PanelArgs {
     int *pairs; int *npairs;
     int *info; double *info_val; int *abort;
     unsigned spin_limit;
+    int poll_sleep;   // s_sleep(2) repetitions after an unsuccessful candidate poll
 };
 
 // W0 selects where the column's critical path runs: false = all four waves
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 wgw = __shfl(wg, src, 64);
                 if (__all(ok)) { row_ok = wgw >= 0 && wgw == guess && __all(rok); break; }
                 guess = wgw;
+                for (int z = 0; z < a.poll_sleep; ++z) __builtin_amdgcn_s_sleep(2);
                 if (++spins > a.spin_limit ||
                     __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     timed_out = true;

@@ -24,7 +24,7 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 
 U64 = 2.0 ** -53
-MODES = {"bf16": 1, "fp16": 2, "bf16x3": 3, "fp32": 4}
+MODES = {"bf16": 1, "fp16": 2, "bf16x3": 3, "fp32": 4, "bf16x6": 5}
 
 
 @pytest.fixture(scope="module")
