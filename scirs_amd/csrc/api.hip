@@ -132,9 +132,11 @@ int Context::init(int dev)
     work.panel_w0 = env_int("SCIRS_GPU_PANEL_W0", 0);
     work.lookahead = env_int("SCIRS_GPU_LOOKAHEAD", 1);
     work.poll_sleep = env_int("SCIRS_GPU_POLL_SLEEP", 0);
+    work.panel_nrep = std::max(1, std::min((int)SCIRS_PANEL_REP_MAX, env_int("SCIRS_GPU_PANEL_REPLICAS", 1)));
     size_t bytes = 0;
-    const size_t xw_b = 2 * (size_t)max_g * 4 * 8, xrow_b = 2 * (size_t)max_g * kbmax * 2 * 8,
-                 xrj_b = 2 * (size_t)kbmax * 2 * 8, flags_b = 64 * sizeof(int),
+    const size_t rep = SCIRS_PANEL_REP_MAX;
+    const size_t xw_b = rep * 2 * (size_t)max_g * 4 * 8, xrow_b = rep * 2 * (size_t)max_g * kbmax * 2 * 8,
+                 xrj_b = rep * 2 * (size_t)kbmax * 2 * 8, flags_b = 64 * sizeof(int),
                  pairs_b = (size_t)work.max_panels * 4 * kbmax * sizeof(int),
                  npairs_b = (size_t)work.max_panels * sizeof(int), linv_b = 128 * 128 * sizeof(double);
     for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b, linv_b}) bytes += (b + 255) & ~(size_t)255;

@@ -332,7 +332,7 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         G = (mrows + R - 1) / R;
         panel::PanelArgs<T> pa;
         pa.A = A; pa.lda = lda; pa.k0 = k0; pa.m = mrows; pa.kb = kbw; pa.R = R; pa.G = G;
-        pa.xw = w.xw; pa.xrow = w.xrow; pa.xrj = w.xrj;
+        pa.xw = w.xw; pa.xrow = w.xrow; pa.xrj = w.xrj; pa.nrep = w.panel_nrep;
         pa.seq_base = w.seq_next;
         w.seq_next += 512u;
         pa.pairs = w.pairs + (size_t)p * 4 * SCIRS_KB_MAX; pa.npairs = w.npairs + p;

@@ -11,9 +11,10 @@ int scirs_launch_dgemm_sub(int M, int N, int K, const double *A, long lda, const
 
 // Device scratch used by the factorisations (allocated once per context).
 struct ScirsWork {
-    unsigned long long *xw;    // [2][MAX_G][4]        panel candidates (sequence-tagged words)
-    unsigned long long *xrow;  // [2][MAX_G][KB_MAX*2] panel candidate rows
-    unsigned long long *xrj;   // [2][KB_MAX*2]        panel row j
+    unsigned long long *xw;    // [REP][2][MAX_G][4]        panel candidates (sequence-tagged words)
+    unsigned long long *xrow;  // [REP][2][MAX_G][KB_MAX*2] panel candidate rows
+    unsigned long long *xrj;   // [REP][2][KB_MAX*2]        panel row j
+    int panel_nrep;            // replicas of the panel's published words (1..SCIRS_PANEL_REP_MAX)
     unsigned seq_next;         // sequence base of the next panel launch (host side)
     int *abort;          // raised by a timed-out exchange
     int *pairs;          // [n_panels_max][2*KB_MAX][2] row gather lists (dst, src), absolute
@@ -44,7 +45,7 @@ enum {
     SCIRS_LOWP_BF16X6 = 4, // FP32 split into hi+mid+lo BF16 terms, six MFMA products (FP32-accurate)
 };
 
-enum { SCIRS_KB_MAX = 256, SCIRS_MAX_G = 512 };
+enum { SCIRS_KB_MAX = 256, SCIRS_MAX_G = 512, SCIRS_PANEL_REP_MAX = 8 };
 
 // Trailing (Schur) update C -= A B of the factorisations: FP64 MFMA for double,
 // the arithmetic chosen by w.lowp_mode for float.

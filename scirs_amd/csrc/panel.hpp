@@ -75,9 +75,10 @@ template <typename T>
 struct PanelArgs {
     T *A; long lda;
     int k0, m, kb, R, G;
-    u64 *xw;       // [2][G][4]   candidate: (seq|row), value words
-    u64 *xrow;     // [2][G][kb*W] candidate rows
-    u64 *xrj;      // [2][kb*W]   row j of the panel
+    u64 *xw;       // [nrep][2][G][4]   candidate: (seq|row), value words
+    u64 *xrow;     // [nrep][2][G][kb*W] candidate rows
+    u64 *xrj;      // [nrep][2][kb*W]   row j of the panel
+    int nrep;      // replicas of every published word; workgroup g reads replica g % nrep
     unsigned seq_base;
     int *pairs; int *npairs;
     int *info; double *info_val; int *abort;
@@ -113,6 +114,14 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
     const int row0 = g * a.R;
     const int Rg = max(0, min(a.R, a.m - row0));
     T *Ap = a.A + (long)a.k0 * a.lda + a.k0;
+    // Every published word is written to nrep replicas and workgroup g reads
+    // replica g % nrep (with round-robin placement, its XCD's copy): the polls
+    // of G workgroups spread over nrep times more cache lines.
+    const int nrep = a.nrep, myrep = g % nrep;
+    const size_t rs_w = (size_t)2 * G * 4, rs_r = (size_t)2 * G * kb * W, rs_j = (size_t)2 * kb * W;
+    auto XW = [&](int par, int q, int r) { return a.xw + r * rs_w + ((size_t)par * G + q) * 4; };
+    auto XR = [&](int par, int q, int r) { return a.xrow + r * rs_r + ((size_t)par * G + q) * (size_t)kb * W; };
+    auto XJ = [&](int par, int r) { return a.xrj + r * rs_j + (size_t)par * kb * W; };
 
     g2s_block<T>(tile, kbp, Ap + (long)row0 * a.lda, a.lda, Rg, kb, Rg, kb);
     if (g == 0) {
@@ -147,17 +156,19 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         const int b = j & 1;
         const unsigned seq = a.seq_base + (unsigned)j + 1u;
         const int c = s_c;
-        u64 *xr = a.xrow + ((size_t)b * G + g) * (size_t)kb * W;
-        if (c >= 0)
-            for (int q = t; q < kb; q += NT) Tag<T>::put(xr + (size_t)q * W, seq, tile[c * kbp + q]);
-        if (j >= row0 && j < row0 + Rg) {
-            u64 *xj = a.xrj + (size_t)b * kb * W;
-            for (int q = t; q < kb; q += NT) Tag<T>::put(xj + (size_t)q * W, seq, tile[(j - row0) * kbp + q]);
-        }
-        if (t == 0) {
-            u64 *xw = a.xw + ((size_t)b * G + g) * 4;
-            st_agent(xw, ((u64)seq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
-            Tag<T>::put(xw + 1, seq, s_cv);
+        for (int r = 0; r < nrep; ++r) {
+            u64 *xr = XR(b, g, r);
+            if (c >= 0)
+                for (int q = t; q < kb; q += NT) Tag<T>::put(xr + (size_t)q * W, seq, tile[c * kbp + q]);
+            if (j >= row0 && j < row0 + Rg) {
+                u64 *xj = XJ(b, r);
+                for (int q = t; q < kb; q += NT) Tag<T>::put(xj + (size_t)q * W, seq, tile[(j - row0) * kbp + q]);
+            }
+            if (t == 0) {
+                u64 *xw = XW(b, g, r);
+                st_agent(xw, ((u64)seq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
+                Tag<T>::put(xw + 1, seq, s_cv);
+            }
         }
     };
 
@@ -172,29 +183,34 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         // ---- 1. wave 0 reduces the candidate list (the same in every workgroup) and,
         //         in the same polls, fetches the current best candidate's row and row j,
         //         so the pivot row usually arrives with the last candidate.  Waves 1-3
-        //         are meanwhile finishing the previous column's bulk update.
+        //         are meanwhile finishing the previous column's bulk update.  (More
+        //         polling waves were measured slower: they add contention on the
+        //         candidate lines.)
         if (wave == 0) {
+            bool mine = false;
             unsigned spins = 0;
             T vv = (T)0; int ii = -1, wgw = -1;
             bool timed_out = false, row_ok = false;
             T sp[SPEC], sj[SPEC];
-            const u64 *xj = a.xrj + (size_t)b * kb * W;
+            const u64 *xj = XJ(b, myrep);
             int guess = -1;   // best workgroup of the previous poll: its row is read with this poll's records
             while (true) {
                 bool ok = true;
                 T v = (T)0; int i = -1, wg = -1;
                 bool rok = guess >= 0;
-                const u64 *xr = a.xrow + ((size_t)b * G + (guess >= 0 ? guess : 0)) * (size_t)kb * W;
+                const u64 *xr = XR(b, guess >= 0 ? guess : 0, myrep);
+                // row j is needed only by the workgroup that holds the pivot row
+                const bool want_j = guess == g;
 #pragma unroll
                 for (int u = 0; u < SPEC; ++u) {
                     const int q = lane + 64 * u;
                     if (q < kb) {
                         if (guess >= 0) rok = Tag<T>::get(xr + (size_t)q * W, seq, sp[u]) && rok;
-                        rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
+                        if (want_j) rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
                     }
                 }
                 for (int q = lane; q < G; q += 64) {
-                    const u64 *xw = a.xw + ((size_t)b * G + q) * 4;
+                    const u64 *xw = XW(b, q, myrep);
                     const u64 w0 = ld_agent(xw);
                     T qv;
                     const bool okv = Tag<T>::get(xw + 1, seq, qv);
@@ -206,19 +222,21 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 wave_argmax(vv, ii);
                 const unsigned long long ball = __ballot(i == ii && i >= 0);
                 const int src = ball ? __ffsll(ball) - 1 : 0;
-                wgw = __shfl(wg, src, 64);
-                if (__all(ok)) { row_ok = wgw >= 0 && wgw == guess && __all(rok); break; }
+                wgw = __builtin_amdgcn_readlane(wg, src);
+                if (__all(ok)) { row_ok = wgw >= 0 && wgw == guess && __all(rok); mine = true; break; }
                 guess = wgw;
                 for (int z = 0; z < a.poll_sleep; ++z) __builtin_amdgcn_s_sleep(2);
                 if (++spins > a.spin_limit ||
                     __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     timed_out = true;
+                    mine = true;
                     break;
                 }
             }
             // the guess missed: read the winner's row (and row j) now
-            if (!row_ok && !timed_out && wgw >= 0 && vv >= eps) {
-                const u64 *xr = a.xrow + ((size_t)b * G + wgw) * (size_t)kb * W;
+            if (mine && !row_ok && !timed_out && wgw >= 0 && vv >= eps) {
+                const u64 *xr = XR(b, wgw, myrep);
+                const bool want_j = wgw == g;
                 while (true) {
                     bool rok = true;
 #pragma unroll
@@ -226,21 +244,25 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                         const int q = lane + 64 * u;
                         if (q < kb) {
                             rok = Tag<T>::get(xr + (size_t)q * W, seq, sp[u]) && rok;
-                            rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
+                            if (want_j) rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
                         }
                     }
                     if (__all(rok)) { row_ok = true; break; }
                     if (++spins > a.spin_limit) { timed_out = true; break; }
                 }
             }
-            if (row_ok) {
+            if (mine && row_ok) {
+                const bool have_j = wgw == g;
 #pragma unroll
                 for (int u = 0; u < SPEC; ++u) {
                     const int q = lane + 64 * u;
-                    if (q < kb) { prow[q] = sp[u]; rjv[q] = sj[u]; }
+                    if (q < kb) {
+                        prow[q] = sp[u];
+                        if (have_j) rjv[q] = sj[u];
+                    }
                 }
             }
-            if (lane == 0) {
+            if (mine && lane == 0) {
                 s_p = ii; s_g = wgw; s_pv = vv;
                 if (timed_out) {
                     __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -312,17 +334,19 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
             }
             const int jj = j + 1, bb = jj & 1;
             const unsigned sq = a.seq_base + (unsigned)jj + 1u;
-            u64 *xr = a.xrow + ((size_t)bb * G + g) * (size_t)kb * W;
-            if (c >= 0)
-                for (int q = lane; q < kb; q += 64) Tag<T>::put(xr + (size_t)q * W, sq, tile[c * kbp + q]);
-            if (jn >= 0) {
-                u64 *xj = a.xrj + (size_t)bb * kb * W;
-                for (int q = lane; q < kb; q += 64) Tag<T>::put(xj + (size_t)q * W, sq, tile[jn * kbp + q]);
-            }
-            if (lane == 0) {
-                u64 *xw = a.xw + ((size_t)bb * G + g) * 4;
-                st_agent(xw, ((u64)sq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
-                Tag<T>::put(xw + 1, sq, s_cv);
+            for (int r = 0; r < nrep; ++r) {
+                u64 *xr = XR(bb, g, r);
+                if (c >= 0)
+                    for (int q = lane; q < kb; q += 64) Tag<T>::put(xr + (size_t)q * W, sq, tile[c * kbp + q]);
+                if (jn >= 0) {
+                    u64 *xj = XJ(bb, r);
+                    for (int q = lane; q < kb; q += 64) Tag<T>::put(xj + (size_t)q * W, sq, tile[jn * kbp + q]);
+                }
+                if (lane == 0) {
+                    u64 *xw = XW(bb, g, r);
+                    st_agent(xw, ((u64)sq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
+                    Tag<T>::put(xw + 1, sq, s_cv);
+                }
             }
         } else if (j + 2 < kb) {
             // ---- 4. bulk rank-1 update of the other rows (waves 1-3), overlapping
@@ -340,22 +364,24 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         const int p = s_p;
         const bool own_j = (j >= row0 && j < row0 + Rg);
         const bool own_p = (p >= row0 && p < row0 + Rg);
-        // ---- 3. row interchange inside the panel and the permutation record
+        // ---- 3. row interchange inside the panel (the permutation record is kept
+        //         later, off the critical path)
         if (p != j) {
             if (own_j)
                 for (int c = t; c < kb; c += NT) tile[(j - row0) * kbp + c] = prow[c];
             if (own_p)
                 for (int c = t; c < kb; c += NT) tile[(p - row0) * kbp + c] = rjv[c];
         }
-        if (g == 0 && wave == 0 && p != j) {
+        // workgroup 0 records row interchanges as labels (wave 0)
+        auto record_swap = [&]() {
             if (p < kb) {
                 if (lane == 0) { int tmp = top_label[j]; top_label[j] = top_label[p]; top_label[p] = tmp; }
             } else {
                 const int nout = s_nout;
                 int found = -1;
                 for (int base = 0; base < nout && found < 0; base += 64) {
-                    const int s = base + lane;
-                    const unsigned long long bl = __ballot(s < nout && out_pos[s] == p);
+                    const int sidx = base + lane;
+                    const unsigned long long bl = __ballot(sidx < nout && out_pos[sidx] == p);
                     if (bl) found = base + __ffsll(bl) - 1;
                 }
                 if (lane == 0) {
@@ -363,7 +389,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                     int tmp = top_label[j]; top_label[j] = out_label[found]; out_label[found] = tmp;
                 }
             }
-        }
+        };
         __syncthreads();
         // ---- 4. multipliers by true division (lapack.rs:138), column j+1, and this
         //         thread's arg-max of column j+1, in one pass over the rows
@@ -382,28 +408,48 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 if (bi < 0 || fabs(x) > bv) { bv = fabs(x); bi = r; }
             }
         }
-        if (!more) break;
+        if (!more) {
+            if (g == 0 && wave == 0 && p != j) record_swap();
+            break;
+        }
         wave_argmax(bv, bi);
         if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
         __syncthreads();
-        if (t == 0) {
-            T v = red_v[0]; int i = red_i[0];
-            for (int w = 1; w < NT / 64; ++w) {
-                const int oi = red_i[w]; const T ov = red_v[w];
-                if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) { v = ov; i = oi; }
-            }
-            s_c = i; s_cv = v;
+        // every thread combines the four wave results itself (lowest row wins ties)
+        T cv = red_v[0]; int c = red_i[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) {
+            const int oi = red_i[w]; const T ov = red_v[w];
+            if (oi >= 0 && (c < 0 || ov > cv || (ov == cv && oi < c))) { cv = ov; c = oi; }
         }
-        __syncthreads();
-        // ---- 5. the next candidate row and row j+1 brought fully up to date, then published
-        const int c = s_c;
+        // ---- 5. the next candidate row and row j+1: each element brought up to
+        //         date and published by the same thread (readers check every
+        //         word's tag, so no barrier is needed before publishing)
         const int jn = (j + 1 >= row0 && j + 1 < row0 + Rg) ? j + 1 - row0 : -1;
-        for (int q = j + 2 + t; q < kb; q += NT) {
-            if (c >= 0) tile[c * kbp + q] -= lcol[c] * prow[q];
-            if (jn >= 0 && jn != c) tile[jn * kbp + q] -= lcol[jn] * prow[q];
+        {
+            const int bb = (j + 1) & 1;
+            const unsigned sq = a.seq_base + (unsigned)j + 2u;
+            for (int q = t; q < kb; q += NT) {
+                if (c >= 0) {
+                    T v = tile[c * kbp + q];
+                    if (q >= j + 2) { v -= lcol[c] * prow[q]; tile[c * kbp + q] = v; }
+                    for (int r = 0; r < nrep; ++r) Tag<T>::put(XR(bb, g, r) + (size_t)q * W, sq, v);
+                }
+                if (jn >= 0) {
+                    T v = tile[jn * kbp + q];
+                    if (q >= j + 2 && jn != c) { v -= lcol[jn] * prow[q]; tile[jn * kbp + q] = v; }
+                    for (int r = 0; r < nrep; ++r) Tag<T>::put(XJ(bb, r) + (size_t)q * W, sq, v);
+                }
+            }
+            if (t == 0) {
+                for (int r = 0; r < nrep; ++r) {
+                    u64 *xw = XW(bb, g, r);
+                    st_agent(xw, ((u64)sq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
+                    Tag<T>::put(xw + 1, sq, cv);
+                }
+            }
         }
-        __syncthreads();
-        publish(j + 1);
+        if (g == 0 && wave == 0 && p != j) record_swap();
         // ---- 6. bulk rank-1 update of the other rows by waves 1-3, overlapping the
         //         exchange; wave 0 goes straight to the next column's poll.  The next
         //         column writes the other parity of prow2, and its first barrier
