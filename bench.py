@@ -40,8 +40,12 @@ def parse():
     ap.add_argument("--kappa", type=float, default=1e4)
     ap.add_argument("--mode", default=os.environ.get("SCIRS_BENCH_MODE", "bf16x6"), choices=sorted(MODES))
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--matrix", default="conditioned", choices=["conditioned", "gaussian"],
+                    help="conditioned: U diag(sigma) V^T with exact kappa (default); gaussian: i.i.d. N(0,1) "
+                         "entries, no QR in the set-up (for counter-collection runs, whose kernels' traffic "
+                         "depends on the shape only)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 LU comparison line")
-    ap.add_argument("--cpu-n", type=int, default=2560, help="size of the CPU baseline sample")
+    ap.add_argument("--cpu-n", type=int, default=4096, help="size of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="mixed", choices=["mixed", "batched"],
                     help="mixed: the headline n=16384 mixed-precision solve; batched: 20,000 uneven "
@@ -50,12 +54,13 @@ def parse():
     ap.add_argument("--loop-sample", type=int, default=1000,
                     help="systems solved one launch sequence at a time for the per-matrix baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB three times (scales rocprofv3 "
-                         "FETCH_SIZE / WRITE_SIZE to bytes) and exit")
+                    help="after the measurement, run a plain device copy of MIB MiB three times "
+                         "(a kernel of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE "
+                         "to bytes); its note goes to stderr")
     return ap.parse_args()
 
 
-def copy_calibration(mib):
+def copy_calibration(mib, out=sys.stdout):
     """A plain copy kernel of known size: reads and writes exactly mib MiB."""
     import torch
     n = mib * (1 << 20) // 4
@@ -65,7 +70,7 @@ def copy_calibration(mib):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes": n * 4, "dispatches": 3}), flush=True)
+    print(json.dumps({"copy_calibration_bytes": n * 4, "dispatches": 3}), file=out, flush=True)
 
 
 PROFILES = os.path.join(ROOT, "profiles")
@@ -91,10 +96,16 @@ def measured_traffic(kernel_class):
     return None
 
 
-def make_system(n, kappa, seed, torch):
+def make_system(n, kappa, seed, torch, kind="conditioned"):
     """A = U diag(sigma) V^T (exact kappa_2), b = A x_true, all FP64 on the GPU."""
     dev = "cuda"
     g = torch.Generator(device=dev).manual_seed(seed)
+    if kind == "gaussian":
+        a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+        x_true = torch.randn(n, 1, dtype=torch.float64, device=dev, generator=g)
+        b = (a @ x_true).contiguous()
+        torch.cuda.synchronize()
+        return a, b, x_true
     u, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
     v, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
     sigma = torch.logspace(0.0, float(torch.log10(torch.tensor(kappa))), n, dtype=torch.float64, device=dev)
@@ -255,9 +266,6 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solve engine does not shard "
               "across GPUs. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
     if args.workload == "batched":
         bench_batched(args)
         return
@@ -266,7 +274,7 @@ def main():
 
     n = args.n
     flops = 2.0 * n ** 3 / 3.0
-    a, b, x_true = make_system(n, args.kappa, args.seed, torch)
+    a, b, x_true = make_system(n, args.kappa, args.seed, torch, args.matrix)
     x = torch.empty_like(b)
     mode = MODES[args.mode]
     tol = n * 2.0 ** -53
@@ -350,7 +358,8 @@ def main():
         "config": {
             "workload": f"mixed-precision IR LU solve, n={n}, 1 rhs, FP64 input, kappa_2={args.kappa:g} "
                         f"(A = U diag(logspace) V^T), FP32 panel, {args.mode} trailing update, FP64 residual",
-            "n": n, "nrhs": 1, "kappa": args.kappa, "mode": args.mode, "seed": args.seed,
+            "n": n, "nrhs": 1, "kappa": args.kappa if args.matrix == "conditioned" else None,
+            "matrix": args.matrix, "mode": args.mode, "seed": args.seed,
         },
         "backward_error": bwd,
         "backward_error_target": tol,
@@ -369,6 +378,8 @@ def main():
     if bwd > tol:
         line["warning"] = f"backward error {bwd:.3e} above target {tol:.3e}: value is not at FP64 accuracy"
     print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, out=sys.stderr)
 
 
 if __name__ == "__main__":
