@@ -112,6 +112,14 @@ def load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m scirs_amd.build` "
             "(there is no CPU fallback)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same soname),
+    # which it cannot share if the system one is loaded first, while this
+    # library binds to whichever copy is already resident.  So torch, when
+    # installed, is loaded before the library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (argtypes, restype) in SIGNATURES.items():
         fn = getattr(lib, name, None)
