@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--kappa", type=float, default=1e4)
+    ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--mode", default=os.environ.get("SCIRS_BENCH_MODE", "bf16x6"), choices=sorted(MODES))
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--matrix", default="conditioned", choices=["conditioned", "gaussian"],
@@ -96,13 +97,13 @@ def measured_traffic(kernel_class):
     return None
 
 
-def make_system(n, kappa, seed, torch, kind="conditioned"):
+def make_system(n, kappa, seed, torch, kind="conditioned", nrhs=1):
     """A = U diag(sigma) V^T (exact kappa_2), b = A x_true, all FP64 on the GPU."""
     dev = "cuda"
     g = torch.Generator(device=dev).manual_seed(seed)
     if kind == "gaussian":
         a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
-        x_true = torch.randn(n, 1, dtype=torch.float64, device=dev, generator=g)
+        x_true = torch.randn(n, nrhs, dtype=torch.float64, device=dev, generator=g)
         b = (a @ x_true).contiguous()
         torch.cuda.synchronize()
         return a, b, x_true
@@ -111,7 +112,7 @@ def make_system(n, kappa, seed, torch, kind="conditioned"):
     sigma = torch.logspace(0.0, float(torch.log10(torch.tensor(kappa))), n, dtype=torch.float64, device=dev)
     a = ((u * sigma) @ v.T).contiguous()
     del u, v
-    x_true = torch.randn(n, 1, dtype=torch.float64, device=dev, generator=g)
+    x_true = torch.randn(n, nrhs, dtype=torch.float64, device=dev, generator=g)
     b = (a @ x_true).contiguous()
     torch.cuda.synchronize()
     return a, b, x_true
@@ -273,8 +274,8 @@ def main():
     from scirs_amd import linalg as L
 
     n = args.n
-    flops = 2.0 * n ** 3 / 3.0
-    a, b, x_true = make_system(n, args.kappa, args.seed, torch, args.matrix)
+    flops = 2.0 * n ** 3 / 3.0 + 2.0 * n * n * args.nrhs
+    a, b, x_true = make_system(n, args.kappa, args.seed, torch, args.matrix, args.nrhs)
     x = torch.empty_like(b)
     mode = MODES[args.mode]
     # FP64 backward error the solve must reach: n 2^-53, and at most 1e-14 (the
@@ -358,9 +359,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"mixed-precision IR LU solve, n={n}, 1 rhs, FP64 input, kappa_2={args.kappa:g} "
+            "workload": f"mixed-precision IR LU solve, n={n}, {args.nrhs} rhs, FP64 input, kappa_2={args.kappa:g} "
                         f"(A = U diag(logspace) V^T), FP32 panel, {args.mode} trailing update, FP64 residual",
-            "n": n, "nrhs": 1, "kappa": args.kappa if args.matrix == "conditioned" else None,
+            "n": n, "nrhs": args.nrhs, "kappa": args.kappa if args.matrix == "conditioned" else None,
             "matrix": args.matrix, "mode": args.mode, "seed": args.seed,
         },
         "backward_error": bwd,

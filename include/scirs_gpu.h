@@ -113,7 +113,7 @@ int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, const double 
  * normwise backward error is <= tol (tol <= 0: n * 2^-53), GMRES-IR
  * (restarted, preconditioned by the low-precision factors) when refinement
  * stalls, and FP64 LU when both fail.  max_iter / max_gmres <= 0 take the
- * defaults (30 refinement steps, 300 GMRES iterations). */
+ * defaults (30 refinement steps; 300 GMRES iterations per right-hand side). */
 int scirs_gpu_dgesv_ir(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
                        double *x, int64_t ldx, int mode, int max_iter, int max_gmres, double tol,
                        scirs_solve_info *info);

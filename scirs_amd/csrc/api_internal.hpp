@@ -27,6 +27,7 @@ struct Context {
     void *scratch_ir = nullptr; size_t scratch_ir_bytes = 0;
     void *scratch_ir2 = nullptr; size_t scratch_ir2_bytes = 0;
     void *xtri_mem = nullptr; size_t xtri_bytes = 0;
+    void *colexp_mem = nullptr; size_t colexp_bytes = 0;
 
     int init(int dev);
     int ensure(void **p, size_t *have, size_t bytes);

@@ -329,6 +329,30 @@ int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const 
     }
 }
 
+namespace {
+// Per-column power-of-two exponent of the FP16 update's right operand: column c
+// of B (K x N, row-major) is multiplied by 2^-colexp[c] before rounding to FP16,
+// which puts its largest entry in [2^14, 2^15) (FP16 tops out below 2^16), and
+// the product is multiplied back by 2^colexp[c] in FP32: exact scalings.
+__global__ void colexp_kernel(int K, int N, const float *B, long ldb, int *colexp, const int *info)
+{
+    if (info && *info != 0) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    float m = 0.0f;
+    for (int k = 0; k < K; ++k) m = fmaxf(m, fabsf(B[(long)k * ldb + c]));
+    colexp[c] = (m > 0.0f && m < INFINITY) ? ilogbf(m) - 14 : 0;
+}
+} // namespace
+
+int scirs_launch_colexp(int K, int N, const float *B, long ldb, int *colexp, const int *info, hipStream_t stream)
+{
+    if (N <= 0) return SCIRS_OK;
+    hipLaunchKernelGGL(colexp_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, K, N, B, ldb, colexp, info);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+
 int scirs_launch_sgemm_fp16_scaled(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
                                    float *C, long ldc, const int *colexp, int a_shift, const int *info,
                                    hipStream_t stream)

@@ -212,9 +212,14 @@ int ir_step(IrState &st, int j, double rn)
 
 // Restarted right-preconditioned GMRES on A d = R_j (FP64 Arnoldi), X_j += d.
 // Returns the number of inner iterations used in *iters.
+// Restarted GMRES on A d = r_j, right-preconditioned by the low-precision LU.
+// *stalled is set when a whole restart cycle fails to halve the true residual:
+// the preconditioner is then too weak for this system, and the caller stops
+// spending iterations on it.
 int gmres_correction(IrState &st, int j, int restart, int max_inner, double rtol, double *V, double *Z, double *w,
-                     double *hdev, int *iters)
+                     double *hdev, int *iters, bool *stalled)
 {
+    *stalled = false;
     const int n = st.n;
     // r0 = R_j (copy of column j)
     std::vector<double> H((size_t)(restart + 1) * restart, 0.0), cs(restart), sn(restart), g(restart + 1);
@@ -291,10 +296,12 @@ int gmres_correction(IrState &st, int j, int restart, int max_inner, double rtol
         // restart from the new residual of this column
         if ((rc = scirs_residual(n, 1, st.A, st.lda, st.X + j, st.ldx, st.B + j, st.ldb, V, 1, st.s))) return rc;
         hipLaunchKernelGGL(multidot_kernel, dim3(1), dim3(NT), 0, st.s, n, 1, V, V, hdev);
+        const double beta_prev = beta;
         SCIRS_HIP_CHECK(hipMemcpyAsync(&beta, hdev, sizeof(double), hipMemcpyDeviceToHost, st.s));
         SCIRS_HIP_CHECK(hipStreamSynchronize(st.s));
         beta = std::sqrt(beta);
         if (!(beta > 0)) break;
+        if (!(beta < 0.5 * beta_prev)) { *stalled = true; break; }
     }
     *iters = total;
     return SCIRS_OK;
@@ -368,6 +375,11 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     if ((rc = scirs_colmax_d(n, nrhs, dB, ldb, st.norms + 128, s))) return rc;
     // 2. low-precision LU
     if ((rc = c.reset_flags(s))) return rc;
+    if (lowp == SCIRS_LOWP_FP16) {
+        if ((rc = c.ensure(&c.colexp_mem, &c.colexp_bytes, 2 * (size_t)n * sizeof(int)))) return rc;
+        c.work.colexp = (int *)c.colexp_mem;
+        c.work.colexp_half = n;
+    }
     const int saved_mode = c.work.lowp_mode;
     c.work.lowp_mode = lowp;
     rc = scirs_getrf_device<float>(n, n, st.LU, n, st.kb, c.work, s);
@@ -393,13 +405,15 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
         // 3. refinement from X = 0
         hipLaunchKernelGGL(zero_ld_kernel, dim3(nb((long)n * nrhs)), dim3(256), 0, s, n, nrhs, dX, ldx);
         std::vector<double> eta, rn, prev(nrhs, INFINITY);
-        std::vector<int> use_gmres(nrhs, 0);
+        std::vector<int> use_gmres(nrhs, 0), gm_col(nrhs, 0);   // GMRES budget per right-hand side
         for (int it = 0; it <= max_iter; ++it) {
             if ((rc = backward_errors(st, eta, rn))) return rc;
             worst = 0;
             for (double e : eta) worst = std::max(worst, e);
             if (worst <= tol) { converged = true; break; }
-            if (it == max_iter || gm_iters >= max_gmres) break;
+            bool budget_left = false;
+            for (int j = 0; j < nrhs; ++j) budget_left = budget_left || (eta[j] > tol && gm_col[j] < max_gmres);
+            if (it == max_iter || !budget_left) break;
             ++ir_iters;
             for (int j = 0; j < nrhs; ++j) {
                 if (eta[j] <= tol) continue;
@@ -409,10 +423,14 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
                     int used = 0;
                     path = SCIRS_PATH_MIXED_GMRES;
                     const double rtol = std::max(1e-12, std::min(0.5, tol * 16 / std::max(eta[j], 1e-300)));
-                    if ((rc = gmres_correction(st, j, restart, max_gmres - gm_iters, std::max(rtol, 1e-10), V, Z, w,
-                                               hdev, &used)))
+                    if (gm_col[j] >= max_gmres) continue;
+                    bool stalled = false;
+                    if ((rc = gmres_correction(st, j, restart, max_gmres - gm_col[j], std::max(rtol, 1e-10), V, Z, w,
+                                               hdev, &used, &stalled)))
                         return rc;
                     gm_iters += used;
+                    gm_col[j] += used;
+                    if (stalled) gm_col[j] = max_gmres;   // no further GMRES for this column
                 } else {
                     if ((rc = ir_step(st, j, rn[j]))) return rc;
                 }

@@ -31,6 +31,8 @@ struct ScirsWork {
     int *ticket;         // chained triangular solve: block-row ticket counter
     void *linv;          // 128 x 128 inverse of a panel's unit-lower block (FP32 TRSM)
     int panel_w0;        // panel critical path on wave 0 alone (1) or on all waves (0)
+    int *colexp;         // FP16 update: per-column exponents of U12, two regions (main / side stream)
+    int colexp_half;     // columns per region
     int lookahead;       // overlap panel k+1 with the rest of update k on side_stream
     int poll_sleep;      // back-off of the panel's candidate poll (s_sleep(2) repetitions)
     hipStream_t side_stream;
@@ -70,6 +72,13 @@ int scirs_trsv_device(int n, int nrhs, const T *A, long lda, T *B, long ldb, int
 int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
                            int trans_b, float *C, long ldc, int lower, int mode, const int *info,
                            hipStream_t stream);
+
+// FP16 update with power-of-two scalings: A times 2^a_shift, column c of B times
+// 2^-colexp[c] (both undone exactly in FP32); colexp from scirs_launch_colexp.
+int scirs_launch_sgemm_fp16_scaled(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
+                                   float *C, long ldc, const int *colexp, int a_shift, const int *info,
+                                   hipStream_t stream);
+int scirs_launch_colexp(int K, int N, const float *B, long ldb, int *colexp, const int *info, hipStream_t stream);
 
 // Triangular solve with 256-row diagonal blocks (two launches per block):
 // X <- op(A)^-1 X with op the lower (unit or not) or upper triangle of A.
