@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 LU comparison line")
     ap.add_argument("--cpu-n", type=int, default=4096, help="size of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "batched"],
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "batched", "fp64"],
                     help="mixed: the headline n=16384 mixed-precision solve; batched: 20,000 uneven "
                          "systems in one launch against a per-matrix launch loop")
     ap.add_argument("--batch", type=int, default=20000)
@@ -261,6 +261,44 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_fp64(args):
+    """Plain FP64 solves on the GPU: LU (`solve_multiple`) of a N(0,1) system and
+    Cholesky (`cholesky` + two substitutions) of G G^T / n + I, n x nrhs
+    right-hand sides, A already in HBM."""
+    import torch
+    from scirs_amd import linalg as L
+    n, nrhs = args.n, args.nrhs
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    a = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    gg = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    s = (gg @ gg.T / n + torch.eye(n, dtype=torch.float64, device="cuda")).contiguous()
+    del gg
+    b = torch.randn(n, nrhs, dtype=torch.float64, device="cuda", generator=g)
+    x = torch.empty_like(b)
+    lu = torch.empty_like(a)
+    perm = torch.empty(n, dtype=torch.int64, device="cuda")
+    lf = torch.empty_like(s)
+    t_lu, _ = time_steps(lambda: L.gesv_dev(a, b, x, lu, perm), args.steps, args.warmup, torch)
+    eta_lu = L.backward_error_dev(a, x, b)
+    t_ch, _ = time_steps(lambda: L.posv_dev(s, b, x, lf), args.steps, args.warmup, torch)
+    eta_ch = L.backward_error_dev(s, x, b)
+    f_lu = 2.0 * n ** 3 / 3.0 + 2.0 * n * n * nrhs
+    f_ch = n ** 3 / 3.0 + 2.0 * n * n * nrhs
+    line = {
+        "metric": "plain FP64 solve TFLOP/s (LU: 2n^3/3 + 2n^2 nrhs; Cholesky: n^3/3 + 2n^2 nrhs)",
+        "value": f_lu / t_lu / 1e12, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_lu * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FP64 LU and Cholesky solves, n={n}, {nrhs} rhs (N(0,1); G G^T/n + I)",
+                   "n": n, "nrhs": nrhs, "seed": args.seed},
+        "lu": {"tflops": f_lu / t_lu / 1e12, "ms": t_lu * 1e3, "backward_error": eta_lu,
+               "target": 8 * n * 2.0 ** -53},
+        "cholesky": {"tflops": f_ch / t_ch / 1e12, "ms": t_ch * 1e3, "backward_error": eta_ch,
+                     "target": 8 * n * 2.0 ** -53},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -269,6 +307,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "fp64":
+        bench_fp64(args)
         return
     import torch
     from scirs_amd import linalg as L

@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "tile_map.hpp"
+#include <cstdlib>
 
 namespace {
 
@@ -49,8 +50,10 @@ struct GemmArgs {
     const int *info;
 };
 
-template <typename T, bool TRANS_B>
-__global__ __launch_bounds__(NTHREADS, 2) void gemm_sub_kernel(GemmArgs<T> g)
+// CPRE: prefetch C before the K loop (one workgroup per CU: the prefetch and
+// the accumulators need the 512-register budget of one wave per SIMD).
+template <typename T, bool TRANS_B, bool CPRE>
+__global__ __launch_bounds__(NTHREADS, CPRE ? 1 : 2) void gemm_sub_kernel(GemmArgs<T> g)
 {
     if (g.info && *g.info != 0) return;
     int tm, tn;
@@ -115,13 +118,32 @@ __global__ __launch_bounds__(NTHREADS, 2) void gemm_sub_kernel(GemmArgs<T> g)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = V4{0, 0, 0, 0};
 
+    const int lr = lane & 15;
+    const int lk = lane >> 4;
+    // C is read once: its loads are issued before the K loop so their latency
+    // overlaps the matrix-core work.  Lane l, register q of a 16x16 block holds
+    // column l%16 and row 4*(l/16)+q for v_mfma_f32_16x16x4f32 but row
+    // (l/16)+4*q for v_mfma_f64_16x16x4f64 (measured on gfx950).
+    V4 cpre[4][4];
+    if (CPRE) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = col0 + wn * 64 + j * 16 + lr;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rin = (sizeof(T) == 8) ? (lk + 4 * q) : (lk * 4 + q);
+                    const int r = row0 + wm * 64 + i * 16 + rin;
+                    cpre[i][j][q] = (r < g.M && c < g.N) ? g.C[(long)r * g.ldc + c] : (T)0;
+                }
+            }
+    }
+
     const int nslab = (g.K + BK - 1) / BK;
     load_slab(0);
     store_slab(0);
     __syncthreads();
-
-    const int lr = lane & 15;
-    const int lk = lane >> 4;
 
     for (int s = 0; s < nslab; ++s) {
         const int buf = s & 1;
@@ -144,9 +166,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void gemm_sub_kernel(GemmArgs<T> g)
         __syncthreads();
     }
 
-    // epilogue: C -= acc.  Lane l, register q of a 16x16 block holds column l%16 and
-    // row 4*(l/16)+q for v_mfma_f32_16x16x4f32 but row (l/16)+4*q for
-    // v_mfma_f64_16x16x4f64 (measured on gfx950).
+    // epilogue: C = Cprefetched - acc
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -159,7 +179,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void gemm_sub_kernel(GemmArgs<T> g)
                 const int r = row0 + wm * 64 + i * 16 + rin;
                 if (r < g.M) {
                     T *p = g.C + (long)r * g.ldc + c;
-                    *p = *p - acc[i][j][q];
+                    *p = (CPRE ? cpre[i][j][q] : *p) - acc[i][j][q];
                 }
             }
         }
@@ -179,10 +199,18 @@ int launch_gemm_sub(int M, int N, int K, const T *A, long lda, const T *B, long 
     g.lower = lower;
     g.info = info;
     const int nwg = (g.tiles_m * g.tiles_n + 7) / 8 * 8;
+    // FP64: C prefetched at one workgroup per CU by default, which also leaves
+    // room on each CU for the look-ahead panel running beside the update
+    static const bool cpre = [] {
+        const char *v = getenv("SCIRS_GPU_DGEMM_CPRE");
+        return !(v && *v && atoi(v) == 0);
+    }();
     if (trans_b)
-        hipLaunchKernelGGL((gemm_sub_kernel<T, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+        hipLaunchKernelGGL((gemm_sub_kernel<T, true, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    else if (cpre && sizeof(T) == 8)
+        hipLaunchKernelGGL((gemm_sub_kernel<T, false, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
     else
-        hipLaunchKernelGGL((gemm_sub_kernel<T, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+        hipLaunchKernelGGL((gemm_sub_kernel<T, false, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }
