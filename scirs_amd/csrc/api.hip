@@ -138,7 +138,8 @@ int Context::init(int dev)
     const size_t xw_b = rep * 2 * (size_t)max_g * 4 * 8, xrow_b = rep * 2 * (size_t)max_g * kbmax * 2 * 8,
                  xrj_b = rep * 2 * (size_t)kbmax * 2 * 8, flags_b = 64 * sizeof(int),
                  pairs_b = (size_t)work.max_panels * 4 * kbmax * sizeof(int),
-                 npairs_b = (size_t)work.max_panels * sizeof(int), linv_b = 128 * 128 * sizeof(double);
+                 npairs_b = (size_t)work.max_panels * sizeof(int),
+                 linv_b = 2 * 128 * 128 * sizeof(float);   // two FP32 L11^-1 (double-buffered)
     for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b, linv_b}) bytes += (b + 255) & ~(size_t)255;
     char *base = nullptr;
     SCIRS_HIP_CHECK(hipMalloc((void **)&base, bytes));
@@ -162,6 +163,7 @@ int Context::init(int dev)
     kb_f64 = env_int("SCIRS_GPU_KB", 128);
     kb_f32 = env_int("SCIRS_GPU_KB_F32", 128);
     kb_chol = env_int("SCIRS_GPU_KB_CHOL", 128);
+    narrow_kb = env_int("SCIRS_GPU_NARROW_KB", 1);
     initialised = true;
     return SCIRS_OK;
 }
@@ -319,7 +321,7 @@ static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm
     SCIRS_HIP_CHECK(hipEventCreate(&e2));
     SCIRS_HIP_CHECK(hipEventRecord(e0, s));
     if ((rc = c.reset_flags(s))) return rc;
-    const int kb = sizeof(T) == 8 ? c.kb_f64 : c.kb_f32;
+    const int kb = c.lu_kb<T>(n);
     if ((rc = scirs_getrf_device<T>(n, n, dLU, ldlu, kb, c.work, s))) return rc;
     if ((rc = scirs_perm_from_pairs(n, n, kb, c.work, dPerm, s))) return rc;
     SCIRS_HIP_CHECK(hipEventRecord(e1, s));
@@ -365,7 +367,7 @@ static int host_getrf(int m, int n, T *a, int64_t lda, int64_t *perm, scirs_solv
     int64_t *dP = (int64_t *)c.scratch_perm;
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), m, hipMemcpyHostToDevice, s));
     if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
-    const int kb = sizeof(T) == 8 ? c.kb_f64 : c.kb_f32;
+    const int kb = c.lu_kb<T>(m);
     if ((rc = scirs_getrf_device<T>(m, n, dA, n, kb, c.work, s))) return finish(info, rc, t0);
     if ((rc = scirs_perm_from_pairs(m, n, kb, c.work, dP, s))) return finish(info, rc, t0);
     int st = 0;

@@ -19,6 +19,13 @@ struct Context {
     void *work_base = nullptr;
     void *host_flags = nullptr;
     int kb_f64 = 128, kb_f32 = 128, kb_chol = 128;
+    int narrow_kb = 1;   // narrower LU panels when a whole panel then fits one workgroup
+    // LU panel width for an m-row factorisation in element type T
+    template <typename T> int lu_kb(int m) const
+    {
+        const int kb = sizeof(T) == 8 ? kb_f64 : kb_f32;
+        return narrow_kb ? scirs_lu_panel_kb(m, kb, sizeof(T)) : kb;
+    }
     // scratch grown on demand (host-memory entry points and internal temporaries)
     void *scratch_a = nullptr; size_t scratch_a_bytes = 0;
     void *scratch_b = nullptr; size_t scratch_b_bytes = 0;

@@ -34,20 +34,20 @@ namespace {
 constexpr int NT = 256;  // threads per workgroup everywhere in this file
 
 
-// Apply a gather list to the columns outside [k0, k0+kb): each workgroup owns 64
-// columns, stages the (dst, src) list in LDS, loads every source row of its
-// columns first (16 loads in flight per thread), then stores.
+// Apply a gather list to nv columns: the first nleft are columns 0..nleft-1,
+// the others start at column right0.  Each workgroup owns 64 columns, stages
+// the (dst, src) list in LDS, loads every source row of its columns first (16
+// loads in flight per thread), then stores.
 template <typename T>
-__global__ __launch_bounds__(NT) void laswp_kernel(T *A, long lda, int ncols_total, int k0, int kb,
+__global__ __launch_bounds__(NT) void laswp_kernel(T *A, long lda, int nleft, int right0, int nv,
                                                    const int *pairs, const int *npairs, const int *info)
 {
     if (*info != 0) return;
     const int np = *npairs;
     if (np == 0) return;
-    // column c of the workgroup: skip the panel columns
-    int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    if (c >= k0) c += kb;
-    const bool ok = c < ncols_total;
+    const int v = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c = v < nleft ? v : right0 + (v - nleft);
+    const bool ok = v < nv;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     int *sp = reinterpret_cast<int *>(smem_raw);                   // 2 * np
     T *vals = reinterpret_cast<T *>(smem_raw + (size_t)4 * SCIRS_KB_MAX * sizeof(int));   // np x 64
@@ -178,10 +178,10 @@ __global__ __launch_bounds__(NT) void trtri_unit_lower_kernel(const float *A, lo
     }
 }
 
-// U12 = L11^-1 A12 for one 64-column strip per workgroup.  Wave w owns rows
+// U12 = L11^-1 A12 for columns [c0, ncols_total), one 64-column strip per workgroup.  Wave w owns rows
 // 32w .. 32w+31 of the strip (lane = column); for each k it reads those 32
 // entries of column k of L11^-1 as eight 16-byte broadcast LDS reads.
-__global__ __launch_bounds__(NT) void trsm_apply_kernel(float *A, long lda, int k0, int kb, int ncols_total,
+__global__ __launch_bounds__(NT) void trsm_apply_kernel(float *A, long lda, int k0, int kb, int c0, int ncols_total,
                                                         const float *LinvT, const int *info)
 {
     if (*info != 0) return;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(NT) void trsm_apply_kernel(float *A, long lda, int 
     float *Lt = reinterpret_cast<float *>(smem_raw);   // TI x TP4: Lt[k][r] = L11^-1[r][k]
     float *Bs = Lt + TI * TP4;                         // TI x 65 strip (rows >= kb zero)
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int cbase = k0 + kb + blockIdx.x * 64;
+    const int cbase = c0 + blockIdx.x * 64;
     g2s_block<float, 16>(Lt, TP4, LinvT, TI, TI, TI, TI, TI);
     g2s_block<float, 8>(Bs, 65, A + (long)k0 * lda + cbase, lda, TI, 64, kb, ncols_total - cbase);
     __syncthreads();
@@ -219,11 +219,11 @@ __global__ __launch_bounds__(NT) void trsm_apply_kernel(float *A, long lda, int 
     }
 }
 
-// U12 = L11^-1 A12 for the kb x (N - k0 - kb) block right of the panel.
+// U12 = L11^-1 A12 for columns [c0, ncols_total) of the kb rows of the panel.
 // One workgroup per 64-column strip; L11 (strict lower, packed by rows) and
 // the strip live in LDS; rows are solved in blocks of 8.
 template <typename T>
-__global__ __launch_bounds__(NT) void trsm_llnu_kernel(T *A, long lda, int k0, int kb, int ncols_total,
+__global__ __launch_bounds__(NT) void trsm_llnu_kernel(T *A, long lda, int k0, int kb, int c0, int ncols_total,
                                                        const int *info)
 {
     if (*info != 0) return;
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(NT) void trsm_llnu_kernel(T *A, long lda, int k0, i
     T *Ls = reinterpret_cast<T *>(smem_raw);        // packed strict lower: row i at i*(i-1)/2
     T *Bs = Ls + (size_t)kb * (kb - 1) / 2;         // kb x 64
     const int t = threadIdx.x, lane = t & 63, rg = t >> 6;
-    const int cbase = k0 + kb + blockIdx.x * 64;
+    const int cbase = c0 + blockIdx.x * 64;
     const int c = cbase + lane;
     const bool ok = c < ncols_total;
     const T *L = A + (long)k0 * lda + k0;
@@ -287,7 +287,155 @@ __global__ __launch_bounds__(NT) void trsm_llnu_kernel(T *A, long lda, int k0, i
         for (int i = rg; i < kb; i += NT / 64) A[(long)(k0 + i) * lda + c] = Bs[i * 64 + lane];
 }
 
+// Row interchanges and U12 = L11^-1 A12 in one pass, for the columns the next
+// panel needs: the look-ahead's critical path between two panels.  One
+// workgroup of 1024 threads per 64-column strip (lane = column):
+//   1. the kb rows of the strip (before the interchanges), the strict lower
+//      L11 (packed by rows) and the interchange list go to LDS;
+//   2. every moved row's new value is read into registers: a row that ends in
+//      the top kb rows comes from the strip or from below the panel; a row that
+//      ends below comes from the strip (each interchange of partial pivoting
+//      involves a top row, so a row below only ever receives a top row);
+//   3. after every read has landed, the values are written: top rows into the
+//      strip, rows below straight to A (so the update that follows sees them);
+//   4. forward substitution with the unit-lower L11, rows in blocks of 8, the
+//      rows below a block updated by all 16 waves; solved rows go to a second
+//      LDS strip;
+//   5. the solved strip back to rows k0 .. k0+kb-1.
+constexpr int NT_LT = 1024;
+template <typename T>
+__global__ __launch_bounds__(NT_LT) void laswp_trsm_kernel(T *__restrict__ A, long lda, int k0, int kb, int c0, int c1,
+                                                           const T *__restrict__ L, const int *pairs,
+                                                           const int *npairs, const int *info)
+{
+    if (*info != 0) return;
+    constexpr int NW = NT_LT / 64, PMAX = (2 * SCIRS_KB_MAX + NW - 1) / NW;   // interchanges per thread
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int *sp = reinterpret_cast<int *>(smem_raw);                                   // 2 * np
+    T *Bs = reinterpret_cast<T *>(smem_raw + (size_t)4 * SCIRS_KB_MAX * sizeof(int));   // kb x 64
+    T *Xs = Bs + (size_t)kb * 64;   // kb x 64 solved rows (separate, so no wave reads a row being replaced)
+    // FP32: L11 in LDS too (pitch kb+4), read by broadcast; FP64 (no room): from memory, scalar loads
+    constexpr bool LDS_L = sizeof(T) == 4;
+    const int lp = kb + 4;
+    T *Ls = Xs + (size_t)kb * 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int c = c0 + blockIdx.x * 64 + lane;
+    const bool ok = c < c1;
+    const int np = *npairs;
+    const int top_end = k0 + kb;
+    for (int q = t; q < 2 * np; q += NT_LT) sp[q] = pairs[q];
+    if (LDS_L) g2s_block<T, 8>(Ls, lp, L, lda, kb, kb, kb, kb);
+    {
+        constexpr int U = 8;   // loads of a thread in flight before its LDS stores
+        for (int r0 = wave; r0 < kb; r0 += U * NW) {
+            T bv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int r = r0 + u * NW;
+                bv[u] = (ok && r < kb) ? A[(long)(k0 + r) * lda + c] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (r0 + u * NW < kb) Bs[(r0 + u * NW) * 64 + lane] = bv[u];
+        }
+    }
+    __syncthreads();
+    T v[PMAX];
+#pragma unroll
+    for (int u = 0; u < PMAX; ++u) {
+        const int q = wave + u * NW;
+        v[u] = (T)0;
+        if (q < np) {
+            const int src = sp[2 * q + 1];
+            if (src < top_end) v[u] = Bs[(src - k0) * 64 + lane];
+            else if (ok) v[u] = A[(long)src * lda + c];
+        }
+    }
+    // every read of A above has returned before any thread writes a row below
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PMAX; ++u) {
+        const int q = wave + u * NW;
+        if (q < np) {
+            const int dst = sp[2 * q];
+            if (dst < top_end) Bs[(dst - k0) * 64 + lane] = v[u];
+            else if (ok) A[(long)dst * lda + c] = v[u];
+        }
+    }
+    __syncthreads();
+    // L11 is read straight from memory at wave-uniform addresses (scalar loads,
+    // cached): every lane of a wave needs the same L entries
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    for (int ib = 0; ib < kb; ib += 8) {
+        const int ie = min(ib + 8, kb);
+        // every wave solves the 8 x 8 diagonal block itself, in registers
+        // (x_q -= sum_{p<q} L[ib+q][ib+p] x_p), so no barrier separates the
+        // diagonal solve from the update of the rows below it
+        T x[8];
+        if (ie - ib == 8) {
+            // full block, branch-free so every wave issues all its loads before waiting
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = Bs[(ib + q) * 64 + lane];
+            T ld[8][8];
+#pragma unroll
+            for (int q = 1; q < 8; ++q)
+#pragma unroll
+                for (int pp = 0; pp < 8; ++pp)
+                    ld[q][pp] = LDS_L ? Ls[(ib + q) * lp + ib + pp] : L[(long)(ib + q) * lda + ib + pp];
+#pragma unroll
+            for (int q = 1; q < 8; ++q)
+#pragma unroll
+                for (int pp = 0; pp < q; ++pp) x[q] -= ld[q][pp] * x[pp];
+            if (wv == 0) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) Xs[(ib + q) * 64 + lane] = x[q];
+            }
+            // rows below the block, four per wave at a time (row index clamped for the
+            // loads, so they need no branch; rows past kb are not stored)
+            for (int r0 = ie + wv; r0 < kb; r0 += 4 * NW) {
+                T acc[4], lr[4][8];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int rr = min(r0 + u * NW, kb - 1);
+                    acc[u] = Bs[rr * 64 + lane];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) lr[u][q] = LDS_L ? Ls[rr * lp + ib + q] : L[(long)rr * lda + ib + q];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc[u] -= lr[u][q] * x[q];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (r0 + u * NW < kb) Bs[(r0 + u * NW) * 64 + lane] = acc[u];
+            }
+        } else {
+            // the last, partial block of a ragged panel
+            for (int q = 0; q < ie - ib; ++q) {
+                T xq = Bs[(ib + q) * 64 + lane];
+                for (int pp = 0; pp < q; ++pp) xq -= L[(long)(ib + q) * lda + ib + pp] * x[pp];
+                x[q] = xq;
+                if (wv == 0) Xs[(ib + q) * 64 + lane] = xq;
+            }
+        }
+        __syncthreads();
+    }
+    if (ok)
+        for (int i = wave; i < kb; i += NW) A[(long)(k0 + i) * lda + c] = Xs[i * 64 + lane];
+}
+
 } // namespace
+
+int scirs_lu_panel_kb(int m, int kb, size_t elem_bytes)
+{
+    for (int w = kb; w >= 32; w = w > 64 ? 64 : w / 2) {
+        const int rmax = elem_bytes == 8 ? panel::max_rows<double>(w) : panel::max_rows<float>(w);
+        if (m <= rmax) return w;
+        if (w == 32) break;
+    }
+    return kb;
+}
 
 template <typename T>
 int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream)
@@ -330,6 +478,8 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
             R = (mrows + G - 1) / G;
         }
         G = (mrows + R - 1) / R;
+        // the whole panel fits one workgroup: factor it there, with no exchange
+        if (mrows <= rmax) { G = 1; R = mrows; }
         panel::PanelArgs<T> pa;
         pa.A = A; pa.lda = lda; pa.k0 = k0; pa.m = mrows; pa.kb = kbw; pa.R = R; pa.G = G;
         pa.xw = w.xw; pa.xrow = w.xrow; pa.xrj = w.xrj; pa.nrep = w.panel_nrep;
@@ -339,7 +489,8 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         pa.info = w.info; pa.info_val = w.info_val; pa.abort = w.abort; pa.spin_limit = w.spin_limit;
         pa.poll_sleep = w.poll_sleep;
         const size_t lds = panel::lds_bytes(R, kbw, sizeof(T));
-        const void *pk = w.panel_w0 ? (const void *)panel::getf2_panel_kernel<T, true>
+        const bool w0 = w.panel_w0 && G > 1;   // the single-workgroup path is the all-waves variant
+        const void *pk = w0 ? (const void *)panel::getf2_panel_kernel<T, true>
                                     : (const void *)panel::getf2_panel_kernel<T, false>;
         if (int rc = scirs_allow_lds(pk, lds)) return rc;
         {
@@ -348,7 +499,7 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
                 // a normal launch, so the panel can share the device with the side
                 // stream's update: its G <= #CUs workgroups all become resident once
                 // that update's workgroups retire (they wait for nothing)
-                if (w.panel_w0)
+                if (w0)
                     hipLaunchKernelGGL((panel::getf2_panel_kernel<T, true>), dim3(G), dim3(panel::NT), lds, stream, pa);
                 else
                     hipLaunchKernelGGL((panel::getf2_panel_kernel<T, false>), dim3(G), dim3(panel::NT), lds, stream, pa);
@@ -358,78 +509,97 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
                 SCIRS_HIP_CHECK(hipLaunchCooperativeKernel(pk, dim3(G), dim3(panel::NT), args, lds, stream));
             }
         }
-        // the rest of the previous panel's update must land before the row interchanges
-        if (lookahead && p > 0) SCIRS_HIP_CHECK(hipStreamWaitEvent(stream, w.ev_upd, 0));
-        // row interchanges outside the panel
-        const int nout_cols = n - kbw;
-        if (nout_cols > 0) {
+        const int nright = n - k0 - kbw;
+        const int mbelow = m - k0 - kbw;
+        // FP32: L11^-1 once (double-buffered, so the side stream's use of the
+        // previous panel's inverse never races with this one), then multiplies
+        const bool inv32 = std::is_same<T, float>::value && kbw <= TI && w.linv && nright > 0;
+        float *linv = w.linv ? (float *)w.linv + (size_t)(p & 1) * TI * TI : nullptr;
+        auto trtri = [&](hipStream_t s) -> int {
+            if (!inv32) return SCIRS_OK;
+            scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * kbw / 3.0, 0.5 * kbw * kbw * sizeof(T), s);
+            const size_t lds_i = ((size_t)TI * TIP + 64 * 64) * sizeof(float);
+            if (int rc = scirs_allow_lds((const void *)trtri_unit_lower_kernel, lds_i)) return rc;
+            hipLaunchKernelGGL(trtri_unit_lower_kernel, dim3(1), dim3(NT), lds_i, s,
+                               (const float *)(A + (long)k0 * lda + k0), lda, kbw, linv, (const int *)w.info);
+            SCIRS_HIP_CHECK(hipGetLastError());
+            return SCIRS_OK;
+        };
+        // row interchanges on nv columns (the first nleft from column 0, the rest from right0)
+        auto laswp = [&](int nleft, int right0, int nv, hipStream_t s) -> int {
+            if (nv <= 0) return SCIRS_OK;
             const size_t lds_sw = (size_t)4 * SCIRS_KB_MAX * sizeof(int) + (size_t)2 * kbw * 64 * sizeof(T);
             if (int rc = scirs_allow_lds((const void *)laswp_kernel<T>, lds_sw)) return rc;
-            scirs::ProfScope ps(SCIRS_PROF_LASWP, 0.0, 4.0 * kbw * nout_cols * sizeof(T), stream);
-            hipLaunchKernelGGL(laswp_kernel<T>, dim3((nout_cols + 63) / 64), dim3(NT), lds_sw, stream,
-                               A, lda, n, k0, kbw, pa.pairs, pa.npairs, (const int *)w.info);
+            scirs::ProfScope ps(SCIRS_PROF_LASWP, 0.0, 4.0 * kbw * nv * sizeof(T), s);
+            hipLaunchKernelGGL(laswp_kernel<T>, dim3((nv + 63) / 64), dim3(NT), lds_sw, s,
+                               A, lda, nleft, right0, nv, pa.pairs, pa.npairs, (const int *)w.info);
             SCIRS_HIP_CHECK(hipGetLastError());
-        }
-        const int nright = n - k0 - kbw;
-        if (nright > 0) {
-            if (std::is_same<T, float>::value && kbw <= TI && w.linv) {
-                // FP32: L11^-1 once, then a parallel multiply
-                scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * nright,
-                                    (0.5 * kbw * kbw + 2.0 * kbw * nright) * sizeof(T), stream);
-                const size_t lds_i = ((size_t)TI * TIP + 64 * 64) * sizeof(float);
-                if (int rc = scirs_allow_lds((const void *)trtri_unit_lower_kernel, lds_i)) return rc;
-                hipLaunchKernelGGL(trtri_unit_lower_kernel, dim3(1), dim3(NT), lds_i, stream,
-                                   (const float *)(A + (long)k0 * lda + k0), lda, kbw, (float *)w.linv,
-                                   (const int *)w.info);
+            return SCIRS_OK;
+        };
+        // U12 = L11^-1 A12 on columns [c0, c1)
+        auto trsm = [&](int c0, int c1, hipStream_t s) -> int {
+            const int nc = c1 - c0;
+            if (nc <= 0) return SCIRS_OK;
+            scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * nc, 2.0 * kbw * nc * sizeof(T), s);
+            if (inv32) {
                 const size_t lds_a = ((size_t)TI * TP4 + (size_t)TI * 65) * sizeof(float);
                 if (int rc = scirs_allow_lds((const void *)trsm_apply_kernel, lds_a)) return rc;
-                hipLaunchKernelGGL(trsm_apply_kernel, dim3((nright + 63) / 64), dim3(NT), lds_a, stream,
-                                   (float *)A, lda, k0, kbw, n, (const float *)w.linv, (const int *)w.info);
-                SCIRS_HIP_CHECK(hipGetLastError());
+                hipLaunchKernelGGL(trsm_apply_kernel, dim3((nc + 63) / 64), dim3(NT), lds_a, s,
+                                   (float *)A, lda, k0, kbw, c0, c1, (const float *)linv, (const int *)w.info);
             } else {
                 const size_t lds_t = ((size_t)kbw * (kbw - 1) / 2 + (size_t)kbw * 64) * sizeof(T);
                 if (int rc = scirs_allow_lds((const void *)trsm_llnu_kernel<T>, lds_t)) return rc;
-                scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * nright,
-                                    (0.5 * kbw * kbw + 2.0 * kbw * nright) * sizeof(T), stream);
-                hipLaunchKernelGGL(trsm_llnu_kernel<T>, dim3((nright + 63) / 64), dim3(NT), lds_t, stream,
-                                   A, lda, k0, kbw, n, (const int *)w.info);
-                SCIRS_HIP_CHECK(hipGetLastError());
+                hipLaunchKernelGGL(trsm_llnu_kernel<T>, dim3((nc + 63) / 64), dim3(NT), lds_t, s,
+                                   A, lda, k0, kbw, c0, c1, (const int *)w.info);
             }
-            const int mbelow = m - k0 - kbw;
-            if (mbelow > 0) {
-                // look-ahead: the next panel's columns now (main stream), the rest on
-                // the side stream, overlapping the next panel's factorisation
-                const int nnext = lookahead ? std::min(nright, kb) : nright;
-                const int nrest = nright - nnext;
-                if (nrest > 0) {
-                    SCIRS_HIP_CHECK(hipEventRecord(w.ev_trsm, stream));
-                    SCIRS_HIP_CHECK(hipStreamWaitEvent(w.side_stream, w.ev_trsm, 0));
-                    scirs::ProfScope ps(SCIRS_PROF_UPDATE, 2.0 * mbelow * nrest * kbw,
-                                        (2.0 * mbelow * nrest + (double)mbelow * kbw + (double)kbw * nrest) *
-                                            sizeof(T), w.side_stream);
-                    int rc = scirs_trailing_update<T>(mbelow, nrest, kbw,
-                                                      A + (long)(k0 + kbw) * lda + k0, lda,
-                                                      A + (long)k0 * lda + k0 + kbw + nnext, lda,
-                                                      A + (long)(k0 + kbw) * lda + k0 + kbw + nnext, lda,
-                                                      w, w.side_stream);
-                    if (rc) return rc;
-                    SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, w.side_stream));
-                }
-                scirs::ProfScope ps(SCIRS_PROF_UPDATE, 2.0 * mbelow * nnext * kbw,
-                                    (2.0 * mbelow * nnext + (double)mbelow * kbw + (double)kbw * nnext) * sizeof(T),
-                                    stream);
-                int rc = scirs_trailing_update<T>(mbelow, nnext, kbw,
-                                                  A + (long)(k0 + kbw) * lda + k0, lda,
-                                                  A + (long)k0 * lda + k0 + kbw, lda,
-                                                  A + (long)(k0 + kbw) * lda + k0 + kbw, lda,
-                                                  w, stream);
-                if (rc) return rc;
-                if (nrest == 0 && lookahead) {
-                    // nothing pending on the side stream for the next panel to wait for
-                    SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, stream));
-                }
-            }
+            SCIRS_HIP_CHECK(hipGetLastError());
+            return SCIRS_OK;
+        };
+        // A22[:, c0:c1) -= L21 U12[:, c0:c1)
+        auto update = [&](int c0, int c1, hipStream_t s) -> int {
+            const int nc = c1 - c0;
+            if (nc <= 0 || mbelow <= 0) return SCIRS_OK;
+            scirs::ProfScope ps(SCIRS_PROF_UPDATE, 2.0 * mbelow * nc * kbw,
+                                (2.0 * mbelow * nc + (double)mbelow * kbw + (double)kbw * nc) * sizeof(T), s);
+            return scirs_trailing_update<T>(mbelow, nc, kbw, A + (long)(k0 + kbw) * lda + k0, lda,
+                                            A + (long)k0 * lda + c0, lda, A + (long)(k0 + kbw) * lda + c0, lda, w,
+                                            s);
+        };
+        const int cn = k0 + kbw;   // first column right of the panel
+        if (!lookahead) {
+            if (int rc = trtri(stream)) return rc;
+            if (int rc = laswp(k0, cn, n - kbw, stream)) return rc;
+            if (int rc = trsm(cn, n, stream)) return rc;
+            if (int rc = update(cn, n, stream)) return rc;
+            continue;
         }
+        // Look-ahead.  Main stream: only what the next panel needs (its columns
+        // interchanged, solved and updated).  Side stream: the interchanges of
+        // every other column and the rest of the update, overlapping the next
+        // panel.  The previous panel's side-stream work updates the next panel's
+        // columns, so it must land first.
+        const int cr = std::min(n, cn + kb);   // end of the next panel's columns
+        if (p > 0) SCIRS_HIP_CHECK(hipStreamWaitEvent(stream, w.ev_upd, 0));
+        SCIRS_HIP_CHECK(hipEventRecord(w.ev_trsm, stream));
+        SCIRS_HIP_CHECK(hipStreamWaitEvent(w.side_stream, w.ev_trsm, 0));
+        if (int rc = laswp(k0, cr, k0 + (n - cr), w.side_stream)) return rc;
+        if (cr < n)
+            if (int rc = trtri(w.side_stream)) return rc;
+        if (int rc = trsm(cr, n, w.side_stream)) return rc;
+        if (int rc = update(cr, n, w.side_stream)) return rc;
+        SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, w.side_stream));
+        if (cr > cn) {
+            const size_t lds_lt = (size_t)4 * SCIRS_KB_MAX * sizeof(int) + (size_t)2 * kbw * 64 * sizeof(T) +
+                                  (sizeof(T) == 4 ? (size_t)kbw * (kbw + 4) * sizeof(T) : 0);
+            if (int rc = scirs_allow_lds((const void *)laswp_trsm_kernel<T>, lds_lt)) return rc;
+            scirs::ProfScope ps(SCIRS_PROF_TRSM, (double)kbw * kbw * (cr - cn),
+                                (2.0 * kbw + 4.0 * kbw) * (cr - cn) * sizeof(T), stream);
+            hipLaunchKernelGGL(laswp_trsm_kernel<T>, dim3((cr - cn + 63) / 64), dim3(NT_LT), lds_lt, stream, A, lda,
+                               k0, kbw, cn, cr, (const T *)(A + (long)k0 * lda + k0), pa.pairs, pa.npairs,
+                               (const int *)w.info);
+            SCIRS_HIP_CHECK(hipGetLastError());
+        }
+        if (int rc = update(cn, cr, stream)) return rc;
     }
     if (lookahead) {
         // the caller's stream sees every update

@@ -362,7 +362,7 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     double *Z = (double *)take((size_t)(restart + 1) * n * sizeof(double));
     double *w = (double *)take((size_t)n * sizeof(double));
     double *hdev = (double *)take((size_t)(restart + 8) * sizeof(double));
-    st.kb = c.kb_f32;
+    st.kb = c.lu_kb<float>(n);
 
     hipEvent_t e0, e1, e2;
     SCIRS_HIP_CHECK(hipEventCreate(&e0));
@@ -446,8 +446,9 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
         dLU64 = (double *)c.scratch_a;
         if ((rc = scirs_d2d_copy(n, n, dA, lda, dLU64, n, s))) return rc;
         if ((rc = c.reset_flags(s))) return rc;
-        if ((rc = scirs_getrf_device<double>(n, n, dLU64, n, c.kb_f64, c.work, s))) return rc;
-        if ((rc = scirs_perm_from_pairs(n, n, c.kb_f64, c.work, st.perm, s))) return rc;
+        const int kb64 = c.lu_kb<double>(n);
+        if ((rc = scirs_getrf_device<double>(n, n, dLU64, n, kb64, c.work, s))) return rc;
+        if ((rc = scirs_perm_from_pairs(n, n, kb64, c.work, st.perm, s))) return rc;
         if ((rc = scirs_permute_rows<double>(n, nrhs, st.perm, dB, ldb, dX, ldx, s))) return rc;
         if ((rc = scirs_getrs_device<double>(n, nrhs, dLU64, n, dX, ldx, c.work.info, s))) return rc;
         int st64 = 0;

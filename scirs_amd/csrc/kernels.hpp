@@ -95,6 +95,11 @@ int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int 
 template <typename T>
 int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);
 
+// Panel width for an m-row factorisation: the widest of kb, 64, 32 (never above
+// kb) at which all m rows of a panel fit one workgroup's LDS, so every panel is
+// factored by one workgroup without inter-workgroup exchange; kb when none fits.
+int scirs_lu_panel_kb(int m, int kb, size_t elem_bytes);
+
 // Row permutation produced by the last getrf: perm[i] = original row now at i
 // (the `piv` vector of scirs2-linalg's LUDecomposition).
 int scirs_perm_from_pairs(int m, int n, int kb, const ScirsWork &w, int64_t *perm, hipStream_t stream);

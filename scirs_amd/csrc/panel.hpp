@@ -54,18 +54,23 @@ template <> struct Tag<float> {
         v = __uint_as_float((unsigned)w);
         return (unsigned)(w >> 32) == seq;
     }
+    __device__ static void put_s(u64 *dst, size_t, unsigned seq, float v) { put(dst, seq, v); }
+    __device__ static bool get_s(const u64 *src, size_t, unsigned seq, float &v) { return get(src, seq, v); }
 };
 template <> struct Tag<double> {
     static constexpr int W = 2;
-    __device__ static void put(u64 *dst, unsigned seq, double v)
+    __device__ static void put(u64 *dst, unsigned seq, double v) { put_s(dst, 1, seq, v); }
+    __device__ static bool get(const u64 *src, unsigned seq, double &v) { return get_s(src, 1, seq, v); }
+    // the two words `stride` apart
+    __device__ static void put_s(u64 *dst, size_t stride, unsigned seq, double v)
     {
         const u64 bits = (u64)__double_as_longlong(v);
         st_agent(dst, ((u64)seq << 32) | (bits & 0xffffffffull));
-        st_agent(dst + 1, ((u64)seq << 32) | (bits >> 32));
+        st_agent(dst + stride, ((u64)seq << 32) | (bits >> 32));
     }
-    __device__ static bool get(const u64 *src, unsigned seq, double &v)
+    __device__ static bool get_s(const u64 *src, size_t stride, unsigned seq, double &v)
     {
-        const u64 lo = ld_agent(src), hi = ld_agent(src + 1);
+        const u64 lo = ld_agent(src), hi = ld_agent(src + stride);
         v = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
         return (unsigned)(lo >> 32) == seq && (unsigned)(hi >> 32) == seq;
     }
@@ -75,7 +80,9 @@ template <typename T>
 struct PanelArgs {
     T *A; long lda;
     int k0, m, kb, R, G;
-    u64 *xw;       // [nrep][2][G][4]   candidate: (seq|row), value words
+    u64 *xw;       // [nrep][2][4][G]   candidate: word 0 (seq|row), then the value words;
+                   //                   word k of workgroup q at [k][q], so one wave's poll of
+                   //                   all candidates reads consecutive words
     u64 *xrow;     // [nrep][2][G][kb*W] candidate rows
     u64 *xrj;      // [nrep][2][kb*W]   row j of the panel
     int nrep;      // replicas of every published word; workgroup g reads replica g % nrep
@@ -85,6 +92,64 @@ struct PanelArgs {
     unsigned spin_limit;
     int poll_sleep;   // s_sleep(2) repetitions after an unsuccessful candidate poll
 };
+
+// Rank-1 update row -= lcol[row] * prow over rows [r1, R) except skip0/skip1
+// and columns [qlo, kb), by nthr threads (tt = 0..nthr-1, nthr a multiple of
+// 32).  32 consecutive threads cover 32 consecutive columns of one row
+// (conflict-free LDS rows), each thread KMAX strided columns whose prow values
+// stay in registers; RU rows are loaded before any is written back, so every
+// thread keeps RU * KMAX independent LDS reads in flight (one wave per SIMD
+// cannot hide LDS latency any other way).  The code is branch-free: an element
+// outside the update (a skipped row, a row past R, a column past kb) is read
+// from and written back to its row's padding column kb, which holds no data.
+template <typename T, int KMAX>
+__device__ __forceinline__ void bulk_rank1(T *tile, int kbp, int kb, const T *lcol, const T *prow, int r1, int R,
+                                           int qlo, int skip0, int skip1, int tt, int nthr)
+{
+    constexpr int TPR = 32, RU = 4;
+    if (qlo >= kb || r1 >= R) return;
+    const int qt = tt % TPR, rstep = nthr / TPR;
+    T pr[KMAX];
+    int qc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int q = qlo + qt + TPR * k;
+        qc[k] = q < kb ? q : kb;
+        pr[k] = prow[min(q, kb - 1)];
+    }
+    for (int r = r1 + tt / TPR; r < R; r += RU * rstep) {
+        T v[RU][KMAX], l[RU];
+        int ad[RU][KMAX];
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int rr = r + u * rstep;
+            const bool ok = rr < R && rr != skip0 && rr != skip1;
+            const int rc = min(rr, R - 1);
+            l[u] = lcol[rc];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                ad[u][k] = rc * kbp + (ok ? qc[k] : kb);
+                v[u][k] = tile[ad[u][k]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RU; ++u)
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) tile[ad[u][k]] = v[u][k] - l[u] * pr[k];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void bulk_update(T *tile, int kbp, int kb, const T *lcol, const T *prow, int r1, int R,
+                                            int qlo, int skip0, int skip1, int tt, int nthr)
+{
+    if (kb - qlo <= 64)
+        bulk_rank1<T, 2>(tile, kbp, kb, lcol, prow, r1, R, qlo, skip0, skip1, tt, nthr);
+    else if (kb - qlo <= 128)
+        bulk_rank1<T, 4>(tile, kbp, kb, lcol, prow, r1, R, qlo, skip0, skip1, tt, nthr);
+    else
+        bulk_rank1<T, SCIRS_PANEL_KB_MAX / 32>(tile, kbp, kb, lcol, prow, r1, R, qlo, skip0, skip1, tt, nthr);
+}
 
 // W0 selects where the column's critical path runs: false = all four waves
 // (five barriers per column), true = wave 0 alone (rows strided over its 64
@@ -119,7 +184,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
     // of G workgroups spread over nrep times more cache lines.
     const int nrep = a.nrep, myrep = g % nrep;
     const size_t rs_w = (size_t)2 * G * 4, rs_r = (size_t)2 * G * kb * W, rs_j = (size_t)2 * kb * W;
-    auto XW = [&](int par, int q, int r) { return a.xw + r * rs_w + ((size_t)par * G + q) * 4; };
+    auto XW = [&](int par, int q, int r) { return a.xw + r * rs_w + (size_t)par * 4 * G + q; };   // word k at + k*G
     auto XR = [&](int par, int q, int r) { return a.xrow + r * rs_r + ((size_t)par * G + q) * (size_t)kb * W; };
     auto XJ = [&](int par, int r) { return a.xrj + r * rs_j + (size_t)par * kb * W; };
 
@@ -167,26 +232,39 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
             if (t == 0) {
                 u64 *xw = XW(b, g, r);
                 st_agent(xw, ((u64)seq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
-                Tag<T>::put(xw + 1, seq, s_cv);
+                Tag<T>::put_s(xw + G, G, seq, s_cv);
             }
         }
     };
 
     local_candidate(0, 0);
-    publish(0);
+    // One workgroup holds the whole panel: its local arg-max is the global one,
+    // and the candidate never leaves the workgroup (no publish, no poll).
+    const bool solo = !W0 && G == 1;
+    int nc = s_c;       // solo: candidate row of the next column (local = global)
+    T ncv = s_cv;
+    if (!solo) publish(0);
 
     const T eps = num_limits<T>::eps();
     for (int j = 0; j < kb; ++j) {
         const int b = j & 1;
         const unsigned seq = a.seq_base + (unsigned)j + 1u;
-        T *prow = prow2 + b * kb;
+        // solo: the pivot row is swapped into tile row j, which no later column changes
+        T *prow = solo ? tile + (size_t)j * kbp : prow2 + b * kb;
         // ---- 1. wave 0 reduces the candidate list (the same in every workgroup) and,
         //         in the same polls, fetches the current best candidate's row and row j,
         //         so the pivot row usually arrives with the last candidate.  Waves 1-3
         //         are meanwhile finishing the previous column's bulk update.  (More
         //         polling waves were measured slower: they add contention on the
         //         candidate lines.)
-        if (wave == 0) {
+        if (solo) {
+            // every thread holds the candidate (nc, ncv): nothing to exchange
+            if (t == 0 && !(nc >= 0 && ncv >= eps)) {   // |pivot| < eps: singular (lapack.rs:106-120)
+                s_stop = 1;
+                *a.info = a.k0 + j + 1;
+                *a.info_val = (double)(nc >= 0 ? ncv : (T)0);
+            }
+        } else if (wave == 0) {
             bool mine = false;
             unsigned spins = 0;
             T vv = (T)0; int ii = -1, wgw = -1;
@@ -213,7 +291,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                     const u64 *xw = XW(b, q, myrep);
                     const u64 w0 = ld_agent(xw);
                     T qv;
-                    const bool okv = Tag<T>::get(xw + 1, seq, qv);
+                    const bool okv = Tag<T>::get_s(xw + G, G, seq, qv);
                     ok = ok && okv && (unsigned)(w0 >> 32) == seq;
                     const int qi = (int)(unsigned)w0;
                     if (qi >= 0 && (i < 0 || qv > v || (qv == v && qi < i))) { v = qv; i = qi; wg = q; }
@@ -345,28 +423,27 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 if (lane == 0) {
                     u64 *xw = XW(bb, g, r);
                     st_agent(xw, ((u64)sq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
-                    Tag<T>::put(xw + 1, sq, s_cv);
+                    Tag<T>::put_s(xw + G, G, sq, s_cv);
                 }
             }
         } else if (j + 2 < kb) {
             // ---- 4. bulk rank-1 update of the other rows (waves 1-3), overlapping
             //         wave 0's publication and its next poll
-            constexpr int TPR = 8;
-            const int tt = t - 64;
-            for (int r = r1 + tt / TPR; r < Rg; r += (NT - 64) / TPR) {
-                if (r == c || r == jn) continue;
-                const T l = lcol[r];
-                T *row = tile + r * kbp;
-                for (int q = j + 2 + (tt % TPR); q < kb; q += TPR) row[q] -= l * prow[q];
-            }
+            bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t - 64, NT - 64);
         }
         } else {
-        const int p = s_p;
+        const int p = solo ? nc : s_p;
         const bool own_j = (j >= row0 && j < row0 + Rg);
         const bool own_p = (p >= row0 && p < row0 + Rg);
         // ---- 3. row interchange inside the panel (the permutation record is kept
         //         later, off the critical path)
-        if (p != j) {
+        if (solo && p != j) {
+            for (int c = t; c < kb; c += NT) {
+                const T v = tile[j * kbp + c];
+                tile[j * kbp + c] = tile[p * kbp + c];
+                tile[p * kbp + c] = v;
+            }
+        } else if (p != j) {
             if (own_j)
                 for (int c = t; c < kb; c += NT) tile[(j - row0) * kbp + c] = prow[c];
             if (own_p)
@@ -426,7 +503,14 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         //         date and published by the same thread (readers check every
         //         word's tag, so no barrier is needed before publishing)
         const int jn = (j + 1 >= row0 && j + 1 < row0 + Rg) ? j + 1 - row0 : -1;
-        {
+        if (solo) {
+            for (int q = j + 2 + t; q < kb; q += NT) {
+                if (c >= 0) tile[c * kbp + q] -= lcol[c] * prow[q];
+                if (jn >= 0 && jn != c) tile[jn * kbp + q] -= lcol[jn] * prow[q];
+            }
+            nc = c;
+            ncv = cv;
+        } else {
             const int bb = (j + 1) & 1;
             const unsigned sq = a.seq_base + (unsigned)j + 2u;
             for (int q = t; q < kb; q += NT) {
@@ -445,7 +529,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 for (int r = 0; r < nrep; ++r) {
                     u64 *xw = XW(bb, g, r);
                     st_agent(xw, ((u64)sq << 32) | (unsigned)(c >= 0 ? row0 + c : -1));
-                    Tag<T>::put(xw + 1, sq, cv);
+                    Tag<T>::put_s(xw + G, G, sq, cv);
                 }
             }
         }
@@ -454,15 +538,11 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         //         exchange; wave 0 goes straight to the next column's poll.  The next
         //         column writes the other parity of prow2, and its first barrier
         //         orders this update before any use of the tile.
-        if (wave > 0 && j + 2 < kb) {
-            constexpr int TPR = 8;   // threads per row, consecutive threads on consecutive columns
-            const int tt = t - 64;
-            for (int r = r1 + tt / TPR; r < Rg; r += (NT - 64) / TPR) {
-                if (r == c || r == jn) continue;
-                const T l = lcol[r];
-                T *row = tile + r * kbp;
-                for (int q = j + 2 + (tt % TPR); q < kb; q += TPR) row[q] -= l * prow[q];
-            }
+        //         (Single workgroup: no exchange to overlap, so wave 0 joins in.)
+        if (solo) {
+            if (j + 2 < kb) bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t, NT);
+        } else if (wave > 0 && j + 2 < kb) {
+            bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t - 64, NT - 64);
         }
     }
         }
