@@ -188,7 +188,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
     auto XR = [&](int par, int q, int r) { return a.xrow + r * rs_r + ((size_t)par * G + q) * (size_t)kb * W; };
     auto XJ = [&](int par, int r) { return a.xrj + r * rs_j + (size_t)par * kb * W; };
 
-    g2s_block<T>(tile, kbp, Ap + (long)row0 * a.lda, a.lda, Rg, kb, Rg, kb);
+    g2s_block<T, 16>(tile, kbp, Ap + (long)row0 * a.lda, a.lda, Rg, kb, Rg, kb);   // 16 loads in flight per thread
     if (g == 0) {
         for (int q = t; q < kb; q += NT) top_label[q] = q;
         if (t == 0) s_nout = 0;
@@ -553,15 +553,26 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
 
     if (g == 0) {
         __syncthreads();
-        if (t == 0) {
+        if (wave == 0) {
+            // the (destination, source) list of every moved row, compacted by wave 0
+            // with ballots: the top rows first, then the rows outside them
+            const int nout = s_nout, total = kb + nout;
             int np = 0;
-            for (int q = 0; q < kb; ++q)
-                if (top_label[q] != q) { a.pairs[2 * np] = a.k0 + q; a.pairs[2 * np + 1] = a.k0 + top_label[q]; ++np; }
-            for (int s = 0; s < s_nout; ++s)
-                if (out_label[s] != out_pos[s]) {
-                    a.pairs[2 * np] = a.k0 + out_pos[s]; a.pairs[2 * np + 1] = a.k0 + out_label[s]; ++np;
+            for (int base = 0; base < total; base += 64) {
+                const int e = base + lane;
+                int dst = 0, src = 0;
+                bool moved = false;
+                if (e < kb) {
+                    dst = e; src = top_label[e]; moved = src != e;
+                } else if (e < total) {
+                    dst = out_pos[e - kb]; src = out_label[e - kb]; moved = src != dst;
                 }
-            *a.npairs = np;
+                const unsigned long long bl = __ballot(moved);
+                const int at = np + __popcll(bl & ((1ull << lane) - 1ull));
+                if (moved) { a.pairs[2 * at] = a.k0 + dst; a.pairs[2 * at + 1] = a.k0 + src; }
+                np += __popcll(bl);
+            }
+            if (lane == 0) *a.npairs = np;
         }
     }
 }
