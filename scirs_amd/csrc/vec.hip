@@ -53,6 +53,52 @@ __global__ __launch_bounds__(NT) void residual_kernel(int n, int nrhs, const dou
     }
 }
 
+// One right-hand side with 16-byte aligned rows (even lda, aligned A and x):
+// r = b - A x, one wave per 4 rows, each lane loading 2 consecutive doubles of
+// 4 rows for 4 column chunks at once (32 loads in flight), x loaded once per
+// chunk for the 4 rows.  The odd last column (n odd) is added by lane 0.
+__global__ __launch_bounds__(NT) void residual1_kernel(int n, const double *A, long lda, const double *x,
+                                                       const double *b, double *r)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    constexpr int U = 4;
+    const int lane = threadIdx.x & 63;
+    const int r0 = ((blockIdx.x * NT + threadIdx.x) >> 6) * 4;
+    if (r0 >= n) return;
+    const int nr = min(4, n - r0);
+    const int ne = n & ~1;   // columns covered by 16-byte loads
+    const double *rows[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) rows[a] = A + (long)(r0 + min(a, nr - 1)) * lda;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 2 * lane; k0 < ne; k0 += 128 * U) {
+        d2 av[U][4], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = min(k0 + 128 * u, ne - 2);   // clamped: the surplus is masked below
+            xv[u] = *reinterpret_cast<const d2 *>(x + k);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[u][a] = *reinterpret_cast<const d2 *>(rows[a] + k);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = k0 + 128 * u < ne;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                acc[a] = fma(in ? av[u][a].x : 0.0, xv[u].x, acc[a]);
+                acc[a] = fma(in ? av[u][a].y : 0.0, xv[u].y, acc[a]);
+            }
+        }
+    }
+    if (lane == 0 && ne < n)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a] = fma(rows[a][ne], x[ne], acc[a]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) acc[a] = wave_sum(acc[a]);
+    if (lane == 0)
+        for (int a = 0; a < nr; ++a) r[r0 + a] = b[r0 + a] - acc[a];
+}
+
 // out[j] = max_i |V[i, j]| for j < ncols (ncols <= 64); one workgroup per column.
 template <typename T>
 __global__ __launch_bounds__(NT) void colmax_kernel(int n, int ncols, const T *V, long ldv, double *out)
@@ -139,7 +185,12 @@ int scirs_residual(int n, int nrhs, const double *A, long lda, const double *X, 
     const int waves = (n + 3) / 4;
     const int blocks = (waves * 64 + NT - 1) / NT;
     scirs::ProfScope ps(SCIRS_PROF_RESIDUAL, 2.0 * n * n * nrhs, 8.0 * n * n + 24.0 * n * nrhs, s);
-    hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(NT), 0, s, n, nrhs, A, lda, X, ldx, B, ldb, R, ldr);
+    const bool vec1 = nrhs == 1 && ldx == 1 && ldb == 1 && ldr == 1 && n >= 2 && lda % 2 == 0 &&
+                      ((uintptr_t)A & 15) == 0 && ((uintptr_t)X & 15) == 0;
+    if (vec1)
+        hipLaunchKernelGGL(residual1_kernel, dim3(blocks), dim3(NT), 0, s, n, A, lda, X, B, R);
+    else
+        hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(NT), 0, s, n, nrhs, A, lda, X, ldx, B, ldb, R, ldr);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }

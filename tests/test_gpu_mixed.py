@@ -106,3 +106,19 @@ def test_mixed_singular_reports_column(L):
     with pytest.raises(oracle.OracleError) as eo:
         oracle.lu_factor(a)
     assert e.value.rc == eo.value.code
+
+
+@pytest.mark.parametrize("n,nrhs", [(2, 1), (3, 1), (333, 1), (1000, 1), (1000, 3), (257, 5)])
+def test_backward_error_matches_host(L, n, nrhs):
+    """The device normwise backward error (FP64 residual, norms) against numpy,
+    for an x that is not a solution: one right-hand side (16-byte vector loads,
+    odd and even n) and several (the general residual kernel)."""
+    import torch
+    rng = np.random.default_rng(n + nrhs)
+    a = rng.standard_normal((n, n))
+    x = rng.standard_normal((n, nrhs))
+    b = rng.standard_normal((n, nrhs))
+    ta, tx, tb = (torch.from_numpy(v).cuda() for v in (a, x, b))
+    got = L.backward_error_dev(ta, tx, tb)
+    want = eta(a, x, b)
+    assert abs(got - want) <= 1e-12 * want, (got, want)
