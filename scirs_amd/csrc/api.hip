@@ -164,6 +164,7 @@ int Context::init(int dev)
     kb_f32 = env_int("SCIRS_GPU_KB_F32", 128);
     kb_chol = env_int("SCIRS_GPU_KB_CHOL", 128);
     narrow_kb = env_int("SCIRS_GPU_NARROW_KB", 1);
+    chol_chain = env_int("SCIRS_GPU_CHOL_CHAIN", 1);
     initialised = true;
     return SCIRS_OK;
 }
@@ -518,7 +519,18 @@ static int posv_device(int n, int nrhs, const double *dA, long lda, const double
     if ((rc = scirs_potrf_device<double>(n, dL, ldl, c.kb_chol, c.work, s))) return rc;
     SCIRS_HIP_CHECK(hipEventRecord(e1, s));
     if ((rc = scirs_d2d_copy(n, nrhs, dB, ldb, dX, ldx, s))) return rc;
-    if ((rc = scirs_potrs_device<double>(n, nrhs, dL, ldl, dX, ldx, c.work.info, s))) return rc;
+    if (c.chol_chain) {
+        // L y = b and L^T x = y as chained solves (true division by the diagonal,
+        // solve.rs:159-199), the second on an explicit row-major L^T
+        if ((rc = c.ensure(&c.scratch_t, &c.scratch_t_bytes, (size_t)n * n * sizeof(double)))) return rc;
+        double *dU = (double *)c.scratch_t;
+        if ((rc = scirs_transpose_lower<double>(n, dL, ldl, dU, n, s))) return rc;
+        if ((rc = c.ensure_xtri(n))) return rc;
+        if ((rc = scirs_trsv_chain<double>(n, nrhs, dL, ldl, dX, ldx, 1, 0, c.work, s))) return rc;
+        if ((rc = scirs_trsv_chain<double>(n, nrhs, dU, n, dX, ldx, 0, 0, c.work, s))) return rc;
+    } else if ((rc = scirs_potrs_device<double>(n, nrhs, dL, ldl, dX, ldx, c.work.info, s))) {
+        return rc;
+    }
     SCIRS_HIP_CHECK(hipEventRecord(e2, s));
     int st = 0;
     double pv = 0;

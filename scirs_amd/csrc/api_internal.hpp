@@ -20,6 +20,7 @@ struct Context {
     void *host_flags = nullptr;
     int kb_f64 = 128, kb_f32 = 128, kb_chol = 128;
     int narrow_kb = 1;   // narrower LU panels when a whole panel then fits one workgroup
+    int chol_chain = 1;  // Cholesky solves through the chained triangular solve (else blocked)
     // LU panel width for an m-row factorisation in element type T
     template <typename T> int lu_kb(int m) const
     {
@@ -33,6 +34,7 @@ struct Context {
     void *scratch_bwd = nullptr; size_t scratch_bwd_bytes = 0;
     void *scratch_ir = nullptr; size_t scratch_ir_bytes = 0;
     void *scratch_ir2 = nullptr; size_t scratch_ir2_bytes = 0;
+    void *scratch_t = nullptr; size_t scratch_t_bytes = 0;   // L^T of a Cholesky solve
     void *xtri_mem = nullptr; size_t xtri_bytes = 0;
     void *colexp_mem = nullptr; size_t colexp_bytes = 0;
 

@@ -48,6 +48,8 @@ enum {
 };
 
 enum { SCIRS_KB_MAX = 256, SCIRS_MAX_G = 512, SCIRS_PANEL_REP_MAX = 8 };
+// chains of a chained triangular solve launched together (one ticket each, in the flags block)
+enum { SCIRS_CHAIN_GROUPS = 32 };
 
 // Trailing (Schur) update C -= A B of the factorisations: FP64 MFMA for double,
 // the arithmetic chosen by w.lowp_mode for float.
@@ -94,6 +96,10 @@ int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int 
 
 template <typename T>
 int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);
+
+// U = L^T (upper triangle of U written; L's strict upper triangle must be zero)
+template <typename T>
+int scirs_transpose_lower(int n, const T *L, long ldl, T *U, long ldu, hipStream_t s);
 
 // Panel width for an m-row factorisation: the widest of kb, 64, 32 (never above
 // kb) at which all m rows of a panel fit one workgroup's LDS, so every panel is

@@ -86,10 +86,17 @@ __device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB],
 
 template <typename T, int QB, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A, long lda, T *X, long ldx, int n,
-                                                         int nr, u64 *xch, unsigned seq, int *ticket, int *abort,
+                                                         int nrhs, u64 *xch, unsigned seq, int *ticket, int *abort,
                                                          unsigned spin_limit, const int *info)
 {
     if (info && *info != 0) return;
+    // blockIdx.y: a group of QB right-hand sides, an independent chain with its
+    // own ticket and exchange region (the groups run side by side)
+    const int grp = blockIdx.y;
+    X += (size_t)grp * QB;
+    const int nr = min(QB, nrhs - grp * QB);
+    xch += (size_t)grp * n * QB * Tag<T>::W;
+    ticket += grp;
     constexpr int W = Tag<T>::W;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *Ld = reinterpret_cast<T *>(smem_raw);   // [TB][LDP] diagonal block
@@ -235,19 +242,20 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
 template <typename T, int QB, bool LOWER, bool UNIT>
 int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork &w, hipStream_t s)
 {
+    const int groups = (nr + QB - 1) / QB;
     const auto kern = trsv_chain_kernel<T, QB, LOWER, UNIT>;
     const size_t lds = (2 * (size_t)TB * LDP + 3 * (size_t)TB * QB) * sizeof(T);
     if (lds > 64 * 1024) {
         int rc = scirs_allow_lds((const void *)kern, lds);
         if (rc) return rc;
     }
-    int rc = scirs_fill_int(w.ticket, 1, 0, s);
+    int rc = scirs_fill_int(w.ticket, groups, 0, s);
     if (rc) return rc;
     const unsigned seq = w.seq_next + 1u;
     w.seq_next += 512u;
     const int nblk = (n + TB - 1) / TB;
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(NTC), lds, s, A, lda, X, ldx, n, nr, w.xtri, seq, w.ticket, w.abort,
-                       w.spin_limit, w.info);
+    hipLaunchKernelGGL(kern, dim3(nblk, groups), dim3(NTC), lds, s, A, lda, X, ldx, n, nr, w.xtri, seq, w.ticket,
+                       w.abort, w.spin_limit, w.info);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }
@@ -269,31 +277,70 @@ int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int 
                      hipStream_t s)
 {
     if (n <= 0 || nrhs <= 0) return SCIRS_OK;
+    // right-hand sides per chain; up to SCIRS_CHAIN_GROUPS chains run side by side in one launch
     constexpr int QMAX = sizeof(T) == 8 ? 2 : 4;
-    if ((size_t)n * QMAX * Tag<T>::W > w.xtri_words) {
-        scirs_set_error("chained triangular solve: exchange buffer holds %zu words, n = %d needs %zu", w.xtri_words,
-                        n, (size_t)n * QMAX * Tag<T>::W);
+    const int qb = nrhs == 1 ? 1 : (nrhs == 2 || QMAX == 2) ? 2 : 4;
+    const int per_launch = qb * SCIRS_CHAIN_GROUPS;
+    const size_t words = (size_t)n * std::min(nrhs, per_launch) * Tag<T>::W + (size_t)n * qb * Tag<T>::W;
+    if (words > w.xtri_words)
+        if (int rc = scirs::ctx().ensure_xtri((int)std::min<size_t>((words + 7) / 8, 1u << 30))) return rc;
+    if (words > w.xtri_words) {
+        scirs_set_error("chained triangular solve: exchange buffer holds %zu words, needs %zu", w.xtri_words, words);
         return SCIRS_ERR_ARG;
     }
     scirs::ProfScope prof(SCIRS_PROF_TRSV, (double)n * n * nrhs, (0.5 * n * n + 2.0 * n * nrhs) * sizeof(T), s);
-    for (int rb = 0; rb < nrhs;) {
-        const int left = nrhs - rb;
+    for (int rb = 0; rb < nrhs; rb += per_launch) {
+        const int nr = std::min(per_launch, nrhs - rb);
         int rc;
-        if (left == 1) {
-            rc = chain_dispatch<T, 1>(n, 1, A, lda, X + rb, ldx, lower, unit, w, s);
-            rb += 1;
-        } else if (QMAX == 2 || left == 2) {
-            rc = chain_dispatch<T, 2>(n, 2, A, lda, X + rb, ldx, lower, unit, w, s);
-            rb += 2;
-        } else {
-            const int nr = left < 4 ? left : 4;
-            rc = chain_dispatch<T, QMAX>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
-            rb += nr;
-        }
+        if (qb == 1) rc = chain_dispatch<T, 1>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
+        else if (qb == 2) rc = chain_dispatch<T, 2>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
+        else rc = chain_dispatch<T, QMAX>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
         if (rc) return rc;
     }
     return SCIRS_OK;
 }
+
+// U = L^T for the lower triangle of L (64 x 64 tiles through LDS; only the tiles
+// on or above U's diagonal are written).  Turns the Cholesky factor into the
+// row-major upper triangle the chained solve streams by rows.
+namespace {
+template <typename T>
+__global__ __launch_bounds__(NTC) void transpose_lower_kernel(int n, const T *__restrict__ L, long ldl, T *U, long ldu)
+{
+    __shared__ T tile[TB][TB + 1];
+    // blockIdx.x enumerates the lower-triangle tiles (bi >= bj) row by row
+    const int idx = blockIdx.x;
+    int bi = (int)((sqrtf(8.0f * (float)idx + 1.0f) - 1.0f) * 0.5f);
+    while (bi * (bi + 1) / 2 > idx) --bi;
+    while ((bi + 1) * (bi + 2) / 2 <= idx) ++bi;
+    const int bj = idx - bi * (bi + 1) / 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = bi * TB, c0 = bj * TB;
+    for (int i = wave; i < TB; i += NTC / 64) {
+        const int r = r0 + i, c = c0 + lane;
+        tile[i][lane] = (r < n && c < n) ? L[(long)r * ldl + c] : (T)0;
+    }
+    __syncthreads();
+    // U[c0 + i][r0 + lane] = L[r0 + lane][c0 + i]
+    for (int i = wave; i < TB; i += NTC / 64) {
+        const int r = c0 + i, c = r0 + lane;
+        if (r < n && c < n) U[(long)r * ldu + c] = tile[lane][i];
+    }
+}
+} // namespace
+
+template <typename T>
+int scirs_transpose_lower(int n, const T *L, long ldl, T *U, long ldu, hipStream_t s)
+{
+    if (n <= 0) return SCIRS_OK;
+    const long nb = (n + TB - 1) / TB;
+    const long tiles = nb * (nb + 1) / 2;
+    hipLaunchKernelGGL(transpose_lower_kernel<T>, dim3((unsigned)tiles), dim3(NTC), 0, s, n, L, ldl, U, ldu);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+template int scirs_transpose_lower<double>(int, const double *, long, double *, long, hipStream_t);
+template int scirs_transpose_lower<float>(int, const float *, long, float *, long, hipStream_t);
 
 template int scirs_trsv_chain<float>(int, int, const float *, long, float *, long, int, int, ScirsWork &, hipStream_t);
 template int scirs_trsv_chain<double>(int, int, const double *, long, double *, long, int, int, ScirsWork &,
