@@ -16,8 +16,9 @@
  * Host-memory entry points copy their inputs to HBM, compute, and copy the
  * results back; they keep no pointer after returning and serialise on one
  * internal mutex (thread safe).  The *_dev entry points take HBM pointers and
- * a hipStream_t (NULL = the library's stream) for data that is already
- * resident; they return after the stream has drained.
+ * a hipStream_t for data that is already resident (NULL = the default stream,
+ * as everywhere in HIP: the work is ordered after what the caller queued on
+ * it); they return after the stream has drained.
  */
 #ifndef SCIRS_GPU_H
 #define SCIRS_GPU_H

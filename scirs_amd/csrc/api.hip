@@ -128,7 +128,7 @@ int Context::init(int dev)
     work.max_panels = 1 << 14;   // n up to 16384 * kb
     work.spin_limit = (unsigned)env_int("SCIRS_GPU_SPIN_LIMIT", 1 << 22);
     work.lowp_mode = SCIRS_LOWP_FP32;   // plain FP32 entry points compute in FP32; the mixed solve switches it
-    work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 128);
+    work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 192);
     work.panel_w0 = env_int("SCIRS_GPU_PANEL_W0", 0);
     work.lookahead = env_int("SCIRS_GPU_LOOKAHEAD", 1);
     work.poll_sleep = env_int("SCIRS_GPU_POLL_SLEEP", 0);
@@ -447,7 +447,7 @@ extern "C" int scirs_gpu_dgesv_dev(int n, int nrhs, const double *dA, int64_t ld
     }
     SCIRS_ENTER(true);
     Context &c = ctx();
-    hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     int rc;
     if (!dLU) {
         if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(double)))) return finish(info, rc, t0);
@@ -593,7 +593,7 @@ extern "C" int scirs_gpu_dposv_dev(int n, int nrhs, const double *dA, int64_t ld
     }
     SCIRS_ENTER(true);
     Context &c = ctx();
-    hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     int rc;
     if (!dL) {
         if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(double)))) return finish(info, rc, t0);
@@ -655,6 +655,6 @@ extern "C" int scirs_gpu_backward_error_dev(int n, int nrhs, const double *dA, i
 {
     if (n <= 0 || nrhs <= 0 || !dA || !dX || !dB || !out) return SCIRS_ERR_ARG;
     SCIRS_ENTER(true);
-    hipStream_t s = stream ? (hipStream_t)stream : ctx().stream;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     return backward_error(n, nrhs, dA, lda, dX, ldx, dB, ldb, s, out);
 }

@@ -577,7 +577,7 @@ extern "C" int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *n
     Context &c = ctx();
     int rc = c.init(0);
     if (rc) return rc;
-    hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     BatchPlan &pl = g_plan;
 
     // ---- plan: descriptors, size-sorted queue, workspace (rebuilt when the batch changes)

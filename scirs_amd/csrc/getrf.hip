@@ -495,10 +495,11 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         if (int rc = scirs_allow_lds(pk, lds)) return rc;
         {
             scirs::ProfScope ps(SCIRS_PROF_PANEL, (double)mrows * kbw * kbw, 2.0 * mrows * kbw * sizeof(T), stream);
-            if (lookahead) {
+            if (lookahead || G == 1) {
                 // a normal launch, so the panel can share the device with the side
                 // stream's update: its G <= #CUs workgroups all become resident once
-                // that update's workgroups retire (they wait for nothing)
+                // that update's workgroups retire (they wait for nothing).  One
+                // workgroup exchanges nothing and needs no co-residency either.
                 if (w0)
                     hipLaunchKernelGGL((panel::getf2_panel_kernel<T, true>), dim3(G), dim3(panel::NT), lds, stream, pa);
                 else
@@ -506,7 +507,18 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
                 SCIRS_HIP_CHECK(hipGetLastError());
             } else {
                 void *args[] = {&pa};
-                SCIRS_HIP_CHECK(hipLaunchCooperativeKernel(pk, dim3(G), dim3(panel::NT), args, lds, stream));
+                if (hipLaunchCooperativeKernel(pk, dim3(G), dim3(panel::NT), args, lds, stream) != hipSuccess) {
+                    // cooperative launches can be refused while another process holds the
+                    // device's cooperative resources: a normal launch of G <= #CUs
+                    // workgroups on an idle device is co-resident in practice, and the
+                    // bounded polls turn the rare case that is not into an error
+                    (void)hipGetLastError();
+                    if (w0)
+                        hipLaunchKernelGGL((panel::getf2_panel_kernel<T, true>), dim3(G), dim3(panel::NT), lds, stream, pa);
+                    else
+                        hipLaunchKernelGGL((panel::getf2_panel_kernel<T, false>), dim3(G), dim3(panel::NT), lds, stream, pa);
+                    SCIRS_HIP_CHECK(hipGetLastError());
+                }
             }
         }
         const int nright = n - k0 - kbw;

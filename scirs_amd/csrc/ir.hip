@@ -498,7 +498,7 @@ extern "C" int scirs_gpu_dgesv_ir_dev(int n, int nrhs, const double *dA, int64_t
     std::lock_guard<std::recursive_mutex> lk(ctx().mu);
     int rc = ctx().init(0);
     if (rc) return finish(info, rc, t0);
-    hipStream_t s = stream ? (hipStream_t)stream : ctx().stream;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     rc = gesv_ir_device(n, nrhs, dA, lda, dB, ldb, dX, ldx, mode, max_iter, max_gmres, tol, s, info);
     return finish(info, rc, t0);
 }

@@ -291,3 +291,22 @@ def test_trsv_singular_diagonal(L):
     with pytest.raises(L.SingularError) as eg:
         L.trsv(a, np.ones(3), True, False)
     assert eg.value.rc == 2
+
+
+def test_dev_entry_points_follow_callers_default_stream(L):
+    """Inputs produced by torch on the default stream and handed over at once
+    (no synchronisation): the device entry points must queue behind that work.
+    Regression for a factorisation that read a half-written matrix."""
+    import torch
+    n = 6144
+    g = torch.Generator(device="cuda").manual_seed(7)
+    gg = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    s = gg @ gg.T.contiguous() / n + torch.eye(n, dtype=torch.float64, device="cuda")
+    b = torch.randn(n, 2, dtype=torch.float64, device="cuda", generator=g)
+    x = torch.empty_like(b)
+    L.posv_dev(s, b, x, torch.empty_like(s))
+    assert L.backward_error_dev(s, x, b) <= 8 * n * 2.0 ** -53
+    a = gg @ gg.T.contiguous() / n + torch.eye(n, dtype=torch.float64, device="cuda")
+    x2 = torch.empty_like(b)
+    L.gesv_dev(a, b, x2, torch.empty_like(a), torch.empty(n, dtype=torch.int64, device="cuda"))
+    assert L.backward_error_dev(a, x2, b) <= 8 * n * 2.0 ** -53
