@@ -15,38 +15,141 @@ namespace {
 constexpr int NT = 256;
 
 template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int k);
+template <>
+__device__ __forceinline__ float readlane_t<float>(float v, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+template <>
+__device__ __forceinline__ double readlane_t<double>(double v, int k)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The kb x kb diagonal block (kb <= 128) in LDS, factored 16 columns at a time:
+//   (a) wave 0 holds the 16 columns of every row in registers (lane = row mod 64,
+//       two rows per lane) and factors them column by column with no barrier:
+//       the diagonal by readlane, sqrt, true division of the rows below
+//       (lapack.rs:661-676), the later columns of the 16 updated with the
+//       column's entries broadcast by readlane;
+//   (b) all waves apply the rank-16 update to the lower triangle right of and
+//       below the 16 columns, in 4 x 4 register tiles.
+// The test of the reference, a_jj - sum <= 0 (lapack.rs:661-670), sets
+// info = row + 1 and stops.
+template <typename T>
 __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int kb, int *info, double *info_val)
 {
     if (*info != 0) return;
+    constexpr int PB = 16;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *S = reinterpret_cast<T *>(smem_raw);   // kb x (kb+1)
     const int p = kb + 1;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     T *D = A + (long)k0 * lda + k0;
-    g2s_block<T>(S, p, D, lda, kb, kb, kb, kb);   // (the strict upper part is never read)
+    g2s_block<T, 16>(S, p, D, lda, kb, kb, kb, kb);   // (the strict upper part is never used)
     __shared__ int s_fail;
     if (t == 0) s_fail = 0;
     __syncthreads();
-    for (int i = 0; i < kb; ++i) {
-        if (t == 0) {
-            T v = S[i * p + i];
-            if (!(v > (T)0)) {
+    for (int jb = 0; jb < kb; jb += PB) {
+        const int pw = min(PB, kb - jb);
+        if (wave == 0) {
+            T a[2][PB];
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl)
+#pragma unroll
+                for (int i = 0; i < PB; ++i) {
+                    const int row = lane + 64 * sl;
+                    a[sl][i] = (row < kb && i < pw) ? S[row * p + jb + i] : (T)0;
+                }
+            int fail_row = -1;
+            T fail_val = (T)0;
+#pragma unroll
+            for (int i = 0; i < PB; ++i) {
+                if (i < pw && fail_row < 0) {
+                    const int cj = jb + i;
+                    const T dv = readlane_t<T>((cj >> 6) ? a[1][i] : a[0][i], cj & 63);
+                    if (!(dv > (T)0)) {
+                        fail_row = cj;
+                        fail_val = dv;
+                    } else {
+                        const T d = sqrt(dv);
+#pragma unroll
+                        for (int sl = 0; sl < 2; ++sl) {
+                            const int row = lane + 64 * sl;
+                            if (row == cj) a[sl][i] = d;
+                            else if (row > cj) a[sl][i] = a[sl][i] / d;
+                        }
+#pragma unroll
+                        for (int i2 = i + 1; i2 < PB; ++i2) {
+                            if (i2 < pw) {
+                                const int c2 = jb + i2;
+                                const T lc = readlane_t<T>((c2 >> 6) ? a[1][i] : a[0][i], c2 & 63);
+#pragma unroll
+                                for (int sl = 0; sl < 2; ++sl)
+                                    if (lane + 64 * sl > cj) a[sl][i2] -= a[sl][i] * lc;
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl)
+#pragma unroll
+                for (int i = 0; i < PB; ++i) {
+                    const int row = lane + 64 * sl;
+                    if (row < kb && i < pw && row >= jb) S[row * p + jb + i] = a[sl][i];
+                }
+            if (fail_row >= 0 && lane == 0) {
                 s_fail = 1;
-                *info = k0 + i + 1;
-                *info_val = (double)v;
-            } else {
-                S[i * p + i] = sqrt(v);
+                *info = k0 + fail_row + 1;
+                *info_val = (double)fail_val;
             }
         }
         __syncthreads();
         if (s_fail) return;
-        const T d = S[i * p + i];
-        for (int r = i + 1 + t; r < kb; r += NT) S[r * p + i] = S[r * p + i] / d;
-        __syncthreads();
-        // trailing lower update of the block: S[r][c] -= S[r][i] * S[c][i], i < c <= r
-        for (int r = i + 1 + wave; r < kb; r += NT / 64) {
-            const T lri = S[r * p + i];
-            for (int c = i + 1 + lane; c <= r; c += 64) S[r * p + c] -= lri * S[c * p + i];
+        // (b) S[r][c] -= sum_k S[r][jb+k] S[c][jb+k] for je <= c <= r < kb, 4 x 4 tiles
+        const int je = jb + pw;
+        const int nt = (kb - je + 3) / 4;
+        const int ntiles = nt * (nt + 1) / 2;
+        for (int tile = t; tile < ntiles; tile += NT) {
+            int ti = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
+            while (ti * (ti + 1) / 2 > tile) --ti;
+            while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+            const int tj = tile - ti * (ti + 1) / 2;
+            const int r0 = je + 4 * ti, c0 = je + 4 * tj;
+            T acc[4][4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[x][y] = (T)0;
+            for (int k = 0; k < pw; k += 4) {
+                T pr[4][4], pc[4][4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const bool kk = k + q < pw;
+                        pr[x][q] = (kk && r0 + x < kb) ? S[(r0 + x) * p + jb + k + q] : (T)0;
+                        pc[x][q] = (kk && c0 + x < kb) ? S[(c0 + x) * p + jb + k + q] : (T)0;
+                    }
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) acc[x][y] += pr[x][q] * pc[y][q];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) {
+                    const int r = r0 + x, c = c0 + y;
+                    if (r < kb && c <= r) S[r * p + c] -= acc[x][y];
+                }
         }
         __syncthreads();
     }
