@@ -266,3 +266,17 @@ int scirs_syrk_update<float>(int N, int K, const float *A, long lda, float *C, l
 {
     return scirs_launch_sgemm_sub(N, N, K, A, lda, A, lda, 1, C, ldc, 1, w.lowp_mode, w.info, stream);
 }
+
+template <>
+int scirs_launch_gemm_sub_t<double>(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                                    double *C, long ldc, const ScirsWork &w, hipStream_t stream)
+{
+    return scirs_launch_dgemm_sub(M, N, K, A, lda, B, ldb, 1, C, ldc, 0, w.info, stream);
+}
+
+template <>
+int scirs_launch_gemm_sub_t<float>(int M, int N, int K, const float *A, long lda, const float *B, long ldb, float *C,
+                                   long ldc, const ScirsWork &w, hipStream_t stream)
+{
+    return scirs_launch_sgemm_sub(M, N, K, A, lda, B, ldb, 1, C, ldc, 0, w.lowp_mode, w.info, stream);
+}

@@ -60,6 +60,10 @@ int scirs_trailing_update(int M, int N, int K, const T *A, long lda, const T *B,
 template <typename T>
 int scirs_syrk_update(int N, int K, const T *A, long lda, T *C, long ldc, const ScirsWork &w,
                       hipStream_t stream);
+// C (M x N) -= A (M x K) B^T with B N x K (the Cholesky trailing update below a corner)
+template <typename T>
+int scirs_launch_gemm_sub_t(int M, int N, int K, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
+                            const ScirsWork &w, hipStream_t stream);
 
 template <typename T>
 int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);

@@ -234,11 +234,21 @@ __global__ void zero_upper_kernel(T *A, long lda, int n)
 }
 } // namespace
 
+// Look-ahead as in the LU: after block k's TRSM the main stream updates only
+// the next block column (rows below it) and goes on to block k+1; the rest of
+// the trailing lower triangle is updated on the side stream meanwhile.  Block
+// k's side-stream update also covers block k+2's columns, so block k+1's
+// next-column update waits for it.
 template <typename T>
 int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream)
 {
     if (n <= 0 || kb <= 0 || kb > 128) return SCIRS_ERR_ARG;
-    for (int k0 = 0; k0 < n; k0 += kb) {
+    const bool lookahead = w.lookahead && w.side_stream && n > 2 * kb;
+    if (lookahead) {
+        SCIRS_HIP_CHECK(hipEventRecord(w.ev_start, stream));
+        SCIRS_HIP_CHECK(hipStreamWaitEvent(w.side_stream, w.ev_start, 0));
+    }
+    for (int k0 = 0, p = 0; k0 < n; k0 += kb, ++p) {
         const int kbw = std::min(kb, n - k0);
         if (int rc = scirs_allow_lds((const void *)potf2_kernel<T>, (size_t)kbw * (kbw + 1) * sizeof(T))) return rc;
         hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(NT), (size_t)kbw * (kbw + 1) * sizeof(T), stream,
@@ -249,11 +259,39 @@ int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t 
             if (int rc = scirs_allow_lds((const void *)trsm_rltn_kernel<T>, lds)) return rc;
             hipLaunchKernelGGL(trsm_rltn_kernel<T>, dim3((rest + 63) / 64), dim3(NT), lds, stream,
                                A, lda, k0, kbw, k0 + kbw, rest, (const int *)w.info);
-            int rc = scirs_syrk_update<T>(rest, kbw, A + (long)(k0 + kbw) * lda + k0, lda,
-                                          A + (long)(k0 + kbw) * lda + k0 + kbw, lda, w, stream);
-            if (rc) return rc;
+            const T *L21 = A + (long)(k0 + kbw) * lda + k0;
+            T *A22 = A + (long)(k0 + kbw) * lda + k0 + kbw;
+            const int nn = lookahead ? std::min(kb, rest) : rest;   // columns updated on the main stream
+            if (lookahead) {
+                // the previous block's side-stream update covers these columns
+                if (p > 0) SCIRS_HIP_CHECK(hipStreamWaitEvent(stream, w.ev_upd, 0));
+                if (rest > nn) {
+                    SCIRS_HIP_CHECK(hipEventRecord(w.ev_trsm, stream));
+                    SCIRS_HIP_CHECK(hipStreamWaitEvent(w.side_stream, w.ev_trsm, 0));
+                    int rc = scirs_syrk_update<T>(rest - nn, kbw, L21 + (long)nn * lda, lda,
+                                                  A22 + (long)nn * lda + nn, lda, w, w.side_stream);
+                    if (rc) return rc;
+                }
+                SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, w.side_stream));
+                // A22[:, 0:nn) for every row below the block: the lower nn x nn corner, then
+                // the rows under it
+                int rc = scirs_syrk_update<T>(nn, kbw, L21, lda, A22, lda, w, stream);
+                if (rc) return rc;
+                if (rest > nn) {
+                    rc = scirs_launch_gemm_sub_t<T>(rest - nn, nn, kbw, L21 + (long)nn * lda, lda, L21, lda,
+                                                    A22 + (long)nn * lda, lda, w, stream);
+                    if (rc) return rc;
+                }
+            } else {
+                int rc = scirs_syrk_update<T>(rest, kbw, L21, lda, A22, lda, w, stream);
+                if (rc) return rc;
+            }
         }
         SCIRS_HIP_CHECK(hipGetLastError());
+    }
+    if (lookahead) {
+        SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, w.side_stream));
+        SCIRS_HIP_CHECK(hipStreamWaitEvent(stream, w.ev_upd, 0));
     }
     long total = (long)n * n;
     hipLaunchKernelGGL(zero_upper_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, A, lda, n);
