@@ -10,6 +10,8 @@
 //   lu               scirs2-linalg/src/decomposition.rs:108-183
 //   cholesky         scirs2-linalg/src/decomposition.rs:44-81
 //   inv, det         scirs2-linalg/src/basic.rs:117-186, 29-93
+//   mixed_precision_solve, iterative_refinement_solve
+//                    scirs2-linalg/src/mixed_precision/adaptive.rs:49-151, 282-378
 //   validation       scirs2-linalg/src/validation.rs:22-329
 //   LinalgError      scirs2-linalg/src/error.rs:8-150
 //
@@ -19,6 +21,7 @@
 
 #include "scirs_gpu.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -539,6 +542,119 @@ Result<Array2<F>> inv(const ArrayView2<F> &a, std::optional<std::size_t> workers
     if (r.is_err() && r.error().kind() == LinalgError::Kind::SingularMatrixError)
         return singular_matrix_with_suggestions("matrix inverse via solve", a.dim(), std::nullopt);
     return r;
+}
+
+// ------------------------------------------------------------------ mixed precision
+// `{:?}` of an ndarray shape slice: "[r, c]"
+inline std::string shape_dbg(std::size_t r, std::size_t c)
+{
+    return "[" + std::to_string(r) + ", " + std::to_string(c) + "]";
+}
+
+// mixed_precision/adaptive.rs:49-151: solve A x = b with the elimination done
+// in precision H and the result returned in C.  The reference eliminates on
+// [A|b] with the first-max partial pivot and the absolute test
+// max |a_ji| < H::epsilon(); the GPU LU in precision H follows the same rule, so
+// a singular system fails the same way (SingularMatrixError "Matrix is
+// singular").
+template <class A, class B, class C, class H>
+Result<Array1<C>> mixed_precision_solve(const ArrayView2<A> &a, const ArrayView1<B> &b)
+{
+    static_assert(detail::supported<H>, "the GPU path computes in f32 or f64");
+    if (a.rows != a.cols)
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           "Matrix must be square, got shape " + shape_dbg(a.rows, a.cols));
+    if (a.rows != b.len)
+        return LinalgError(LinalgError::Kind::ShapeError, "Matrix rows (" + std::to_string(a.rows) +
+                                                              ") must match vector length (" +
+                                                              std::to_string(b.len) + ")");
+    const std::size_t n = a.rows;
+    Array1<C> out(n, C(0));
+    if (n == 0) return out;
+    std::vector<H> ah(n * n), bh(n), xh(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        for (std::size_t j = 0; j < n; ++j) ah[i * n + j] = static_cast<H>(a(i, j));
+        bh[i] = static_cast<H>(b[i]);
+    }
+    scirs_solve_info info;
+    int rc = detail::gesv<H>((int)n, 1, ah.data(), bh.data(), xh.data(), &info);
+    if (rc > 0) return LinalgError(LinalgError::Kind::SingularMatrixError, "Matrix is singular");
+    if (rc < 0) return detail::gpu_failure(rc);
+    for (std::size_t i = 0; i < n; ++i) out[i] = static_cast<C>(xh[i]);
+    return out;
+}
+
+// mixed_precision/adaptive.rs:282-378: solve in working precision W, refine in
+// higher precision H, return C.  The reference re-factorises in W for every
+// correction and stops on the absolute ||r||_inf < tol (default 1e-8) or after
+// max_iter (default 10) corrections.  Here, with H = f64 and W = f32, the
+// system is factored once in FP32 on the GPU and refined with FP64 residuals
+// until the normwise backward error reaches n * 2^-53 (scirs_gpu_dgesv_ir): the
+// FP64-accurate solution, at least as accurate as the reference's stopping rule
+// gives; `tol` is not used on that path.  Errors of the working-precision
+// solve are those of `solve` (solve.rs:54-111).
+template <class A, class B, class C, class H, class W>
+Result<Array1<C>> iterative_refinement_solve(const ArrayView2<A> &a, const ArrayView1<B> &b,
+                                             std::optional<std::size_t> max_iter, std::optional<H> tol)
+{
+    static_assert(detail::supported<H> && detail::supported<W>, "the GPU path computes in f32 or f64");
+    const std::size_t iters = max_iter.value_or(10);
+    if (a.rows != a.cols)
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           "Expected square matrix for iterative refinement, got " + shape_dbg(a.rows, a.cols));
+    if (a.rows != b.len)
+        return LinalgError(LinalgError::Kind::ShapeError, "Matrix shape " + shape_dbg(a.rows, a.cols) +
+                                                              " is incompatible with vector length " +
+                                                              std::to_string(b.len));
+    const std::size_t n = a.rows;
+    Array1<C> out(n, C(0));
+    if constexpr (std::is_same<H, double>::value && std::is_same<W, float>::value) {
+        if (n > 4) {
+            std::vector<double> ah(n * n), bh(n), xh(n);
+            for (std::size_t i = 0; i < n; ++i) {
+                for (std::size_t j = 0; j < n; ++j) ah[i * n + j] = static_cast<double>(a(i, j));
+                bh[i] = static_cast<double>(b[i]);
+            }
+            scirs_solve_info info;
+            int rc = scirs_gpu_dgesv_ir((int)n, 1, ah.data(), (int64_t)n, bh.data(), 1, xh.data(), 1,
+                                        SCIRS_MODE_AUTO, (int)iters, 0, 0.0, &info);
+            if (rc > 0) return singular_matrix_with_suggestions("linear system solve", a.dim(), std::nullopt);
+            if (rc < 0) return detail::gpu_failure(rc);
+            for (std::size_t i = 0; i < n; ++i) out[i] = static_cast<C>(xh[i]);
+            return out;
+        }
+    }
+    // Otherwise the reference's loop itself (n <= 4, or other precisions):
+    // solves in W through `solve` (the GPU for n > 4, the inverse branch below),
+    // residuals in H on the host (O(n^2)), stop on ||r||_inf < tol.
+    const H tl = tol.value_or(static_cast<H>(1e-8));
+    std::vector<W> aw(n * n), bw(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        for (std::size_t j = 0; j < n; ++j) aw[i * n + j] = static_cast<W>(a(i, j));
+        bw[i] = static_cast<W>(b[i]);
+    }
+    ArrayView2<W> av{aw.data(), n, n, (std::ptrdiff_t)n, 1};
+    auto x = solve<W>(av, view1(bw), std::nullopt);
+    if (x.is_err()) return x.error();
+    std::vector<H> xh(n), r(n);
+    for (std::size_t i = 0; i < n; ++i) xh[i] = static_cast<H>(x.value()[i]);
+    for (std::size_t it = 0; it < iters; ++it) {
+        H rn = H(0);
+        for (std::size_t i = 0; i < n; ++i) {
+            H ax = H(0);
+            for (std::size_t j = 0; j < n; ++j) ax += static_cast<H>(a(i, j)) * xh[j];
+            r[i] = static_cast<H>(b[i]) - ax;
+            rn = std::max(rn, std::abs(r[i]));
+        }
+        if (rn < tl) break;
+        std::vector<W> rw(n);
+        for (std::size_t i = 0; i < n; ++i) rw[i] = static_cast<W>(r[i]);
+        auto dx = solve<W>(av, view1(rw), std::nullopt);
+        if (dx.is_err()) return dx.error();
+        for (std::size_t i = 0; i < n; ++i) xh[i] += static_cast<H>(dx.value()[i]);
+    }
+    for (std::size_t i = 0; i < n; ++i) out[i] = static_cast<C>(xh[i]);
+    return out;
 }
 
 } // namespace scirs2_linalg

@@ -3,7 +3,8 @@
 //
 // Input, whitespace separated, one case after another:
 //   <op> <type> <flag1> <flag2> <ar> <ac> <ar*ac values> <br> <bc> <br*bc values>
-//   op   : solve | solve_multiple | lu | cholesky | solve_triangular | det | inv
+//   op   : solve | solve_multiple | lu | cholesky | solve_triangular | det | inv |
+//          mixed_precision_solve | iterative_refinement_solve
 //   type : f64 | f32
 //   flags: lower / unit_diagonal for solve_triangular, ignored otherwise
 //   b    : the right-hand side (br = length, bc = 1 for a vector; 0 0 when unused)
@@ -105,6 +106,21 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
         if (r.is_err()) return print_err(r.error());
         std::printf("OK 1");
         put(r.value().rows, r.value().cols, r.value().data.data());
+    } else if (op == "mixed_precision_solve") {
+        // f1 selects the elimination precision H: 0 f64, 1 f32 (A, B, C = the case type)
+        auto r = f1 ? mixed_precision_solve<F, F, F, float>(a.view(), view1(bvec))
+                    : mixed_precision_solve<F, F, F, double>(a.view(), view1(bvec));
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().size(), 1, r.value().data());
+    } else if (op == "iterative_refinement_solve") {
+        // H = f64; f1 selects the working precision W: 0 f32, 1 f64; f2 = max_iter (0: default)
+        const std::optional<std::size_t> it = f2 > 0 ? std::optional<std::size_t>(f2) : std::nullopt;
+        auto r = f1 ? iterative_refinement_solve<F, F, F, double, double>(a.view(), view1(bvec), it, std::nullopt)
+                    : iterative_refinement_solve<F, F, F, double, float>(a.view(), view1(bvec), it, std::nullopt);
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().size(), 1, r.value().data());
     } else {
         std::printf("ERR Other unknown op %s\n", op.c_str());
         return;

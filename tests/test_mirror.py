@@ -322,3 +322,64 @@ def test_mirror_compute_matches_oracle():
         assert g[0] == want, (label, g)
         res = chk(g)
         assert res is None or res, (label, g)
+
+
+def test_mixed_precision_dropins_validation_texts():
+    """mixed_precision_solve / iterative_refinement_solve (mixed_precision/
+    adaptive.rs:49-151, 282-378): the reference's own shape-error texts (the
+    `{:?}` of the shape slice) before any device work."""
+    a34 = np.ones((3, 4))
+    a33 = np.eye(3)
+    got = run([
+        ("mixed_precision_solve", "f64", 0, 0, a34, np.ones(3)),
+        ("mixed_precision_solve", "f64", 0, 0, a33, np.ones(2)),
+        ("iterative_refinement_solve", "f64", 0, 0, a34, np.ones(3)),
+        ("iterative_refinement_solve", "f64", 0, 0, a33, np.ones(2)),
+    ])
+    assert got[0] == ("ERR", "ShapeError", "Shape error: Matrix must be square, got shape [3, 4]")
+    assert got[1] == ("ERR", "ShapeError", "Shape error: Matrix rows (3) must match vector length (2)")
+    assert got[2] == ("ERR", "ShapeError",
+                      "Shape error: Expected square matrix for iterative refinement, got [3, 4]")
+    assert got[3] == ("ERR", "ShapeError",
+                      "Shape error: Matrix shape [3, 3] is incompatible with vector length 2")
+
+
+@pytest.mark.gpu
+def test_mixed_precision_dropins_match_oracle():
+    """The drop-ins on the GPU: the FP64 elimination of mixed_precision_solve and
+    the FP32-factor + FP64-refinement form of iterative_refinement_solve both give
+    the FP64 solution (backward error <= 8 n 2^-53, the oracle's solution within
+    8 n u kappa); an FP32 elimination gives the FP32 one; a singular system fails
+    as the reference's elimination (`Matrix is singular`) and working-precision
+    solve (`linear system solve`) do."""
+    rng = np.random.default_rng(77)
+    n = 200
+    a = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    b = rng.standard_normal(n)
+    xo = oracle.solve(a, b)
+    kappa = np.linalg.cond(a, np.inf)
+    sing = rng.standard_normal((8, 8))
+    sing[:, 5] = 0.0
+    a3 = rng.standard_normal((3, 3)) + 3 * np.eye(3)
+    b3 = rng.standard_normal(3)
+    got = run([
+        ("mixed_precision_solve", "f64", 0, 0, a, b),
+        ("mixed_precision_solve", "f64", 1, 0, a, b),
+        ("iterative_refinement_solve", "f64", 0, 0, a, b),
+        ("iterative_refinement_solve", "f64", 1, 0, a, b),
+        ("iterative_refinement_solve", "f64", 0, 3, a3, b3),
+        ("mixed_precision_solve", "f64", 0, 0, sing, np.ones(8)),
+        ("iterative_refinement_solve", "f64", 0, 0, sing, np.ones(8)),
+    ])
+    for i in (0, 2, 3):
+        assert got[i][0] == "OK", got[i]
+        x = got[i][1][0].ravel()
+        assert _bwd(a, x, b) <= 8 * n * U64, i
+        assert np.abs(x - xo).max() <= 8 * n * U64 * kappa * np.abs(xo).max(), i
+    x32 = got[1][1][0].ravel()
+    assert np.abs(x32 - xo).max() <= 8 * n * 2.0 ** -24 * kappa * np.abs(xo).max()
+    assert got[4][0] == "OK"
+    # n <= 3: FP32 inverse-branch solve, then the reference's FP64 refinement loop
+    assert np.abs(got[4][1][0].ravel() - np.linalg.solve(a3, b3)).max() <= 1e-12
+    assert got[5] == ("ERR", "SingularMatrixError", "Singular matrix error: Matrix is singular")
+    assert got[6] == ("ERR", "SingularMatrixError", singular("linear system solve", 8, 8))
