@@ -196,24 +196,43 @@ __global__ __launch_bounds__(NT) void trsm_rltn_kernel(T *A, long lda, int k0, i
     for (int jb = 0; jb < kb; jb += 8) {
         const int je = min(jb + 8, kb);
         if (rg == 0) {
-            for (int j = jb; j < je; ++j) {
-                T s = Bs[j * 65 + lane];
-                const T *Lj = Ls + (size_t)j * (j + 1) / 2;
-                for (int q = jb; q < j; ++q) s -= Lj[q] * Bs[q * 65 + lane];
-                Bs[j * 65 + lane] = s / Lj[j];
+            // the 8 columns of the block in registers: x_j = (b_j - sum_q L[j][q] x_q) / L[j][j]
+            T x[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = (jb + q < je) ? Bs[(jb + q) * 65 + lane] : (T)0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (jb + q < je) {
+                    const T *Lj = Ls + (size_t)(jb + q) * (jb + q + 1) / 2 + jb;
+#pragma unroll
+                    for (int pp = 0; pp < q; ++pp) x[q] -= Lj[pp] * x[pp];
+                    x[q] = x[q] / Lj[q];
+                    Bs[(jb + q) * 65 + lane] = x[q];
+                }
             }
         }
         __syncthreads();
         T x[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) x[q] = (jb + q < je) ? Bs[(jb + q) * 65 + lane] : (T)0;
-        for (int j = je + rg; j < kb; j += NT / 64) {
-            const T *Lj = Ls + (size_t)j * (j + 1) / 2 + jb;
-            T s = Bs[j * 65 + lane];
+        // later columns, four per wave at a time (all loads issued before the arithmetic)
+        for (int j0 = je + rg; j0 < kb; j0 += 4 * (NT / 64)) {
+            T acc[4], lv[4][8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (jb + q < je) s -= Lj[q] * x[q];
-            Bs[j * 65 + lane] = s;
+            for (int u = 0; u < 4; ++u) {
+                const int j = min(j0 + u * (NT / 64), kb - 1);
+                acc[u] = Bs[j * 65 + lane];
+                const T *Lj = Ls + (size_t)j * (j + 1) / 2 + jb;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) lv[u][q] = (jb + q < je) ? Lj[q] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[u] -= lv[u][q] * x[q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j0 + u * (NT / 64) < kb) Bs[(j0 + u * (NT / 64)) * 65 + lane] = acc[u];
         }
         __syncthreads();
     }
