@@ -21,6 +21,7 @@ struct Context {
     int kb_f64 = 128, kb_f32 = 128, kb_chol = 128;
     int narrow_kb = 1;   // narrower LU panels when a whole panel then fits one workgroup
     int chol_chain = 1;  // Cholesky solves through the chained triangular solve (else blocked)
+    int trsv_dinv = 1;   // FP32 refinement solves with inverted 64 x 64 diagonal blocks
     // LU panel width for an m-row factorisation in element type T
     template <typename T> int lu_kb(int m) const
     {
@@ -46,9 +47,11 @@ struct Context {
     int ensure_xtri(int n);
 };
 
-// X <- U^-1 L^-1 X with the packed LU factors (X already row-permuted).
+// X <- U^-1 L^-1 X with the packed LU factors (X already row-permuted);
+// dinv_l / dinv_u (optional): inverses of the 64 x 64 diagonal blocks of L and U.
 template <typename T>
-int lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s);
+int lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s,
+                  const T *dinv_l = nullptr, const T *dinv_u = nullptr);
 
 Context &ctx();
 double now_ms();

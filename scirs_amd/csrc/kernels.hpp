@@ -94,9 +94,15 @@ int scirs_trsv_big(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lo
 
 // Chained triangular solve in one launch (64-row block rows handed over through
 // w.xtri, which must hold n * 4 * 2 words): X <- op(A)^-1 X as scirs_trsv_big.
+// dinv (optional): inverses of the 64 x 64 diagonal blocks (scirs_tri_block_inverse),
+// applied instead of the substitution inside each block
 template <typename T>
 int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
-                     hipStream_t s);
+                     hipStream_t s, const T *dinv = nullptr);
+
+// Dinv[p] = (T_pp)^-1 for the 64 x 64 diagonal blocks of the lower (unit or
+// not) or upper triangle of the FP32 matrix A; dinv holds ceil(n/64) * 64 * 64 floats
+int scirs_tri_block_inverse(int n, const float *A, long lda, int lower, int unit, float *dinv, hipStream_t s);
 
 template <typename T>
 int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t stream);

@@ -87,7 +87,8 @@ __device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB],
 template <typename T, int QB, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A, long lda, T *X, long ldx, int n,
                                                          int nrhs, u64 *xch, unsigned seq, int *ticket, int *abort,
-                                                         unsigned spin_limit, const int *info)
+                                                         unsigned spin_limit, const int *info,
+                                                         const T *__restrict__ dinv)
 {
     if (info && *info != 0) return;
     // blockIdx.y: a group of QB right-hand sides, an independent chain with its
@@ -157,9 +158,15 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
                 if ((lane & 15) == m) part[((wave - 1) * TB + 4 * m + g) * QB + q] = v;
             }
     } else {
-        // ---- wave 0: stage the diagonal block and the last dependency's tile (coalesced rows)
-        for (int i = 0; i < TB; ++i)
-            Ld[i * LDP + lane] = (i < bs && lane < bs) ? A[(long)(r0 + i) * lda + r0 + lane] : (T)0;
+        // ---- wave 0: stage the diagonal block (or its inverse) and the last
+        //      dependency's tile (coalesced rows)
+        if (dinv) {
+            const T *Dp = dinv + (size_t)p * TB * TB;
+            for (int i = 0; i < TB; ++i) Ld[i * LDP + lane] = Dp[i * TB + lane];
+        } else {
+            for (int i = 0; i < TB; ++i)
+                Ld[i * LDP + lane] = (i < bs && lane < bs) ? A[(long)(r0 + i) * lda + r0 + lane] : (T)0;
+        }
         if (b > 0) {
             const int cb = LOWER ? b - 1 : nblk - b;
             const int c0 = cb * TB, cs = min(TB, n - c0);
@@ -192,6 +199,21 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         }
     }
     if (failed) return;
+    if (dinv) {
+        // x = D^-1 s with the block's precomputed inverse: 64 independent
+        // products instead of the 64-step dependent substitution
+        T xo[QB];
+#pragma unroll
+        for (int q = 0; q < QB; ++q) xo[q] = (T)0;
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            const T d = Ld[lane * LDP + k];
+#pragma unroll
+            for (int q = 0; q < QB; ++q) xo[q] += d * readlane(s[q], k);
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) s[q] = xo[q];
+    } else {
     // FP64 divides by the diagonal as the reference does; FP32 multiplies by a
     // reciprocal each lane computes up front, which keeps the division off the
     // 64-step dependent chain (the FP32 factors only precondition refinement).
@@ -230,6 +252,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
             }
         }
     }
+    }
     if (lane < bs) {
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
@@ -240,7 +263,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
 }
 
 template <typename T, int QB, bool LOWER, bool UNIT>
-int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork &w, hipStream_t s)
+int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork &w, hipStream_t s, const T *dinv)
 {
     const int groups = (nr + QB - 1) / QB;
     const auto kern = trsv_chain_kernel<T, QB, LOWER, UNIT>;
@@ -255,26 +278,26 @@ int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork 
     w.seq_next += 512u;
     const int nblk = (n + TB - 1) / TB;
     hipLaunchKernelGGL(kern, dim3(nblk, groups), dim3(NTC), lds, s, A, lda, X, ldx, n, nr, w.xtri, seq, w.ticket,
-                       w.abort, w.spin_limit, w.info);
+                       w.abort, w.spin_limit, w.info, dinv);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }
 
 template <typename T, int QB>
 int chain_dispatch(int n, int nr, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
-                   hipStream_t s)
+                   hipStream_t s, const T *dinv)
 {
-    if (lower && unit) return launch_chain<T, QB, true, true>(n, nr, A, lda, X, ldx, w, s);
-    if (lower) return launch_chain<T, QB, true, false>(n, nr, A, lda, X, ldx, w, s);
-    if (unit) return launch_chain<T, QB, false, true>(n, nr, A, lda, X, ldx, w, s);
-    return launch_chain<T, QB, false, false>(n, nr, A, lda, X, ldx, w, s);
+    if (lower && unit) return launch_chain<T, QB, true, true>(n, nr, A, lda, X, ldx, w, s, dinv);
+    if (lower) return launch_chain<T, QB, true, false>(n, nr, A, lda, X, ldx, w, s, dinv);
+    if (unit) return launch_chain<T, QB, false, true>(n, nr, A, lda, X, ldx, w, s, dinv);
+    return launch_chain<T, QB, false, false>(n, nr, A, lda, X, ldx, w, s, dinv);
 }
 
 } // namespace
 
 template <typename T>
 int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
-                     hipStream_t s)
+                     hipStream_t s, const T *dinv)
 {
     if (n <= 0 || nrhs <= 0) return SCIRS_OK;
     // right-hand sides per chain; up to SCIRS_CHAIN_GROUPS chains run side by side in one launch
@@ -292,11 +315,63 @@ int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int 
     for (int rb = 0; rb < nrhs; rb += per_launch) {
         const int nr = std::min(per_launch, nrhs - rb);
         int rc;
-        if (qb == 1) rc = chain_dispatch<T, 1>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
-        else if (qb == 2) rc = chain_dispatch<T, 2>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
-        else rc = chain_dispatch<T, QMAX>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s);
+        if (qb == 1) rc = chain_dispatch<T, 1>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv);
+        else if (qb == 2) rc = chain_dispatch<T, 2>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv);
+        else rc = chain_dispatch<T, QMAX>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv);
         if (rc) return rc;
     }
+    return SCIRS_OK;
+}
+
+// Inverses of the 64 x 64 diagonal blocks of a triangle (FP32 factors that
+// only precondition refinement): Dinv[p] (row-major 64 x 64) with
+// Dinv[p] * T_pp = I.  One wave per block, lane = column c of the inverse,
+// found by substitution on e_c in registers (the block rows broadcast from
+// LDS); rows past n are the identity.
+namespace {
+template <bool LOWER, bool UNIT>
+__global__ __launch_bounds__(64) void tri_block_inverse_kernel(int n, const float *__restrict__ A, long lda,
+                                                               float *__restrict__ dinv)
+{
+    __shared__ float D[TB][TB + 1];
+    const int p = blockIdx.x, lane = threadIdx.x;
+    const int r0 = p * TB, bs = min(TB, n - r0);
+    for (int i = 0; i < TB; ++i)
+        D[i][lane] = (i < bs && lane < bs) ? A[(long)(r0 + i) * lda + r0 + lane] : (i == lane ? 1.0f : 0.0f);
+    __syncthreads();
+    float y[TB];
+    if (LOWER) {
+#pragma unroll
+        for (int i = 0; i < TB; ++i) {
+            float v = (i == lane) ? 1.0f : 0.0f;
+#pragma unroll
+            for (int j = 0; j < i; ++j) v -= D[i][j] * y[j];
+            y[i] = UNIT ? v : v / D[i][i];
+        }
+    } else {
+#pragma unroll
+        for (int i = TB - 1; i >= 0; --i) {
+            float v = (i == lane) ? 1.0f : 0.0f;
+#pragma unroll
+            for (int j = i + 1; j < TB; ++j) v -= D[i][j] * y[j];
+            y[i] = UNIT ? v : v / D[i][i];
+        }
+    }
+    float *Dp = dinv + (size_t)p * TB * TB;
+#pragma unroll
+    for (int i = 0; i < TB; ++i) Dp[i * TB + lane] = y[i];
+}
+} // namespace
+
+int scirs_tri_block_inverse(int n, const float *A, long lda, int lower, int unit, float *dinv, hipStream_t s)
+{
+    if (n <= 0) return SCIRS_OK;
+    const int nblk = (n + TB - 1) / TB;
+    if (lower && unit) hipLaunchKernelGGL((tri_block_inverse_kernel<true, true>), dim3(nblk), dim3(64), 0, s, n, A, lda, dinv);
+    else if (lower) hipLaunchKernelGGL((tri_block_inverse_kernel<true, false>), dim3(nblk), dim3(64), 0, s, n, A, lda, dinv);
+    else if (unit) hipLaunchKernelGGL((tri_block_inverse_kernel<false, true>), dim3(nblk), dim3(64), 0, s, n, A, lda, dinv);
+    else hipLaunchKernelGGL((tri_block_inverse_kernel<false, false>), dim3(nblk), dim3(64), 0, s, n, A, lda, dinv);
+    SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }
 
@@ -342,6 +417,7 @@ int scirs_transpose_lower(int n, const T *L, long ldl, T *U, long ldu, hipStream
 template int scirs_transpose_lower<double>(int, const double *, long, double *, long, hipStream_t);
 template int scirs_transpose_lower<float>(int, const float *, long, float *, long, hipStream_t);
 
-template int scirs_trsv_chain<float>(int, int, const float *, long, float *, long, int, int, ScirsWork &, hipStream_t);
+template int scirs_trsv_chain<float>(int, int, const float *, long, float *, long, int, int, ScirsWork &, hipStream_t,
+                                     const float *);
 template int scirs_trsv_chain<double>(int, int, const double *, long, double *, long, int, int, ScirsWork &,
-                                      hipStream_t);
+                                      hipStream_t, const double *);
