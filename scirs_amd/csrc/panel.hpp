@@ -272,19 +272,23 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
             T sp[SPEC], sj[SPEC];
             const u64 *xj = XJ(b, myrep);
             int guess = -1;   // best workgroup of the previous poll: its row is read with this poll's records
+            bool have = false;   // sp (and sj) already hold the guess's complete row: not read again
             while (true) {
                 bool ok = true;
                 T v = (T)0; int i = -1, wg = -1;
-                bool rok = guess >= 0;
+                const bool fetch = guess >= 0 && !have;
+                bool rok = fetch || have;
                 const u64 *xr = XR(b, guess >= 0 ? guess : 0, myrep);
                 // row j is needed only by the workgroup that holds the pivot row
                 const bool want_j = guess == g;
+                if (fetch) {
 #pragma unroll
-                for (int u = 0; u < SPEC; ++u) {
-                    const int q = lane + 64 * u;
-                    if (q < kb) {
-                        if (guess >= 0) rok = Tag<T>::get(xr + (size_t)q * W, seq, sp[u]) && rok;
-                        if (want_j) rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
+                    for (int u = 0; u < SPEC; ++u) {
+                        const int q = lane + 64 * u;
+                        if (q < kb) {
+                            rok = Tag<T>::get(xr + (size_t)q * W, seq, sp[u]) && rok;
+                            if (want_j) rok = Tag<T>::get(xj + (size_t)q * W, seq, sj[u]) && rok;
+                        }
                     }
                 }
                 for (int q = lane; q < G; q += 64) {
@@ -302,7 +306,8 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
                 const int src = ball ? __ffsll(ball) - 1 : 0;
                 wgw = __builtin_amdgcn_readlane(wg, src);
                 if (__all(ok)) { row_ok = wgw >= 0 && wgw == guess && __all(rok); mine = true; break; }
-                guess = wgw;
+                if (wgw != guess) { guess = wgw; have = false; }
+                else if (fetch && __all(rok)) have = true;
                 for (int z = 0; z < a.poll_sleep; ++z) __builtin_amdgcn_s_sleep(2);
                 if (++spins > a.spin_limit ||
                     __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
