@@ -292,15 +292,11 @@ int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t 
                     if (rc) return rc;
                 }
                 SCIRS_HIP_CHECK(hipEventRecord(w.ev_upd, w.side_stream));
-                // A22[:, 0:nn) for every row below the block: the lower nn x nn corner, then
-                // the rows under it
-                int rc = scirs_syrk_update<T>(nn, kbw, L21, lda, A22, lda, w, stream);
+                // A22[:, 0:nn) for every row below the block, in one launch (the strict
+                // upper part of the nn x nn corner is updated too: it is never read,
+                // and the factorisation zeroes it at the end)
+                int rc = scirs_launch_gemm_sub_t<T>(rest, nn, kbw, L21, lda, L21, lda, A22, lda, w, stream);
                 if (rc) return rc;
-                if (rest > nn) {
-                    rc = scirs_launch_gemm_sub_t<T>(rest - nn, nn, kbw, L21 + (long)nn * lda, lda, L21, lda,
-                                                    A22 + (long)nn * lda, lda, w, stream);
-                    if (rc) return rc;
-                }
             } else {
                 int rc = scirs_syrk_update<T>(rest, kbw, L21, lda, A22, lda, w, stream);
                 if (rc) return rc;
