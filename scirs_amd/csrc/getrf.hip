@@ -495,11 +495,16 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         if (int rc = scirs_allow_lds(pk, lds)) return rc;
         {
             scirs::ProfScope ps(SCIRS_PROF_PANEL, (double)mrows * kbw * kbw, 2.0 * mrows * kbw * sizeof(T), stream);
-            if (lookahead || G == 1) {
+            if (G == 1) {
+                // one workgroup: nothing to exchange, 512 threads share the rows
+                constexpr int NTS = 2 * panel::NT;
+                if (int rc = scirs_allow_lds((const void *)panel::getf2_panel_kernel<T, false, NTS>, lds)) return rc;
+                hipLaunchKernelGGL((panel::getf2_panel_kernel<T, false, NTS>), dim3(1), dim3(NTS), lds, stream, pa);
+                SCIRS_HIP_CHECK(hipGetLastError());
+            } else if (lookahead) {
                 // a normal launch, so the panel can share the device with the side
                 // stream's update: its G <= #CUs workgroups all become resident once
-                // that update's workgroups retire (they wait for nothing).  One
-                // workgroup exchanges nothing and needs no co-residency either.
+                // that update's workgroups retire (they wait for nothing)
                 if (w0)
                     hipLaunchKernelGGL((panel::getf2_panel_kernel<T, true>), dim3(G), dim3(panel::NT), lds, stream, pa);
                 else

@@ -154,8 +154,10 @@ __device__ __forceinline__ void bulk_update(T *tile, int kbp, int kb, const T *l
 // W0 selects where the column's critical path runs: false = all four waves
 // (five barriers per column), true = wave 0 alone (rows strided over its 64
 // lanes; two barriers per column), waves 1-3 doing only the bulk update.
-template <typename T, bool W0>
-__global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
+// NTK: threads per workgroup (256; a single-workgroup panel, with no
+// exchange to overlap, uses 512)
+template <typename T, bool W0, int NTK = NT>
+__global__ __launch_bounds__(NTK) void getf2_panel_kernel(PanelArgs<T> a)
 {
     if (*a.info != 0) return;
     constexpr int W = Tag<T>::W;
@@ -169,8 +171,8 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
     int *top_label = reinterpret_cast<int *>(lcol + a.R);   // kb (workgroup 0)
     int *out_pos = top_label + kb;                    // kb
     int *out_label = out_pos + kb;                    // kb
-    __shared__ T red_v[NT / 64];
-    __shared__ int red_i[NT / 64];
+    __shared__ T red_v[NTK / 64];
+    __shared__ int red_i[NTK / 64];
     __shared__ int s_p, s_g, s_stop, s_c, s_nout;
     __shared__ T s_pv, s_cv;
 
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
 
     g2s_block<T, 16>(tile, kbp, Ap + (long)row0 * a.lda, a.lda, Rg, kb, Rg, kb);   // 16 loads in flight per thread
     if (g == 0) {
-        for (int q = t; q < kb; q += NT) top_label[q] = q;
+        for (int q = t; q < kb; q += NTK) top_label[q] = q;
         if (t == 0) s_nout = 0;
     }
     if (t == 0) s_stop = 0;
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
     // local arg-max of column `col` over local rows >= rstart -> s_c (local row, -1 none), s_cv
     auto local_candidate = [&](int col, int rstart) {
         T bv = (T)0; int bi = -1;
-        for (int r = rstart + t; r < Rg; r += NT) {
+        for (int r = rstart + t; r < Rg; r += NTK) {
             const T v = fabs(tile[r * kbp + col]);
             if (bi < 0 || v > bv) { bv = v; bi = r; }
         }
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         __syncthreads();
         if (t == 0) {
             T v = red_v[0]; int i = red_i[0];
-            for (int w = 1; w < NT / 64; ++w) {
+            for (int w = 1; w < NTK / 64; ++w) {
                 const int oi = red_i[w]; const T ov = red_v[w];
                 if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) { v = ov; i = oi; }
             }
@@ -224,10 +226,10 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         for (int r = 0; r < nrep; ++r) {
             u64 *xr = XR(b, g, r);
             if (c >= 0)
-                for (int q = t; q < kb; q += NT) Tag<T>::put(xr + (size_t)q * W, seq, tile[c * kbp + q]);
+                for (int q = t; q < kb; q += NTK) Tag<T>::put(xr + (size_t)q * W, seq, tile[c * kbp + q]);
             if (j >= row0 && j < row0 + Rg) {
                 u64 *xj = XJ(b, r);
-                for (int q = t; q < kb; q += NT) Tag<T>::put(xj + (size_t)q * W, seq, tile[(j - row0) * kbp + q]);
+                for (int q = t; q < kb; q += NTK) Tag<T>::put(xj + (size_t)q * W, seq, tile[(j - row0) * kbp + q]);
             }
             if (t == 0) {
                 u64 *xw = XW(b, g, r);
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         } else if (j + 2 < kb) {
             // ---- 4. bulk rank-1 update of the other rows (waves 1-3), overlapping
             //         wave 0's publication and its next poll
-            bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t - 64, NT - 64);
+            bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t - 64, NTK - 64);
         }
         } else {
         const int p = solo ? nc : s_p;
@@ -443,16 +445,16 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         // ---- 3. row interchange inside the panel (the permutation record is kept
         //         later, off the critical path)
         if (solo && p != j) {
-            for (int c = t; c < kb; c += NT) {
+            for (int c = t; c < kb; c += NTK) {
                 const T v = tile[j * kbp + c];
                 tile[j * kbp + c] = tile[p * kbp + c];
                 tile[p * kbp + c] = v;
             }
         } else if (p != j) {
             if (own_j)
-                for (int c = t; c < kb; c += NT) tile[(j - row0) * kbp + c] = prow[c];
+                for (int c = t; c < kb; c += NTK) tile[(j - row0) * kbp + c] = prow[c];
             if (own_p)
-                for (int c = t; c < kb; c += NT) tile[(p - row0) * kbp + c] = rjv[c];
+                for (int c = t; c < kb; c += NTK) tile[(p - row0) * kbp + c] = rjv[c];
         }
         // workgroup 0 records row interchanges as labels (wave 0)
         auto record_swap = [&]() {
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         const bool more = j + 1 < kb;
         const T u1 = more ? prow[j + 1] : (T)0;
         T bv = (T)0; int bi = -1;
-        for (int r = r1 + t; r < Rg; r += NT) {
+        for (int r = r1 + t; r < Rg; r += NTK) {
             const T l = tile[r * kbp + j] / pivot;
             tile[r * kbp + j] = l;
             lcol[r] = l;
@@ -500,7 +502,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         // every thread combines the four wave results itself (lowest row wins ties)
         T cv = red_v[0]; int c = red_i[0];
 #pragma unroll
-        for (int w = 1; w < NT / 64; ++w) {
+        for (int w = 1; w < NTK / 64; ++w) {
             const int oi = red_i[w]; const T ov = red_v[w];
             if (oi >= 0 && (c < 0 || ov > cv || (ov == cv && oi < c))) { cv = ov; c = oi; }
         }
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         //         word's tag, so no barrier is needed before publishing)
         const int jn = (j + 1 >= row0 && j + 1 < row0 + Rg) ? j + 1 - row0 : -1;
         if (solo) {
-            for (int q = j + 2 + t; q < kb; q += NT) {
+            for (int q = j + 2 + t; q < kb; q += NTK) {
                 if (c >= 0) tile[c * kbp + q] -= lcol[c] * prow[q];
                 if (jn >= 0 && jn != c) tile[jn * kbp + q] -= lcol[jn] * prow[q];
             }
@@ -518,7 +520,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         } else {
             const int bb = (j + 1) & 1;
             const unsigned sq = a.seq_base + (unsigned)j + 2u;
-            for (int q = t; q < kb; q += NT) {
+            for (int q = t; q < kb; q += NTK) {
                 if (c >= 0) {
                     T v = tile[c * kbp + q];
                     if (q >= j + 2) { v -= lcol[c] * prow[q]; tile[c * kbp + q] = v; }
@@ -545,15 +547,15 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(PanelArgs<T> a)
         //         orders this update before any use of the tile.
         //         (Single workgroup: no exchange to overlap, so wave 0 joins in.)
         if (solo) {
-            if (j + 2 < kb) bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t, NT);
+            if (j + 2 < kb) bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t, NTK);
         } else if (wave > 0 && j + 2 < kb) {
-            bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t - 64, NT - 64);
+            bulk_update<T>(tile, kbp, kb, lcol, prow, r1, Rg, j + 2, c, jn, t - 64, NTK - 64);
         }
     }
         }
     __syncthreads();
 
-    for (int r = wave; r < Rg; r += NT / 64)
+    for (int r = wave; r < Rg; r += NTK / 64)
         for (int c = lane; c < kb; c += 64) Ap[(long)(row0 + r) * a.lda + c] = tile[r * kbp + c];
 
     if (g == 0) {
