@@ -12,6 +12,7 @@
 //   inv, det         scirs2-linalg/src/basic.rs:117-186, 29-93
 //   mixed_precision_solve, iterative_refinement_solve
 //                    scirs2-linalg/src/mixed_precision/adaptive.rs:49-151, 282-378
+//   compat_solve     scirs2-linalg/src/compat.rs:461-499
 //   validation       scirs2-linalg/src/validation.rs:22-329
 //   LinalgError      scirs2-linalg/src/error.rs:8-150
 //
@@ -655,6 +656,51 @@ Result<Array1<C>> iterative_refinement_solve(const ArrayView2<A> &a, const Array
     }
     for (std::size_t i = 0; i < n; ++i) out[i] = static_cast<C>(xh[i]);
     return out;
+}
+
+// ------------------------------------------------------------------ compat
+// compat.rs:461-499 (scipy.linalg.solve).  The reference rejects every
+// specialised option with NotImplementedError; here assume_a = "pos" (without
+// `lower` / `transposed`) is served by the GPU Cholesky solve, the other
+// options keep the reference's error.  overwrite_a / overwrite_b are ignored, as
+// in the reference.
+template <class F>
+Result<Array2<F>> compat_solve(const ArrayView2<F> &a, const ArrayView2<F> &b, bool lower, bool overwrite_a,
+                               bool overwrite_b, bool check_finite, std::optional<std::string> assume_a,
+                               bool transposed)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    (void)overwrite_a;
+    (void)overwrite_b;
+    if (check_finite) {
+        for (std::size_t i = 0; i < a.rows; ++i)
+            for (std::size_t j = 0; j < a.cols; ++j)
+                if (!std::isfinite(a(i, j)))
+                    return LinalgError(LinalgError::Kind::ValueError, "Matrix contains non-finite values");
+        for (std::size_t i = 0; i < b.rows; ++i)
+            for (std::size_t j = 0; j < b.cols; ++j)
+                if (!std::isfinite(b(i, j)))
+                    return LinalgError(LinalgError::Kind::ValueError, "Matrix contains non-finite values");
+    }
+    const bool pos = assume_a && *assume_a == "pos" && !lower && !transposed;
+    if (!pos && (lower || assume_a || transposed))
+        return LinalgError(LinalgError::Kind::NotImplementedError, "Specialized solve options not yet supported");
+    if (!pos) return solve_multiple(a, b, std::nullopt);
+    if (auto e = validate_multiple_linear_systems(a, b, "Multiple linear systems solve")) return *e;
+    const std::size_t n = a.rows, nrhs = b.cols;
+    std::vector<double> ad(n * n), bd(n * nrhs), xd(n * nrhs);
+    for (std::size_t i = 0; i < n; ++i) {
+        for (std::size_t j = 0; j < n; ++j) ad[i * n + j] = static_cast<double>(a(i, j));
+        for (std::size_t j = 0; j < nrhs; ++j) bd[i * nrhs + j] = static_cast<double>(b(i, j));
+    }
+    scirs_solve_info info;
+    int rc = scirs_gpu_dposv((int)n, (int)nrhs, ad.data(), (int64_t)n, bd.data(), (int64_t)nrhs, xd.data(),
+                             (int64_t)nrhs, &info);
+    if (rc > 0) return non_positive_definite_with_suggestions("Cholesky decomposition", a.dim(), std::nullopt);
+    if (rc < 0) return detail::gpu_failure(rc);
+    Array2<F> x(n, nrhs);
+    for (std::size_t i = 0; i < n * nrhs; ++i) x.data[i] = static_cast<F>(xd[i]);
+    return x;
 }
 
 } // namespace scirs2_linalg

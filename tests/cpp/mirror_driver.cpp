@@ -4,7 +4,7 @@
 // Input, whitespace separated, one case after another:
 //   <op> <type> <flag1> <flag2> <ar> <ac> <ar*ac values> <br> <bc> <br*bc values>
 //   op   : solve | solve_multiple | lu | cholesky | solve_triangular | det | inv |
-//          mixed_precision_solve | iterative_refinement_solve
+//          mixed_precision_solve | iterative_refinement_solve | compat_solve
 //   type : f64 | f32
 //   flags: lower / unit_diagonal for solve_triangular, ignored otherwise
 //   b    : the right-hand side (br = length, bc = 1 for a vector; 0 0 when unused)
@@ -30,6 +30,8 @@ static const char *kind_name(LinalgError::Kind k)
     case LinalgError::Kind::SingularMatrixError: return "SingularMatrixError";
     case LinalgError::Kind::NonPositiveDefiniteError: return "NonPositiveDefiniteError";
     case LinalgError::Kind::ComputationError: return "ComputationError";
+    case LinalgError::Kind::ValueError: return "ValueError";
+    case LinalgError::Kind::NotImplementedError: return "NotImplementedError";
     default: return "Other";
     }
 }
@@ -121,6 +123,15 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
         if (r.is_err()) return print_err(r.error());
         std::printf("OK 1");
         put(r.value().size(), 1, r.value().data());
+    } else if (op == "compat_solve") {
+        // f1 bits: 1 lower, 2 transposed, 4 check_finite; f2: 0 no assume_a, 1 "pos", 2 "sym"
+        std::optional<std::string> assume;
+        if (f2 == 1) assume = "pos";
+        else if (f2 == 2) assume = "sym";
+        auto r = compat_solve<F>(a.view(), b.view(), (f1 & 1) != 0, false, false, (f1 & 4) != 0, assume, (f1 & 2) != 0);
+        if (r.is_err()) return print_err(r.error());
+        std::printf("OK 1");
+        put(r.value().rows, r.value().cols, r.value().data.data());
     } else {
         std::printf("ERR Other unknown op %s\n", op.c_str());
         return;

@@ -383,3 +383,52 @@ def test_mixed_precision_dropins_match_oracle():
     assert np.abs(got[4][1][0].ravel() - np.linalg.solve(a3, b3)).max() <= 1e-12
     assert got[5] == ("ERR", "SingularMatrixError", "Singular matrix error: Matrix is singular")
     assert got[6] == ("ERR", "SingularMatrixError", singular("linear system solve", 8, 8))
+
+
+def test_compat_solve_options_and_texts():
+    """compat_solve (compat.rs:461-499): the finiteness scan and the options the
+    reference does not support keep its errors (no device work)."""
+    a = np.eye(3)
+    b = np.ones((3, 1))
+    bad = a.copy()
+    bad[1, 2] = np.nan
+    got = run([
+        ("compat_solve", "f64", 4, 0, bad, b),         # check_finite
+        ("compat_solve", "f64", 1, 0, a, b),           # lower
+        ("compat_solve", "f64", 2, 0, a, b),           # transposed
+        ("compat_solve", "f64", 0, 2, a, b),           # assume_a = "sym"
+        ("compat_solve", "f64", 1, 1, a, b),           # "pos" with lower
+    ])
+    assert got[0] == ("ERR", "ValueError", "Value error: Matrix contains non-finite values")
+    for g in got[1:]:
+        assert g == ("ERR", "NotImplementedError", "Not implemented: Specialized solve options not yet supported")
+
+
+@pytest.mark.gpu
+def test_compat_solve_general_and_pos():
+    """compat_solve with no options is solve_multiple; assume_a = "pos" runs the
+    GPU Cholesky solve (LAPACK's solution within 8 n u kappa) and reports a
+    non-positive-definite matrix as the Cholesky decomposition does."""
+    rng = np.random.default_rng(91)
+    n, nrhs = 150, 3
+    g = rng.standard_normal((n, n))
+    s = g @ g.T / n + np.eye(n)
+    a = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, nrhs))
+    npd = s.copy()
+    npd[7, 7] = -1.0
+    got = run([
+        ("compat_solve", "f64", 4, 0, a, b),
+        ("compat_solve", "f64", 4, 1, s, b),
+        ("compat_solve", "f32", 0, 1, s, b),
+        ("compat_solve", "f64", 0, 1, npd, b),
+    ])
+    for i, m in ((0, a), (1, s)):
+        assert got[i][0] == "OK", got[i]
+        x = got[i][1][0]
+        xo = np.linalg.solve(m, b)
+        kappa = np.linalg.cond(m, np.inf)
+        assert np.abs(x - xo).max() <= 8 * n * U64 * kappa * np.abs(xo).max(), i
+    x32 = got[2][1][0]
+    assert np.abs(x32 - np.linalg.solve(s, b)).max() <= 1e-5
+    assert got[3] == ("ERR", "NonPositiveDefiniteError", not_pd("Cholesky decomposition", n, n))
