@@ -292,7 +292,7 @@ int scirs::Context::ensure_xtri(int n)
 // solver, SCIRS_GPU_TRSV=blocked the 64-row one.
 template <typename T>
 int scirs::lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s, const T *dinv_l,
-                         const T *dinv_u)
+                         const T *dinv_u, const T *dmt_l, const T *dmt_u)
 {
     Context &c = ctx();
     const char *mode = getenv("SCIRS_GPU_TRSV");
@@ -305,14 +305,14 @@ int scirs::lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long l
         return scirs_trsv_big<T>(n, nrhs, dLU, ldlu, dX, ldx, 0, 0, c.work.info, s);
     }
     if ((rc = c.ensure_xtri(n))) return rc;
-    rc = scirs_trsv_chain<T>(n, nrhs, dLU, ldlu, dX, ldx, 1, 1, c.work, s, dinv_l);
+    rc = scirs_trsv_chain<T>(n, nrhs, dLU, ldlu, dX, ldx, 1, 1, c.work, s, dinv_l, dmt_l);
     if (rc) return rc;
-    return scirs_trsv_chain<T>(n, nrhs, dLU, ldlu, dX, ldx, 0, 0, c.work, s, dinv_u);
+    return scirs_trsv_chain<T>(n, nrhs, dLU, ldlu, dX, ldx, 0, 0, c.work, s, dinv_u, dmt_u);
 }
 template int scirs::lu_substitute<float>(int, int, const float *, long, float *, long, hipStream_t, const float *,
-                                         const float *);
+                                         const float *, const float *, const float *);
 template int scirs::lu_substitute<double>(int, int, const double *, long, double *, long, hipStream_t,
-                                          const double *, const double *);
+                                          const double *, const double *, const double *, const double *);
 
 template <typename T>
 static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm, const T *dB, long ldb, T *dX,

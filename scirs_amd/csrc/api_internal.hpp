@@ -48,10 +48,12 @@ struct Context {
 };
 
 // X <- U^-1 L^-1 X with the packed LU factors (X already row-permuted);
-// dinv_l / dinv_u (optional): inverses of the 64 x 64 diagonal blocks of L and U.
+// dinv_l / dinv_u (optional): inverses of the 64 x 64 diagonal blocks of L and U;
+// dmt_l / dmt_u (optional, with them): those inverses times each block row's last tile.
 template <typename T>
 int lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hipStream_t s,
-                  const T *dinv_l = nullptr, const T *dinv_u = nullptr);
+                  const T *dinv_l = nullptr, const T *dinv_u = nullptr, const T *dmt_l = nullptr,
+                  const T *dmt_u = nullptr);
 
 Context &ctx();
 double now_ms();

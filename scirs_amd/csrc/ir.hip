@@ -155,12 +155,14 @@ struct IrState {
     std::vector<double> normB;
     int kb;
     const float *dinv_l = nullptr, *dinv_u = nullptr;   // inverted 64 x 64 diagonal blocks of L and U
+    const float *dmt_l = nullptr, *dmt_u = nullptr;     // those times each block row's last tile
 };
 
 // d32 <- U^-1 L^-1 d32 with the FP32 factors
 int lu_solve_f32(IrState &st, float *d32)
 {
-    return scirs::lu_substitute<float>(st.n, 1, st.LU, st.n, d32, 1, st.s, st.dinv_l, st.dinv_u);
+    return scirs::lu_substitute<float>(st.n, 1, st.LU, st.n, d32, 1, st.s, st.dinv_l, st.dinv_u, st.dmt_l,
+                                       st.dmt_u);
 }
 
 // apply z = (LU)^-1 P v in FP32 for one vector v (FP64), result FP64; scale keeps FP32 in range
@@ -348,7 +350,7 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     const size_t dinv_elems = (size_t)((n + 63) / 64) * 64 * 64;
     const size_t other = ((size_t)n * nrhs + (size_t)n + 256 + (size_t)n + 2 * (size_t)(restart + 1) * n + n + restart + 8) *
                          sizeof(double) + (size_t)n * sizeof(float) + (size_t)n * sizeof(int64_t) +
-                         2 * dinv_elems * sizeof(float) + 4096;
+                         4 * dinv_elems * sizeof(float) + 4096;
     if ((rc = c.ensure(&c.scratch_ir, &c.scratch_ir_bytes, a32))) return rc;
     if ((rc = c.ensure(&c.scratch_ir2, &c.scratch_ir2_bytes, other))) return rc;
     IrState st;
@@ -367,6 +369,8 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     double *hdev = (double *)take((size_t)(restart + 8) * sizeof(double));
     float *dinv_l = (float *)take(dinv_elems * sizeof(float));
     float *dinv_u = (float *)take(dinv_elems * sizeof(float));
+    float *dmt_l = (float *)take(dinv_elems * sizeof(float));
+    float *dmt_u = (float *)take(dinv_elems * sizeof(float));
     st.kb = c.lu_kb<float>(n);
 
     hipEvent_t e0, e1, e2;
@@ -396,8 +400,12 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     if (c.trsv_dinv) {
         if ((rc = scirs_tri_block_inverse(n, st.LU, n, 1, 1, dinv_l, s))) return rc;
         if ((rc = scirs_tri_block_inverse(n, st.LU, n, 0, 0, dinv_u, s))) return rc;
+        if ((rc = scirs_tri_block_coupling(n, st.LU, n, 1, dinv_l, dmt_l, s))) return rc;
+        if ((rc = scirs_tri_block_coupling(n, st.LU, n, 0, dinv_u, dmt_u, s))) return rc;
         st.dinv_l = dinv_l;
         st.dinv_u = dinv_u;
+        st.dmt_l = dmt_l;
+        st.dmt_u = dmt_u;
     }
     SCIRS_HIP_CHECK(hipEventRecord(e1, s));
     int lu_info = 0;

@@ -96,9 +96,14 @@ int scirs_trsv_big(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lo
 // w.xtri, which must hold n * 4 * 2 words): X <- op(A)^-1 X as scirs_trsv_big.
 // dinv (optional): inverses of the 64 x 64 diagonal blocks (scirs_tri_block_inverse),
 // applied instead of the substitution inside each block
+// dmt (optional, with dinv): Dinv[p] times the tile block row p meets last
+// (scirs_tri_block_coupling)
 template <typename T>
 int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
-                     hipStream_t s, const T *dinv = nullptr);
+                     hipStream_t s, const T *dinv = nullptr, const T *dmt = nullptr);
+
+// M[p] = Dinv[p] * (block row p's last tile in the chained solve's order), ceil(n/64) * 64 * 64 floats
+int scirs_tri_block_coupling(int n, const float *A, long lda, int lower, const float *dinv, float *dmt, hipStream_t s);
 
 // Dinv[p] = (T_pp)^-1 for the 64 x 64 diagonal blocks of the lower (unit or
 // not) or upper triangle of the FP32 matrix A; dinv holds ceil(n/64) * 64 * 64 floats

@@ -88,7 +88,7 @@ template <typename T, int QB, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A, long lda, T *X, long ldx, int n,
                                                          int nrhs, u64 *xch, unsigned seq, int *ticket, int *abort,
                                                          unsigned spin_limit, const int *info,
-                                                         const T *__restrict__ dinv)
+                                                         const T *__restrict__ dinv, const T *__restrict__ dmt)
 {
     if (info && *info != 0) return;
     // blockIdx.y: a group of QB right-hand sides, an independent chain with its
@@ -170,8 +170,13 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         if (b > 0) {
             const int cb = LOWER ? b - 1 : nblk - b;
             const int c0 = cb * TB, cs = min(TB, n - c0);
-            for (int i = 0; i < TB; ++i)
-                Lt[i * LDP + lane] = (i < bs && lane < cs) ? A[(long)(r0 + i) * lda + c0 + lane] : (T)0;
+            if (dmt) {
+                const T *Mp = dmt + (size_t)p * TB * TB;   // D^-1 times the last tile
+                for (int i = 0; i < TB; ++i) Lt[i * LDP + lane] = Mp[i * TB + lane];
+            } else {
+                for (int i = 0; i < TB; ++i)
+                    Lt[i * LDP + lane] = (i < bs && lane < cs) ? A[(long)(r0 + i) * lda + c0 + lane] : (T)0;
+            }
         }
     }
     __syncthreads();
@@ -184,6 +189,34 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         s[q] = (lane < bs && q < nr) ? X[(long)(r0 + lane) * ldx + q] : (T)0;
         if (b > 1) s[q] -= part[lane * QB + q] + part[(TB + lane) * QB + q] + part[(2 * TB + lane) * QB + q];
     }
+    if (dinv && dmt) {
+        // x = D^-1 s - (D^-1 T) x_prev: the first product before the last
+        // dependency's x arrives, only the second one after it
+        T xo[QB];
+#pragma unroll
+        for (int q = 0; q < QB; ++q) xo[q] = (T)0;
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            const T d = Ld[lane * LDP + k];
+#pragma unroll
+            for (int q = 0; q < QB; ++q) xo[q] += d * readlane(s[q], k);
+        }
+        if (b > 0) {
+            const int cb = LOWER ? b - 1 : nblk - b;
+            const int c0 = cb * TB;
+            T xk[QB];
+            if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort))
+                return;
+#pragma unroll
+            for (int k = 0; k < TB; ++k) {
+                const T m = Lt[lane * LDP + k];
+#pragma unroll
+                for (int q = 0; q < QB; ++q) xo[q] -= m * readlane(xk[q], k);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) s[q] = xo[q];
+    } else {
     if (b > 0) {
         const int cb = LOWER ? b - 1 : nblk - b;
         const int c0 = cb * TB;
@@ -253,6 +286,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         }
     }
     }
+    }
     if (lane < bs) {
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
@@ -263,7 +297,8 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
 }
 
 template <typename T, int QB, bool LOWER, bool UNIT>
-int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork &w, hipStream_t s, const T *dinv)
+int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork &w, hipStream_t s, const T *dinv,
+                 const T *dmt)
 {
     const int groups = (nr + QB - 1) / QB;
     const auto kern = trsv_chain_kernel<T, QB, LOWER, UNIT>;
@@ -278,26 +313,26 @@ int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork 
     w.seq_next += 512u;
     const int nblk = (n + TB - 1) / TB;
     hipLaunchKernelGGL(kern, dim3(nblk, groups), dim3(NTC), lds, s, A, lda, X, ldx, n, nr, w.xtri, seq, w.ticket,
-                       w.abort, w.spin_limit, w.info, dinv);
+                       w.abort, w.spin_limit, w.info, dinv, dmt);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }
 
 template <typename T, int QB>
 int chain_dispatch(int n, int nr, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
-                   hipStream_t s, const T *dinv)
+                   hipStream_t s, const T *dinv, const T *dmt)
 {
-    if (lower && unit) return launch_chain<T, QB, true, true>(n, nr, A, lda, X, ldx, w, s, dinv);
-    if (lower) return launch_chain<T, QB, true, false>(n, nr, A, lda, X, ldx, w, s, dinv);
-    if (unit) return launch_chain<T, QB, false, true>(n, nr, A, lda, X, ldx, w, s, dinv);
-    return launch_chain<T, QB, false, false>(n, nr, A, lda, X, ldx, w, s, dinv);
+    if (lower && unit) return launch_chain<T, QB, true, true>(n, nr, A, lda, X, ldx, w, s, dinv, dmt);
+    if (lower) return launch_chain<T, QB, true, false>(n, nr, A, lda, X, ldx, w, s, dinv, dmt);
+    if (unit) return launch_chain<T, QB, false, true>(n, nr, A, lda, X, ldx, w, s, dinv, dmt);
+    return launch_chain<T, QB, false, false>(n, nr, A, lda, X, ldx, w, s, dinv, dmt);
 }
 
 } // namespace
 
 template <typename T>
 int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
-                     hipStream_t s, const T *dinv)
+                     hipStream_t s, const T *dinv, const T *dmt)
 {
     if (n <= 0 || nrhs <= 0) return SCIRS_OK;
     // right-hand sides per chain; up to SCIRS_CHAIN_GROUPS chains run side by side in one launch
@@ -315,9 +350,9 @@ int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int 
     for (int rb = 0; rb < nrhs; rb += per_launch) {
         const int nr = std::min(per_launch, nrhs - rb);
         int rc;
-        if (qb == 1) rc = chain_dispatch<T, 1>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv);
-        else if (qb == 2) rc = chain_dispatch<T, 2>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv);
-        else rc = chain_dispatch<T, QMAX>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv);
+        if (qb == 1) rc = chain_dispatch<T, 1>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv, dmt);
+        else if (qb == 2) rc = chain_dispatch<T, 2>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv, dmt);
+        else rc = chain_dispatch<T, QMAX>(n, nr, A, lda, X + rb, ldx, lower, unit, w, s, dinv, dmt);
         if (rc) return rc;
     }
     return SCIRS_OK;
@@ -375,6 +410,48 @@ int scirs_tri_block_inverse(int n, const float *A, long lda, int lower, int unit
     return SCIRS_OK;
 }
 
+// M[p] = Dinv[p] * T_p for the tile T_p that block row p meets last in the
+// chained solve (block column p-1 of a lower triangle, p+1 of an upper one),
+// so a link applies D^-1 to its partial sums before that dependency's x
+// arrives and only M[p] x after.  One workgroup per block row.
+namespace {
+template <bool LOWER>
+__global__ __launch_bounds__(256) void tri_block_coupling_kernel(int n, const float *__restrict__ A, long lda,
+                                                                 const float *__restrict__ dinv,
+                                                                 float *__restrict__ dmt)
+{
+    __shared__ float Dl[TB][TB + 1], Tl[TB][TB + 1];
+    const int p = blockIdx.x, nblk = (n + TB - 1) / TB;
+    const int cb = LOWER ? p - 1 : p + 1;
+    if (cb < 0 || cb >= nblk) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int r0 = p * TB, bs = min(TB, n - r0), c0 = cb * TB, cs = min(TB, n - c0);
+    const float *Dp = dinv + (size_t)p * TB * TB;
+    for (int i = wave; i < TB; i += 4) {
+        Dl[i][lane] = Dp[i * TB + lane];
+        Tl[i][lane] = (i < bs && lane < cs) ? A[(long)(r0 + i) * lda + c0 + lane] : 0.0f;
+    }
+    __syncthreads();
+    float *Mp = dmt + (size_t)p * TB * TB;
+    for (int i = wave; i < TB; i += 4) {
+        float acc = 0.0f;
+#pragma unroll 16
+        for (int k = 0; k < TB; ++k) acc += Dl[i][k] * Tl[k][lane];
+        Mp[i * TB + lane] = acc;
+    }
+}
+} // namespace
+
+int scirs_tri_block_coupling(int n, const float *A, long lda, int lower, const float *dinv, float *dmt, hipStream_t s)
+{
+    if (n <= 0) return SCIRS_OK;
+    const int nblk = (n + TB - 1) / TB;
+    if (lower) hipLaunchKernelGGL(tri_block_coupling_kernel<true>, dim3(nblk), dim3(256), 0, s, n, A, lda, dinv, dmt);
+    else hipLaunchKernelGGL(tri_block_coupling_kernel<false>, dim3(nblk), dim3(256), 0, s, n, A, lda, dinv, dmt);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+
 // U = L^T for the lower triangle of L (64 x 64 tiles through LDS; only the tiles
 // on or above U's diagonal are written).  Turns the Cholesky factor into the
 // row-major upper triangle the chained solve streams by rows.
@@ -418,6 +495,6 @@ template int scirs_transpose_lower<double>(int, const double *, long, double *, 
 template int scirs_transpose_lower<float>(int, const float *, long, float *, long, hipStream_t);
 
 template int scirs_trsv_chain<float>(int, int, const float *, long, float *, long, int, int, ScirsWork &, hipStream_t,
-                                     const float *);
+                                     const float *, const float *);
 template int scirs_trsv_chain<double>(int, int, const double *, long, double *, long, int, int, ScirsWork &,
-                                      hipStream_t, const double *);
+                                      hipStream_t, const double *, const double *);
