@@ -16,13 +16,18 @@
 // when exceeded, so no wave can wait forever.
 //
 // Latency hiding: (1) every poll of the candidate list also reads the row of
-// the best candidate of the previous poll (in the same round trip), so when
-// the last candidate arrives the pivot row is usually already in registers —
-// one memory round trip per column instead of two; (2) right after the pivot row arrives, a workgroup computes
-// the multipliers, updates column j+1 and finds its next candidate in one
-// pass, brings that row (and row j+1 when it holds it) fully up to date and
+// the best candidate of the previous poll (in the same round trip, and kept
+// once complete), so when the last candidate arrives the pivot row is usually
+// already in registers — one memory round trip per column instead of two;
+// (2) right after the pivot row arrives, a workgroup computes the
+// multipliers, updates column j+1 and finds its next candidate in one pass,
+// brings that row (and row j+1 when it holds it) fully up to date and
 // publishes; (3) the bulk rank-1 update of its other rows runs on waves 1-3
 // while wave 0 already polls the next column.
+//
+// A panel whose rows fit one workgroup's LDS (small matrices, the last panels)
+// runs as a single 512-thread workgroup: its local arg-max is the global one,
+// so nothing is published or polled, and the pivot row is swapped in place.
 #pragma once
 #include "common.hpp"
 
