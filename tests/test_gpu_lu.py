@@ -170,6 +170,26 @@ def test_tie_rule_first_max(L):
     assert np.array_equal(perm, piv_o)
 
 
+@pytest.mark.parametrize("n,dtype", [(1024, np.float64), (2000, np.float64), (1536, np.float32)])
+def test_getrf_exchange_path_matches_oracle(L, n, dtype):
+    """Sizes whose panels span several workgroups (the tagged candidate
+    exchange, the fused next-block kernel and the look-ahead streams; the
+    golden sizes above all fit one workgroup per panel): the pivot sequence
+    equals the oracle's and P A = L U to rounding."""
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n)).astype(dtype)
+    lu, perm, _ = L.getrf(a)
+    _, piv_o, _ = oracle.lu_factor(a)
+    assert np.array_equal(perm, piv_o), "pivot sequence differs from the oracle"
+    u_ = U64 if dtype == np.float64 else U32
+    a64 = a.astype(np.float64)
+    lu = lu.astype(np.float64)
+    l = np.tril(lu, -1) + np.eye(n)
+    u = np.triu(lu)
+    res = np.abs(a64[perm] - l @ u).max() / np.abs(a64).max()
+    assert res <= 8 * n * u_
+
+
 def test_sgetrf_sgesv(L):
     d = gold(512)
     a32 = d["a"].astype(np.float32)
