@@ -160,9 +160,13 @@ int Context::init(int dev)
     work.seq_next = 0;
     work_base = base;
     SCIRS_HIP_CHECK(hipHostMalloc((void **)&host_flags, 64, hipHostMallocDefault));
-    kb_f64 = env_int("SCIRS_GPU_KB", 128);
-    kb_f32 = env_int("SCIRS_GPU_KB_F32", 128);
-    kb_chol = env_int("SCIRS_GPU_KB_CHOL", 128);
+    // panel widths: 8 .. 128 (the LDS budgets of the panel's neighbours — the
+    // next-block interchange + TRSM kernel, the TRSM and Cholesky kernels — are
+    // sized for kb <= 128)
+    auto kb_env = [](const char *name) { return std::max(8, std::min(128, env_int(name, 128))); };
+    kb_f64 = kb_env("SCIRS_GPU_KB");
+    kb_f32 = kb_env("SCIRS_GPU_KB_F32");
+    kb_chol = kb_env("SCIRS_GPU_KB_CHOL");
     narrow_kb = env_int("SCIRS_GPU_NARROW_KB", 1);
     chol_chain = env_int("SCIRS_GPU_CHOL_CHAIN", 1);
     trsv_dinv = env_int("SCIRS_GPU_TRSV_DINV", 1);
