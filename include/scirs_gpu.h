@@ -107,6 +107,23 @@ int scirs_gpu_dposv(int n, int nrhs, const double *a, int64_t lda, const double 
 int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
                     double *x, int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info);
 
+/* ---- QR and least squares --------------------------------------------------- */
+/* Replaces scirs2-linalg/src/lapack.rs:171-253 `qr_factor` (behind
+ * decomposition.rs:208 `qr`) for f64: a is rows x cols with rows >= cols >= 1;
+ * q (rows x rows) and r (rows x cols) receive the reference's Householder
+ * factors (unit reflectors, alpha = -sign(x0) ||x||, columns with ||x|| <= eps
+ * skipped; r's sub-diagonal holds what the reflections leave, as in the
+ * reference). */
+int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, double *q, int64_t ldq, double *r,
+                  int64_t ldr, scirs_solve_info *info);
+
+/* Replaces the rows >= cols branch of scirs2-linalg/src/solve.rs:234-290
+ * `lstsq` for f64: x (cols) minimises ||b - A x||_2 through R x = (Q^T b)[..cols],
+ * *residual = ||b - A x||_2^2.  Returns k+1 when the back substitution meets
+ * |r_kk| < eps (solve.rs:184-191), bottom row first. */
+int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda, const double *b, double *x,
+                     double *residual, scirs_solve_info *info);
+
 /* ---- mixed precision ---------------------------------------------------------- */
 /* The factor-once counterpart of scirs2-linalg/src/mixed_precision/adaptive.rs:282-378
  * `iterative_refinement_solve::<f64,f64,f64,f64,f32>`: FP32 panel, low-precision

@@ -10,6 +10,8 @@ scirs2-linalg's CPU solve path:
 * ``solve``            -> scirs2-linalg/src/solve.rs:54-111 (+ basic.rs:117-186)
 * ``solve_multiple``   -> scirs2-linalg/src/solve.rs:366-424
 * ``lu``               -> scirs2-linalg/src/decomposition.rs:157-180 (P, L, U build)
+* ``qr_factor``        -> scirs2-linalg/src/lapack.rs:171-253 (f64)
+* ``lstsq``            -> scirs2-linalg/src/solve.rs:234-290, rows >= cols (f64)
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import this module; it is the checker, never the product.
@@ -59,6 +61,8 @@ def _load():
         getattr(lib, f"oracle_solve_triangular_{s}").argtypes = [I, P, L, P, P, I, I]
         getattr(lib, f"oracle_solve_{s}").argtypes = [I, P, L, P, P]
         getattr(lib, f"oracle_solve_multiple_{s}").argtypes = [I, I, P, L, P, L, P, L]
+    lib.oracle_qr_factor_d.argtypes = [I, I, P, L, P, P]
+    lib.oracle_lstsq_d.argtypes = [I, I, P, L, P, P, P]
     _lib = lib
     return lib
 
@@ -198,3 +202,33 @@ def cholesky_solve(a, b):
         y = solve_triangular(l, cols[:, j], True, False)
         out[:, j] = solve_triangular(lt, y, False, False)
     return out.reshape(b.shape)
+
+
+def qr_factor(a):
+    """(Q, R) of an f64 rows x cols matrix (rows >= cols) as lapack.rs:171-253."""
+    a = _c(a, np.float64)
+    rows, cols = a.shape
+    q = np.empty((rows, rows))
+    r = np.empty((rows, cols))
+    rc = _load().oracle_qr_factor_d(rows, cols, _ptr(a), cols, _ptr(q), _ptr(r))
+    if rc:
+        raise OracleError("empty" if rc == -1 else "shape", rc)
+    return q, r
+
+
+def lstsq(a, b):
+    """(x, residuals) of the rows >= cols branch of solve.rs:234-290 (f64).
+
+    Raises OracleError('singular', k) where the back substitution on R hits
+    |r_kk| < eps (1-based row k)."""
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    rows, cols = a.shape
+    x = np.empty(cols)
+    res = np.zeros(1)
+    rc = _load().oracle_lstsq_d(rows, cols, _ptr(a), cols, _ptr(b), _ptr(x), _ptr(res))
+    if rc > 0:
+        raise OracleError("singular", rc)
+    if rc < 0:
+        raise OracleError("empty" if rc == -1 else "shape", rc)
+    return x, float(res[0])

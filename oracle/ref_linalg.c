@@ -16,6 +16,8 @@
  *   oracle_solve_*            scirs2-linalg/src/solve.rs:54-111
  *                             (n <= 4 branch: basic.rs:117-186 inv, basic.rs:29-93 det)
  *   oracle_solve_multiple_*   scirs2-linalg/src/solve.rs:366-424
+ *   oracle_qr_factor_d        scirs2-linalg/src/lapack.rs:171-253
+ *   oracle_lstsq_d            scirs2-linalg/src/solve.rs:234-290 (rows >= cols)
  *
  * All matrices are dense row-major with an explicit leading dimension, the
  * layout of an ndarray in standard order.
@@ -224,6 +226,98 @@ int oracle_solve_##S(int n, const T *a, long lda, const T *b, T *x)             
 
 ORACLE_DEFINE(double, d, DBL_EPSILON, fabs, sqrt)
 ORACLE_DEFINE(float, s, FLT_EPSILON, fabsf, sqrtf)
+
+/* lapack.rs:171-253 (qr_factor), f64.  a is rows x cols (rows >= cols), q is
+ * rows x rows, r is rows x cols.  Householder reflectors with v normalised to
+ * unit length (H = I - 2 v v^T), alpha = -sign(x0)*||x|| (x0 >= 0 -> -||x||),
+ * a column skipped when ||x|| <= eps or ||v|| <= eps.  Q is accumulated as
+ * Q^T (reflectors applied to the identity from the left) and transposed at
+ * the end; sub-diagonal entries of R keep whatever the reflections leave.
+ * Returns -1 for an empty matrix, -2 for rows < cols. */
+int oracle_qr_factor_d(int rows, int cols, const double *a, long lda, double *q, double *r)
+{
+    if (rows == 0 || cols == 0) return -1;
+    if (rows < cols) return -2;
+    const int n = rows, m = cols;
+    double *qt = (double *)malloc(sizeof(double) * (size_t)n * (size_t)n);
+    double *v = (double *)malloc(sizeof(double) * (size_t)n);
+    double *col = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!qt || !v || !col) { free(qt); free(v); free(col); return -3; }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) r[(size_t)i * m + j] = a[(size_t)i * lda + j];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) qt[(size_t)i * n + j] = i == j ? 1.0 : 0.0;
+    for (int k = 0; k < (m < n ? m : n); ++k) {
+        const int len = n - k;
+        double xs = 0.0;
+        for (int i = 0; i < len; ++i) {
+            v[i] = r[(size_t)(k + i) * m + k];
+            xs += v[i] * v[i];
+        }
+        const double x_norm = sqrt(xs);
+        if (!(x_norm > DBL_EPSILON)) continue;
+        const double alpha = v[0] >= 0.0 ? -x_norm : x_norm;
+        v[0] -= alpha;
+        double vs = 0.0;
+        for (int i = 0; i < len; ++i) vs += v[i] * v[i];
+        const double v_norm = sqrt(vs);
+        if (!(v_norm > DBL_EPSILON)) continue;
+        for (int i = 0; i < len; ++i) v[i] /= v_norm;
+        for (int j = k; j < m; ++j) {
+            double dot = 0.0;
+            for (int i = 0; i < len; ++i) dot = dot + v[i] * r[(size_t)(k + i) * m + j];
+            for (int i = 0; i < len; ++i) r[(size_t)(k + i) * m + j] -= 2.0 * v[i] * dot;
+        }
+        for (int j = 0; j < n; ++j) {
+            double dot = 0.0;
+            for (int i = 0; i < len; ++i) col[i] = qt[(size_t)(k + i) * n + j];
+            for (int i = 0; i < len; ++i) dot = dot + v[i] * col[i];
+            for (int i = 0; i < len; ++i) qt[(size_t)(k + i) * n + j] -= 2.0 * v[i] * dot;
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) q[(size_t)i * n + j] = qt[(size_t)j * n + i];
+    free(qt); free(v); free(col);
+    return 0;
+}
+
+/* solve.rs:234-290, the rows >= cols branch of lstsq, f64: (Q, R) = qr(A),
+ * Q^T b with Q^T formed explicitly and summed in order, back substitution on
+ * the leading cols x cols block of R (solve.rs:139-202, its |r_ii| < eps test
+ * returning the 1-based row), then the residual sum of squares ||b - A x||^2
+ * accumulated row by row.  Returns what qr_factor or the substitution returns. */
+int oracle_lstsq_d(int rows, int cols, const double *a, long lda, const double *b, double *x,
+                   double *residual)
+{
+    if (rows == 0 || cols == 0) return -1;
+    if (rows < cols) return -2;
+    const int n = rows, m = cols;
+    double *q = (double *)malloc(sizeof(double) * (size_t)n * (size_t)n);
+    double *r = (double *)malloc(sizeof(double) * (size_t)n * (size_t)m);
+    double *qtb = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!q || !r || !qtb) { free(q); free(r); free(qtb); return -3; }
+    int rc = oracle_qr_factor_d(rows, cols, a, lda, q, r);
+    if (rc == 0) {
+        for (int i = 0; i < n; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j < n; ++j) acc += q[(size_t)j * n + i] * b[j];
+            qtb[i] = acc;
+        }
+        rc = oracle_solve_triangular_d(m, r, m, qtb, x, 0, 0);
+    }
+    if (rc == 0) {
+        double res = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double ax = 0.0;
+            for (int j = 0; j < m; ++j) ax += a[(size_t)i * lda + j] * x[j];
+            const double d = b[i] - ax;
+            res += d * d;
+        }
+        *residual = res;
+    }
+    free(q); free(r); free(qtb);
+    return rc;
+}
 
 /* Version tag so a test can tell that the built library matches this source. */
 int oracle_version(void) { return 1; }

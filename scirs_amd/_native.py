@@ -82,6 +82,8 @@ SIGNATURES = {
     "scirs_gpu_spotrf": ([_I, _P, _L, _INFO], _I),
     "scirs_gpu_dposv": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
     "scirs_gpu_dtrsv": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _INFO], _I),
+    "scirs_gpu_dqr": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
+    "scirs_gpu_dlstsq": ([_I, _I, _P, _L, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dgesv_ir": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _I, _D, _INFO], _I),
     "scirs_gpu_dgesv_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P, _P, _INFO], _I),
     "scirs_gpu_dposv_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P, _INFO], _I),

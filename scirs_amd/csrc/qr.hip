@@ -1,0 +1,281 @@
+// Householder QR and the least-squares solve built on it.
+//
+//   scirs_gpu_dqr     <- qr_factor   scirs2-linalg/src/lapack.rs:171-253
+//                        (qr          scirs2-linalg/src/decomposition.rs:208-260)
+//   scirs_gpu_dlstsq  <- lstsq       scirs2-linalg/src/solve.rs:234-290 (rows >= cols)
+//
+// The factorisation follows the reference's reflector sequence: for column k
+// the reflector v = x - alpha e1 (alpha = -||x|| when x0 >= 0, else +||x||) is
+// normalised to unit length and H = I - 2 v v^T is applied to the trailing
+// columns of R and to the rows k.. of the accumulated Q^T; a column whose
+// ||x|| or ||v|| is not above machine epsilon is left as it is.  Per column
+// two launches: one workgroup forms v (two block reductions in a fixed order),
+// then every column of the trailing block is owned by one thread column of an
+// apply workgroup, which forms v^T c over its rows and subtracts 2 v (v^T c).
+// Row-major storage makes each 32-column strip of a row a 256-byte coalesced
+// segment.  This is the unblocked (level-2) form: bound by one pass over the
+// trailing block per column; the blocked (compact-WY, MFMA) form is the next
+// step for large matrices.
+#include "api_internal.hpp"
+#include <cfloat>
+#include <vector>
+
+using namespace scirs;
+
+namespace {
+
+constexpr int HT = 1024;   // threads of the reflector kernel
+constexpr int AC = 32;     // columns per apply workgroup
+constexpr int AG = 8;      // row groups per apply workgroup
+constexpr int AT = AC * AG;
+
+// Sum over the workgroup, the same order on every call (waves, then wave sums).
+__device__ double block_sum(double v, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+    return s;
+}
+
+// v[0 .. rows-k) <- the unit reflector of column k of R; flag <- 1, or 0 when
+// the reference skips the column (lapack.rs:205-215).
+__global__ __launch_bounds__(HT) void house_kernel(const double *__restrict__ R, long ldr, int rows, int k,
+                                                   double *__restrict__ v, int *__restrict__ flag)
+{
+    __shared__ double red[HT / 64];
+    const int len = rows - k;
+    const double *col = R + (long)k * ldr + k;
+    double ss = 0.0;
+    for (int i = threadIdx.x; i < len; i += HT) {
+        const double x = col[(long)i * ldr];
+        ss += x * x;
+    }
+    const double xn = sqrt(block_sum(ss, red));
+    if (!(xn > DBL_EPSILON)) {
+        if (threadIdx.x == 0) *flag = 0;
+        return;
+    }
+    const double x0 = col[0];
+    const double alpha = x0 >= 0.0 ? -xn : xn;
+    const double v0 = x0 - alpha;
+    double vs = 0.0;
+    for (int i = threadIdx.x; i < len; i += HT) {
+        const double x = i == 0 ? v0 : col[(long)i * ldr];
+        vs += x * x;
+    }
+    const double vn = sqrt(block_sum(vs, red));
+    if (!(vn > DBL_EPSILON)) {
+        if (threadIdx.x == 0) *flag = 0;
+        return;
+    }
+    for (int i = threadIdx.x; i < len; i += HT) {
+        const double x = i == 0 ? v0 : col[(long)i * ldr];
+        v[i] = x / vn;
+    }
+    if (threadIdx.x == 0) *flag = 1;
+}
+
+// Apply H = I - 2 v v^T to rows k.. of two matrices: R's columns [rc0, rc1)
+// (workgroups [0, nbr)) and all qcols columns of Qt (the remaining workgroups).
+__global__ __launch_bounds__(AT) void house_apply_kernel(double *__restrict__ R, long ldr, int rc0, int rc1, int nbr,
+                                                         double *__restrict__ Qt, long ldq, int qcols, int rows,
+                                                         int k, const double *__restrict__ v,
+                                                         const int *__restrict__ flag)
+{
+    if (*flag == 0) return;
+    __shared__ double part[AG][AC];
+    const int cl = threadIdx.x % AC, rg = threadIdx.x / AC;
+    int b = blockIdx.x;
+    double *M;
+    long ld;
+    int c, cend;
+    if (b < nbr) {
+        M = R; ld = ldr; c = rc0 + b * AC + cl; cend = rc1;
+    } else {
+        b -= nbr;
+        M = Qt; ld = ldq; c = b * AC + cl; cend = qcols;
+    }
+    const bool on = c < cend;
+    const int len = rows - k;
+    double *base = M + (long)k * ld + c;
+    double acc = 0.0;
+    if (on)
+        for (int i = rg; i < len; i += AG) acc += v[i] * base[(long)i * ld];
+    part[rg][cl] = acc;
+    __syncthreads();
+    double dot = 0.0;
+#pragma unroll
+    for (int g = 0; g < AG; ++g) dot += part[g][cl];
+    if (on)
+        for (int i = rg; i < len; i += AG) base[(long)i * ld] -= 2.0 * v[i] * dot;
+}
+
+__global__ void eye_kernel(double *__restrict__ Q, long ldq, int n)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    Q[(long)i * ldq + j] = i == j ? 1.0 : 0.0;
+}
+
+// dst (cols x rows) <- src^T (src rows x cols), 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void transpose_kernel(const double *__restrict__ src, long lds, int rows, int cols,
+                                                        double *__restrict__ dst, long ldd)
+{
+    __shared__ double t[32][33];
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    for (int y = ty; y < 32; y += 8)
+        if (r0 + y < rows && c0 + tx < cols) t[y][tx] = src[(long)(r0 + y) * lds + c0 + tx];
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8)
+        if (c0 + y < cols && r0 + tx < rows) dst[(long)(c0 + y) * ldd + r0 + tx] = t[tx][y];
+}
+
+// sq[i] <- (b_i - A_i x)^2, one wave per row
+__global__ __launch_bounds__(256) void resid_sq_kernel(const double *__restrict__ A, long lda, int rows, int cols,
+                                                       const double *__restrict__ b, const double *__restrict__ x,
+                                                       double *__restrict__ sq)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    if (row >= rows) return;
+    double acc = 0.0;
+    for (int j = l; j < cols; j += 64) acc += A[(long)row * lda + j] * x[j];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (l == 0) {
+        const double d = b[row] - acc;
+        sq[row] = d * d;
+    }
+}
+
+__global__ __launch_bounds__(HT) void sum_kernel(const double *__restrict__ sq, int n, double *__restrict__ out)
+{
+    __shared__ double red[HT / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += HT) s += sq[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// Householder sweep over the first min(rows, cols) columns of R (rows x rcols,
+// ld ldr); Qt (rows x rows, ld ldq) accumulates Q^T when not null.
+int householder_sweep(int rows, int cols, int rcols, double *R, long ldr, double *Qt, long ldq, double *v, int *flag,
+                      hipStream_t s)
+{
+    const int kmax = std::min(rows, cols);
+    for (int k = 0; k < kmax; ++k) {
+        house_kernel<<<1, HT, 0, s>>>(R, ldr, rows, k, v, flag);
+        const int nbr = (rcols - k + AC - 1) / AC;
+        const int nbq = Qt ? (rows + AC - 1) / AC : 0;
+        house_apply_kernel<<<nbr + nbq, AT, 0, s>>>(R, ldr, k, rcols, nbr, Qt, ldq, rows, rows, k, v, flag);
+    }
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+long round_up(long x, long m) { return (x + m - 1) / m * m; }
+
+} // namespace
+
+extern "C" int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, double *q, int64_t ldq, double *r,
+                             int64_t ldr, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (rows <= 0 || cols <= 0 || rows < cols || !a || !q || !r || lda < cols || ldq < rows || ldr < cols) {
+        scirs_set_error("dqr: invalid arguments (rows=%d cols=%d; rows >= cols >= 1)", rows, cols);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    Context &c = ctx();
+    int rc = c.init(0);
+    if (rc) return finish(info, rc, t0);
+    hipStream_t s = c.stream;
+    const long pr = round_up(cols, 64), pq = round_up(rows, 64);
+    const size_t rbytes = (size_t)rows * pr * sizeof(double), qbytes = (size_t)rows * pq * sizeof(double);
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, rbytes))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, 2 * qbytes + (size_t)(rows + 64) * sizeof(double))))
+        return finish(info, rc, t0);
+    double *dR = (double *)c.scratch_a, *dQt = (double *)c.scratch_b, *dQ = dQt + (size_t)rows * pq;
+    double *dv = dQ + (size_t)rows * pq;
+    int *dflag = (int *)(dv + rows);
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
+                                     hipMemcpyHostToDevice, s));
+    eye_kernel<<<(unsigned)(((long)rows * rows + 255) / 256), 256, 0, s>>>(dQt, pq, rows);
+    if ((rc = householder_sweep(rows, cols, cols, dR, pr, dQt, pq, dv, dflag, s))) return finish(info, rc, t0);
+    dim3 tg((rows + 31) / 32, (rows + 31) / 32);
+    transpose_kernel<<<tg, 256, 0, s>>>(dQt, pq, rows, rows, dQ, pq);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(q, ldq * sizeof(double), dQ, pq * sizeof(double), rows * sizeof(double), rows,
+                                     hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(r, ldr * sizeof(double), dR, pr * sizeof(double), cols * sizeof(double), rows,
+                                     hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    return finish(info, SCIRS_OK, t0);
+}
+
+extern "C" int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda, const double *b, double *x,
+                                double *residual, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (rows <= 0 || cols <= 0 || rows < cols || !a || !b || !x || !residual || lda < cols) {
+        scirs_set_error("dlstsq: invalid arguments (rows=%d cols=%d; rows >= cols >= 1)", rows, cols);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    Context &c = ctx();
+    int rc = c.init(0);
+    if (rc) return finish(info, rc, t0);
+    hipStream_t s = c.stream;
+    // R_ext = [A | b], rows x (cols + 1), padded to a 512-byte row pitch; b's
+    // column takes every reflector, which forms Q^T b without Q.
+    const long pr = round_up(cols + 1, 64), pa = round_up(cols, 64);
+    const size_t rbytes = (size_t)rows * pr * sizeof(double);
+    const size_t wbytes = ((size_t)rows * pa + 3 * (size_t)rows + 2 * (size_t)cols + 128) * sizeof(double);
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, rbytes))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, wbytes))) return finish(info, rc, t0);
+    double *dR = (double *)c.scratch_a;
+    // dy (the substitution's right-hand side and solution) starts 512-byte aligned
+    double *dA = (double *)c.scratch_b, *dy = dA + (size_t)rows * pa, *db = dy + cols + 64, *dv = db + rows;
+    double *dsq = dv + rows, *dres = dsq + rows;
+    int *dflag = (int *)(dres + 1);
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
+                                     hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR + cols, pr * sizeof(double), b, sizeof(double), sizeof(double), rows,
+                                     hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, pa * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
+                                     hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, rows * sizeof(double), hipMemcpyHostToDevice, s));
+    if ((rc = householder_sweep(rows, cols, cols + 1, dR, pr, nullptr, 0, dv, dflag, s))) return finish(info, rc, t0);
+    // the back substitution's |r_ii| < eps test (solve.rs:184-191), first row
+    // in substitution order (bottom up)
+    std::vector<double> diag(cols);
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(diag.data(), sizeof(double), dR, (pr + 1) * sizeof(double), sizeof(double), cols,
+                                     hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = cols - 1; i >= 0; --i)
+        if (fabs(diag[i]) < DBL_EPSILON) {
+            if (info) info->pivot_value = fabs(diag[i]);
+            return finish(info, i + 1, t0);
+        }
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dy, sizeof(double), dR + cols, pr * sizeof(double), sizeof(double), cols,
+                                     hipMemcpyDeviceToDevice, s));
+    if ((rc = c.ensure_xtri(cols))) return finish(info, rc, t0);
+    if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
+    if ((rc = scirs_trsv_chain<double>(cols, 1, dR, pr, dy, 1, 0, 0, c.work, s))) return finish(info, rc, t0);
+    int st = 0;
+    if ((rc = c.read_flags(s, &st, nullptr))) return finish(info, rc, t0);
+    resid_sq_kernel<<<(rows + 3) / 4, 256, 0, s>>>(dA, pa, rows, cols, db, dy, dsq);
+    sum_kernel<<<1, HT, 0, s>>>(dsq, rows, dres);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    SCIRS_HIP_CHECK(hipMemcpyAsync(x, dy, cols * sizeof(double), hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipMemcpyAsync(residual, dres, sizeof(double), hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    return finish(info, SCIRS_OK, t0);
+}

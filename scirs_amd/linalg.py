@@ -118,6 +118,33 @@ def trsv(a, b, lower, unit_diagonal):
     return (x[:, 0] if vec else x), info
 
 
+def qr(a):
+    """(Q, R) of an f64 rows x cols matrix, rows >= cols (lapack.rs:171-253)."""
+    lib = N.load()
+    a = _c(a, np.float64)
+    rows, cols = a.shape
+    q = np.empty((rows, rows))
+    r = np.empty((rows, cols))
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dqr(rows, cols, _ptr(a), cols, _ptr(q), rows, _ptr(r), cols, ctypes.byref(info))
+    _check("qr", rc, info)
+    return q, r
+
+
+def lstsq(a, b):
+    """(x, residuals) of the rows >= cols least-squares solve (solve.rs:234-290)."""
+    lib = N.load()
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    rows, cols = a.shape
+    x = np.empty(cols)
+    res = ctypes.c_double(0.0)
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dlstsq(rows, cols, _ptr(a), cols, _ptr(b), _ptr(x), ctypes.byref(res), ctypes.byref(info))
+    _check("lstsq", rc, info)
+    return x, res.value
+
+
 def gesv_ir(a, b, mode=N.MODE_AUTO, max_iter=0, max_gmres=0, tol=0.0):
     lib = N.load()
     a = _c(a, np.float64)

@@ -314,3 +314,85 @@ def test_unmutated_python_copy_passes():
     check_lu_kat(f)
     check_lu_lapack(f, 16)
     check_lu_tie_rule(f)
+
+
+# ------------------------------------------------------------- QR / lstsq ---
+# qr_factor is pinned by LAPACK's Householder QR (numpy.linalg.qr, complete
+# mode): dgeqrf's reflector sign beta = -sign(alpha)*||x|| (with sign(0) = +)
+# is the reference's alpha rule, so Q and R agree entry for entry to rounding,
+# signs included (except where LAPACK leaves a column with a zero tail alone
+# and the reference reflects it: the last column of a square matrix); a flipped sign, a dropped factor 2 or a transposed Q fails
+# it.  The reference's own checks (lapack.rs:757-774, decomposition.rs:794-815,
+# tests/scipy_validation.rs:78-99) are the Q^T Q = I / Q R = A / upper-R ones.
+# lstsq is pinned by numpy.linalg.lstsq (x and the residual sum of squares)
+# and by the documented example of solve.rs:225-233.
+@pytest.mark.parametrize("shape", [(1, 1), (2, 2), (3, 2), (5, 3), (40, 25), (64, 64), (130, 7)])
+def test_qr_matches_lapack(shape):
+    rng = np.random.default_rng(shape[0] * 131 + shape[1])
+    a = rng.standard_normal(shape)
+    q, r = oracle.qr_factor(a)
+    qn, rn = np.linalg.qr(a, mode="complete")
+    if shape[0] == shape[1]:
+        # the reference also reflects the last 1-element column (v = e1, H = -1,
+        # so r_nn changes sign), where LAPACK's dlarfg takes tau = 0
+        qn[:, -1] *= -1.0
+        rn[-1, :] *= -1.0
+    assert q.shape == (shape[0], shape[0]) and r.shape == shape
+    scale = max(1.0, np.abs(a).max())
+    assert np.abs(q - qn).max() <= 64 * shape[0] * EPS64
+    assert np.abs(r - rn).max() <= 64 * shape[0] * EPS64 * scale
+    assert np.abs(q @ r - a).max() <= 64 * shape[0] * EPS64 * scale
+    assert np.abs(q.T @ q - np.eye(shape[0])).max() <= 64 * shape[0] * EPS64
+
+
+def test_qr_reference_examples():
+    for a in ([[2.0, 1.0], [4.0, 3.0]], [[1.0, 2.0], [3.0, 4.0]], [[1.0, 2.0], [3.0, 4.0], [5.0, 6.0]]):
+        a = np.array(a)
+        q, r = oracle.qr_factor(a)
+        assert np.allclose(q.T @ q, np.eye(a.shape[0]), atol=1e-12)
+        assert np.allclose(q @ r, a, atol=1e-12)
+        assert np.allclose(np.tril(r, -1), 0.0, atol=1e-12)
+    # [[1, 2], [3, 4]]: r_00 = -sqrt(10) (x0 = 1 >= 0 -> alpha = -||x||)
+    _, r = oracle.qr_factor(np.array([[1.0, 2.0], [3.0, 4.0]]))
+    assert abs(r[0, 0] + np.sqrt(10.0)) < 1e-14
+    assert abs(r[1, 1] - 2.0 / np.sqrt(10.0)) < 1e-14
+
+
+def test_qr_zero_column_is_skipped():
+    a = np.array([[0.0, 1.0], [0.0, 2.0], [0.0, 3.0]])
+    q, r = oracle.qr_factor(a)
+    # column 0 has ||x|| = 0 <= eps: no reflector; column 1 is reflected from row 1
+    assert np.all(r[:, 0] == 0.0)
+    assert r[0, 1] == 1.0 and q[0, 0] == 1.0
+    assert np.allclose(q @ r, a, atol=1e-14)
+
+
+def test_qr_rejects_wide_and_empty():
+    with pytest.raises(oracle.OracleError):
+        oracle.qr_factor(np.ones((2, 3)))
+    with pytest.raises(oracle.OracleError):
+        oracle.qr_factor(np.ones((0, 0)))
+
+
+@pytest.mark.parametrize("shape", [(3, 2), (50, 20), (200, 200), (301, 45)])
+def test_lstsq_matches_lapack(shape):
+    rng = np.random.default_rng(shape[0] + 7 * shape[1])
+    a = rng.standard_normal(shape)
+    b = rng.standard_normal(shape[0])
+    x, res = oracle.lstsq(a, b)
+    xn, rn, _, _ = np.linalg.lstsq(a, b, rcond=None)
+    assert np.abs(x - xn).max() <= 1e-10 * max(1.0, np.abs(xn).max())
+    expect = float(rn[0]) if rn.size else float(np.sum((b - a @ xn) ** 2))
+    assert abs(res - expect) <= 1e-10 * max(1.0, expect)
+
+
+def test_lstsq_documented_example():
+    x, res = oracle.lstsq(np.array([[1.0, 1.0], [1.0, 2.0], [1.0, 3.0]]), np.array([6.0, 9.0, 12.0]))
+    assert np.allclose(x, [3.0, 3.0], atol=1e-12) and res < 1e-24
+
+
+def test_lstsq_rank_deficient_reports_the_row():
+    a = np.array([[1.0, 0.0, 2.0], [3.0, 0.0, 1.0], [4.0, 0.0, 5.0], [2.0, 0.0, 7.0]])
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.lstsq(a, np.ones(4))
+    assert e.value.code == 2   # r_11 = 0; row 3 (bottom) passes, row 2 fails
