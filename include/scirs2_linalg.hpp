@@ -13,6 +13,7 @@
 //   mixed_precision_solve, iterative_refinement_solve
 //                    scirs2-linalg/src/mixed_precision/adaptive.rs:49-151, 282-378
 //   compat_solve     scirs2-linalg/src/compat.rs:461-499
+//   qr, lstsq        scirs2-linalg/src/decomposition.rs:208-249, solve.rs:234-290 (rows >= cols)
 //   validation       scirs2-linalg/src/validation.rs:22-329
 //   LinalgError      scirs2-linalg/src/error.rs:8-150
 //
@@ -31,6 +32,8 @@
 #include <optional>
 #include <string>
 #include <tuple>
+#include <type_traits>
+#include <utility>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -659,6 +662,68 @@ Result<Array1<C>> iterative_refinement_solve(const ArrayView2<A> &a, const Array
 }
 
 // ------------------------------------------------------------------ compat
+// decomposition.rs:208-249 (qr over lapack.rs:171-253), f64 on the GPU.
+template <class F>
+Result<std::pair<Array2<F>, Array2<F>>> qr(const ArrayView2<F> &a, std::optional<std::size_t> workers)
+{
+    static_assert(std::is_same<F, double>::value, "the GPU QR handles f64");
+    if (a.is_empty())
+        return LinalgError(LinalgError::Kind::ShapeError, "QR decomposition failed: Input matrix cannot be empty");
+    for (std::size_t i = 0; i < a.rows; ++i)
+        for (std::size_t j = 0; j < a.cols; ++j)
+            if (!std::isfinite(a(i, j)))
+                return LinalgError(LinalgError::Kind::InvalidInputError,
+                                   "QR decomposition failed: Matrix contains non-finite values");
+    detail::set_workers(workers);
+    // lapack.rs:185-190
+    if (a.rows < a.cols)
+        return LinalgError(LinalgError::Kind::ComputationError, "QR decomposition requires rows >= columns");
+    std::vector<F> ad = detail::standard(a);
+    Array2<F> q(a.rows, a.rows), r(a.rows, a.cols);
+    scirs_solve_info info;
+    int rc = scirs_gpu_dqr((int)a.rows, (int)a.cols, ad.data(), (int64_t)a.cols, q.data.data(), (int64_t)a.rows,
+                           r.data.data(), (int64_t)a.cols, &info);
+    if (rc < 0) return detail::gpu_failure(rc);
+    return std::make_pair(std::move(q), std::move(r));
+}
+
+// solve.rs:17-26 (LstsqResult)
+template <class F>
+struct LstsqResult {
+    Array1<F> x;
+    F residuals;
+    std::size_t rank;
+    Array1<F> s;
+};
+
+// solve.rs:234-290: the rows >= cols (QR) branch runs on the GPU; the rows <
+// cols branch is the reference's SVD path, which this library does not
+// provide (the shim keeps the reference's own code for it).
+template <class F>
+Result<LstsqResult<F>> lstsq(const ArrayView2<F> &a, const ArrayView1<F> &b, std::optional<std::size_t> workers)
+{
+    static_assert(std::is_same<F, double>::value, "the GPU least-squares solve handles f64");
+    if (auto e = validate_not_empty_matrix(a, "Least squares solve")) return *e;
+    if (auto e = validate_not_empty_vector(b, "Least squares solve")) return *e;
+    if (auto e = validate_matrix_vector_dimensions(a, b, "Least squares solve")) return *e;
+    if (auto e = validate_finite_matrix(a, "Least squares solve")) return *e;
+    if (auto e = validate_finite_vector(b, "Least squares solve")) return *e;
+    detail::set_workers(workers);
+    if (a.rows < a.cols)
+        return LinalgError(LinalgError::Kind::NotImplementedError,
+                           "lstsq with rows < columns (the SVD branch) is not on the GPU path");
+    std::vector<F> ad = detail::standard(a), bd(a.rows);
+    for (std::size_t i = 0; i < a.rows; ++i) bd[i] = b[i];
+    LstsqResult<F> out{Array1<F>(a.cols), F(0), a.cols, Array1<F>()};
+    scirs_solve_info info;
+    int rc = scirs_gpu_dlstsq((int)a.rows, (int)a.cols, ad.data(), (int64_t)a.cols, bd.data(), out.x.data(),
+                              &out.residuals, &info);
+    if (rc > 0)
+        return singular_matrix_with_suggestions("triangular system solve (back substitution)", {a.cols, a.cols}, 1e16);
+    if (rc < 0) return detail::gpu_failure(rc);
+    return out;
+}
+
 // compat.rs:461-499 (scipy.linalg.solve).  The reference rejects every
 // specialised option with NotImplementedError; here assume_a = "pos" (without
 // `lower` / `transposed`) is served by the GPU Cholesky solve, the other

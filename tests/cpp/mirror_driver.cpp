@@ -4,7 +4,7 @@
 // Input, whitespace separated, one case after another:
 //   <op> <type> <flag1> <flag2> <ar> <ac> <ar*ac values> <br> <bc> <br*bc values>
 //   op   : solve | solve_multiple | lu | cholesky | solve_triangular | det | inv |
-//          mixed_precision_solve | iterative_refinement_solve | compat_solve
+//          mixed_precision_solve | iterative_refinement_solve | compat_solve | qr | lstsq
 //   type : f64 | f32
 //   flags: lower / unit_diagonal for solve_triangular, ignored otherwise
 //   b    : the right-hand side (br = length, bc = 1 for a vector; 0 0 when unused)
@@ -18,6 +18,7 @@
 #include <iostream>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 using namespace scirs2_linalg;
@@ -132,6 +133,29 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
         if (r.is_err()) return print_err(r.error());
         std::printf("OK 1");
         put(r.value().rows, r.value().cols, r.value().data.data());
+    } else if (op == "qr" || op == "lstsq") {
+        if constexpr (std::is_same<F, double>::value) {
+            if (op == "qr") {
+                auto r = qr<F>(a.view(), workers);
+                if (r.is_err()) return print_err(r.error());
+                std::printf("OK 2");
+                put(r.value().first.rows, r.value().first.cols, r.value().first.data.data());
+                put(r.value().second.rows, r.value().second.cols, r.value().second.data.data());
+            } else {
+                auto r = lstsq<F>(a.view(), view1(bvec), workers);
+                if (r.is_err()) return print_err(r.error());
+                const auto &v = r.value();
+                const double rk = (double)v.rank;
+                std::printf("OK 4");
+                put(v.x.size(), 1, v.x.data());
+                put(1, 1, &v.residuals);
+                put(1, 1, &rk);
+                put(v.s.size(), 1, v.s.data());
+            }
+        } else {
+            std::printf("ERR Other %s is f64 only\n", op.c_str());
+            return;
+        }
     } else {
         std::printf("ERR Other unknown op %s\n", op.c_str());
         return;

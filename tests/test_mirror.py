@@ -432,3 +432,58 @@ def test_compat_solve_general_and_pos():
     x32 = got[2][1][0]
     assert np.abs(x32 - np.linalg.solve(s, b)).max() <= 1e-5
     assert got[3] == ("ERR", "NonPositiveDefiniteError", not_pd("Cholesky decomposition", n, n))
+
+
+def test_qr_lstsq_validation_texts():
+    """qr (decomposition.rs:208-249, lapack.rs:185-190) and lstsq
+    (solve.rs:234-250, validation.rs:247-261): errors raised before any device
+    work keep the reference's kinds and texts."""
+    bad = np.ones((3, 2))
+    bad[2, 1] = np.nan
+    got = run([
+        ("qr", "f64", 0, 0, np.zeros((0, 0)), None),
+        ("qr", "f64", 0, 0, bad, None),
+        ("qr", "f64", 0, 0, np.ones((2, 3)), None),
+        ("lstsq", "f64", 0, 0, bad, np.ones(3)),
+        ("lstsq", "f64", 0, 0, np.ones((3, 2)), np.array([1.0, INF, 2.0])),
+        ("lstsq", "f64", 0, 0, np.ones((2, 3)), np.ones(2)),
+    ])
+    assert got[0] == ("ERR", "ShapeError", "Shape error: QR decomposition failed: Input matrix cannot be empty")
+    assert got[1] == ("ERR", "InvalidInputError",
+                      "Invalid input error: QR decomposition failed: Matrix contains non-finite values")
+    assert got[2] == ("ERR", "ComputationError", "Computation error: QR decomposition requires rows >= columns")
+    assert got[3] == ("ERR", "InvalidInputError",
+                      "Invalid input error: Least squares solve failed: Matrix contains non-finite values")
+    assert got[4] == ("ERR", "InvalidInputError",
+                      "Invalid input error: Least squares solve failed: Vector contains non-finite values")
+    assert got[5][0] == "ERR" and got[5][1] == "NotImplementedError"
+
+
+@pytest.mark.gpu
+def test_qr_lstsq_results():
+    """qr and lstsq through the mirror: the oracle's Q, R (lapack.rs:171-253) and
+    x, residuals (solve.rs:234-290) within 64 rows u; rank = cols, s empty; a
+    rank-deficient R reports the back substitution's singular error."""
+    from oracle import oracle
+    rng = np.random.default_rng(17)
+    a = rng.standard_normal((60, 24))
+    b = rng.standard_normal(60)
+    dep = np.array([[1.0, 0.0, 2.0], [3.0, 0.0, 1.0], [4.0, 0.0, 5.0], [2.0, 0.0, 7.0]])
+    got = run([
+        ("qr", "f64", 0, 0, a, None),
+        ("lstsq", "f64", 0, 0, a, b),
+        ("lstsq", "f64", 0, 0, [[1.0, 1.0], [1.0, 2.0], [1.0, 3.0]], [6.0, 9.0, 12.0]),
+        ("lstsq", "f64", 0, 0, dep, np.ones(4)),
+    ])
+    tol = 64 * 60 * U64 * np.abs(a).max()
+    qo, ro = oracle.qr_factor(a)
+    assert got[0][0] == "OK"
+    assert np.abs(got[0][1][0] - qo).max() <= tol and np.abs(got[0][1][1] - ro).max() <= tol
+    xo, reso = oracle.lstsq(a, b)
+    x, res, rank, s = got[1][1]
+    assert np.abs(x[:, 0] - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert abs(res[0, 0] - reso) <= 1e-9 * reso
+    assert rank[0, 0] == 24 and s.size == 0
+    assert np.allclose(got[2][1][0][:, 0], [3.0, 3.0], atol=1e-12)
+    assert got[3] == ("ERR", "SingularMatrixError",
+                      singular("triangular system solve (back substitution)", 3, 3, "1.00e16"))
