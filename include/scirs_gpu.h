@@ -124,6 +124,35 @@ int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, double *q, i
 int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda, const double *b, double *x,
                      double *residual, scirs_solve_info *info);
 
+/* ---- batched array operations ------------------------------------------------ */
+/* Replace scirs2-linalg/src/batch/mod.rs for f64 / f32, bit-identical to its
+ * loops (each element summed from +0 in the reference's order, no fused
+ * multiply-add).  Batches are contiguous C-order arrays batch x m x k.
+ *   batch_matmul (mod.rs:62-107):  c[p] = a[p] b,  b is k x n, c batch x m x n
+ *   batch_matvec (mod.rs:139-172): y[p] = a[p] x,  a batch x m x n, y batch x m
+ *   batch_add    (mod.rs:220-291): out = a + v along axis 0 (v[i], len m) or
+ *                                  axis 1 (v[j], len n); other axes -> SCIRS_ERR_ARG
+ *   batch_sum    (mod.rs:326-336): out (m x n) = sum over the batch, in batch order
+ * Empty shapes return 0 and write nothing (k = 0 gives zeros). */
+int scirs_gpu_dbatch_matmul(int batch, int m, int k, int n, const double *a, const double *b, double *c,
+                            scirs_solve_info *info);
+int scirs_gpu_sbatch_matmul(int batch, int m, int k, int n, const float *a, const float *b, float *c,
+                            scirs_solve_info *info);
+int scirs_gpu_dbatch_matvec(int batch, int m, int n, const double *a, const double *x, double *y,
+                            scirs_solve_info *info);
+int scirs_gpu_sbatch_matvec(int batch, int m, int n, const float *a, const float *x, float *y,
+                            scirs_solve_info *info);
+int scirs_gpu_dbatch_add(int batch, int m, int n, const double *a, const double *v, int axis, double *out,
+                         scirs_solve_info *info);
+int scirs_gpu_sbatch_add(int batch, int m, int n, const float *a, const float *v, int axis, float *out,
+                         scirs_solve_info *info);
+int scirs_gpu_dbatch_sum(int batch, int m, int n, const double *a, double *out, scirs_solve_info *info);
+int scirs_gpu_sbatch_sum(int batch, int m, int n, const float *a, float *out, scirs_solve_info *info);
+/* batch_matmul on device-resident arrays, on the caller's stream (NULL = the
+ * default stream); no synchronisation. */
+int scirs_gpu_dbatch_matmul_dev(int batch, int m, int k, int n, const double *dA, const double *dB, double *dC,
+                                void *stream);
+
 /* ---- mixed precision ---------------------------------------------------------- */
 /* The factor-once counterpart of scirs2-linalg/src/mixed_precision/adaptive.rs:282-378
  * `iterative_refinement_solve::<f64,f64,f64,f64,f32>`: FP32 panel, low-precision

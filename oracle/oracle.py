@@ -12,6 +12,8 @@ scirs2-linalg's CPU solve path:
 * ``lu``               -> scirs2-linalg/src/decomposition.rs:157-180 (P, L, U build)
 * ``qr_factor``        -> scirs2-linalg/src/lapack.rs:171-253 (f64)
 * ``lstsq``            -> scirs2-linalg/src/solve.rs:234-290, rows >= cols (f64)
+* ``batch_matmul``, ``batch_matvec``, ``batch_add``, ``batch_sum``
+                       -> scirs2-linalg/src/batch/mod.rs:62-336
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import this module; it is the checker, never the product.
@@ -61,6 +63,11 @@ def _load():
         getattr(lib, f"oracle_solve_triangular_{s}").argtypes = [I, P, L, P, P, I, I]
         getattr(lib, f"oracle_solve_{s}").argtypes = [I, P, L, P, P]
         getattr(lib, f"oracle_solve_multiple_{s}").argtypes = [I, I, P, L, P, L, P, L]
+    for s in ("d", "s"):
+        getattr(lib, f"oracle_batch_matmul_{s}").argtypes = [I, I, I, I, P, P, P]
+        getattr(lib, f"oracle_batch_matvec_{s}").argtypes = [I, I, I, P, P, P]
+        getattr(lib, f"oracle_batch_add_{s}").argtypes = [I, I, I, P, P, I, P]
+        getattr(lib, f"oracle_batch_sum_{s}").argtypes = [I, I, I, P, P]
     lib.oracle_qr_factor_d.argtypes = [I, I, P, L, P, P]
     lib.oracle_lstsq_d.argtypes = [I, I, P, L, P, P, P]
     _lib = lib
@@ -232,3 +239,49 @@ def lstsq(a, b):
     if rc < 0:
         raise OracleError("empty" if rc == -1 else "shape", rc)
     return x, float(res[0])
+
+
+def _bt(a):
+    a = np.asarray(a)
+    dt = np.float32 if a.dtype == np.float32 else np.float64
+    return _c(a, dt), dt
+
+
+def batch_matmul(a, b):
+    """batch/mod.rs:62-107: (batch, m, k) x (k, n) -> (batch, m, n), sums in k order."""
+    a, dt = _bt(a)
+    b = _c(b, dt)
+    p, m, k = a.shape
+    n = b.shape[1]
+    c = np.empty((p, m, n), dtype=dt)
+    getattr(_load(), f"oracle_batch_matmul_{_sfx(dt)}")(p, m, k, n, _ptr(a), _ptr(b), _ptr(c))
+    return c
+
+
+def batch_matvec(a, x):
+    """batch/mod.rs:139-172: (batch, m, n) x (n,) -> (batch, m)."""
+    a, dt = _bt(a)
+    x = _c(x, dt)
+    p, m, n = a.shape
+    y = np.empty((p, m), dtype=dt)
+    getattr(_load(), f"oracle_batch_matvec_{_sfx(dt)}")(p, m, n, _ptr(a), _ptr(x), _ptr(y))
+    return y
+
+
+def batch_add(a, v, axis):
+    """batch/mod.rs:220-291: v broadcast along axis 0 (rows) or 1 (columns)."""
+    a, dt = _bt(a)
+    v = _c(v, dt)
+    p, m, n = a.shape
+    out = np.empty_like(a)
+    getattr(_load(), f"oracle_batch_add_{_sfx(dt)}")(p, m, n, _ptr(a), _ptr(v), int(axis), _ptr(out))
+    return out
+
+
+def batch_sum(a):
+    """batch/mod.rs:326-336: the sum over the batch axis, added in batch order."""
+    a, dt = _bt(a)
+    p, m, n = a.shape
+    out = np.empty((m, n), dtype=dt)
+    getattr(_load(), f"oracle_batch_sum_{_sfx(dt)}")(p, m, n, _ptr(a), _ptr(out))
+    return out

@@ -18,6 +18,7 @@
  *   oracle_solve_multiple_*   scirs2-linalg/src/solve.rs:366-424
  *   oracle_qr_factor_d        scirs2-linalg/src/lapack.rs:171-253
  *   oracle_lstsq_d            scirs2-linalg/src/solve.rs:234-290 (rows >= cols)
+ *   oracle_batch_*            scirs2-linalg/src/batch/mod.rs:62-336
  *
  * All matrices are dense row-major with an explicit leading dimension, the
  * layout of an ndarray in standard order.
@@ -318,6 +319,56 @@ int oracle_lstsq_d(int rows, int cols, const double *a, long lda, const double *
     free(q); free(r); free(qtb);
     return rc;
 }
+
+/* batch/mod.rs:62-336.  batch a is batch x m x k (C order, contiguous).
+ *   batch_matmul: c[p,i,j] = sum_k a[p,i,k] * b[k,j], summed from zero in k order
+ *   batch_matvec: y[p,i]   = sum_j a[p,i,j] * x[j], summed from zero in j order
+ *   batch_add:    a[p,i,j] + v[i] (axis 0) or + v[j] (axis 1)
+ *   batch_sum:    sum over p of a[p,i,j], accumulated from zero in p order
+ *                 (ndarray's sum_axis over a non-minimal-stride axis adds the
+ *                 sub-views one after another) */
+#define ORACLE_BATCH(T, S)                                                          \
+void oracle_batch_matmul_##S(int batch, int m, int k, int n, const T *a,            \
+                             const T *b, T *c)                                      \
+{                                                                                   \
+    for (int p = 0; p < batch; ++p)                                                 \
+        for (int i = 0; i < m; ++i)                                                 \
+            for (int j = 0; j < n; ++j) {                                           \
+                T sum = (T)0;                                                       \
+                const T *ar = a + ((size_t)p * m + i) * k;                          \
+                for (int q = 0; q < k; ++q) sum += ar[q] * b[(size_t)q * n + j];    \
+                c[((size_t)p * m + i) * n + j] = sum;                               \
+            }                                                                       \
+}                                                                                   \
+void oracle_batch_matvec_##S(int batch, int m, int n, const T *a, const T *x, T *y)  \
+{                                                                                   \
+    for (int p = 0; p < batch; ++p)                                                 \
+        for (int i = 0; i < m; ++i) {                                               \
+            T sum = (T)0;                                                           \
+            const T *ar = a + ((size_t)p * m + i) * n;                              \
+            for (int j = 0; j < n; ++j) sum += ar[j] * x[j];                        \
+            y[(size_t)p * m + i] = sum;                                             \
+        }                                                                           \
+}                                                                                   \
+void oracle_batch_add_##S(int batch, int m, int n, const T *a, const T *v, int axis, \
+                          T *out)                                                   \
+{                                                                                   \
+    for (int p = 0; p < batch; ++p)                                                 \
+        for (int i = 0; i < m; ++i)                                                 \
+            for (int j = 0; j < n; ++j) {                                           \
+                const size_t o = ((size_t)p * m + i) * n + j;                       \
+                out[o] = a[o] + (axis == 0 ? v[i] : v[j]);                          \
+            }                                                                       \
+}                                                                                   \
+void oracle_batch_sum_##S(int batch, int m, int n, const T *a, T *out)              \
+{                                                                                   \
+    for (size_t o = 0; o < (size_t)m * n; ++o) out[o] = (T)0;                        \
+    for (int p = 0; p < batch; ++p)                                                 \
+        for (size_t o = 0; o < (size_t)m * n; ++o) out[o] = out[o] + a[(size_t)p * m * n + o]; \
+}
+
+ORACLE_BATCH(double, d)
+ORACLE_BATCH(float, s)
 
 /* Version tag so a test can tell that the built library matches this source. */
 int oracle_version(void) { return 1; }

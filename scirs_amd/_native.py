@@ -82,6 +82,15 @@ SIGNATURES = {
     "scirs_gpu_spotrf": ([_I, _P, _L, _INFO], _I),
     "scirs_gpu_dposv": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
     "scirs_gpu_dtrsv": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _INFO], _I),
+    "scirs_gpu_dbatch_matmul": ([_I, _I, _I, _I, _P, _P, _P, _INFO], _I),
+    "scirs_gpu_sbatch_matmul": ([_I, _I, _I, _I, _P, _P, _P, _INFO], _I),
+    "scirs_gpu_dbatch_matvec": ([_I, _I, _I, _P, _P, _P, _INFO], _I),
+    "scirs_gpu_sbatch_matvec": ([_I, _I, _I, _P, _P, _P, _INFO], _I),
+    "scirs_gpu_dbatch_add": ([_I, _I, _I, _P, _P, _I, _P, _INFO], _I),
+    "scirs_gpu_sbatch_add": ([_I, _I, _I, _P, _P, _I, _P, _INFO], _I),
+    "scirs_gpu_dbatch_sum": ([_I, _I, _I, _P, _P, _INFO], _I),
+    "scirs_gpu_sbatch_sum": ([_I, _I, _I, _P, _P, _INFO], _I),
+    "scirs_gpu_dbatch_matmul_dev": ([_I, _I, _I, _I, _P, _P, _P, _P], _I),
     "scirs_gpu_dqr": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
     "scirs_gpu_dlstsq": ([_I, _I, _P, _L, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dgesv_ir": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _I, _D, _INFO], _I),

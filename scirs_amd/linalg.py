@@ -145,6 +145,65 @@ def lstsq(a, b):
     return x, res.value
 
 
+def _batch_args(a):
+    a = np.asarray(a)
+    dt = np.float32 if a.dtype == np.float32 else np.float64
+    return _c(a, dt), dt, ("s" if dt == np.float32 else "d")
+
+
+def batch_matmul(a, b):
+    """(batch, m, k) x (k, n) -> (batch, m, n), bit-identical to batch/mod.rs:62-107."""
+    lib = N.load()
+    a, dt, p = _batch_args(a)
+    b = _c(b, dt)
+    nb, m, k = a.shape
+    n = b.shape[1]
+    c = np.empty((nb, m, n), dtype=dt)
+    info = N.SolveInfo()
+    rc = getattr(lib, f"scirs_gpu_{p}batch_matmul")(nb, m, k, n, _ptr(a), _ptr(b), _ptr(c), ctypes.byref(info))
+    _check("batch_matmul", rc, info)
+    return c
+
+
+def batch_matvec(a, x):
+    """(batch, m, n) x (n,) -> (batch, m), bit-identical to batch/mod.rs:139-172."""
+    lib = N.load()
+    a, dt, p = _batch_args(a)
+    x = _c(x, dt)
+    nb, m, n = a.shape
+    y = np.empty((nb, m), dtype=dt)
+    info = N.SolveInfo()
+    rc = getattr(lib, f"scirs_gpu_{p}batch_matvec")(nb, m, n, _ptr(a), _ptr(x), _ptr(y), ctypes.byref(info))
+    _check("batch_matvec", rc, info)
+    return y
+
+
+def batch_add(a, v, axis):
+    """a + v along axis 0 (rows) or 1 (columns), as batch/mod.rs:220-291."""
+    lib = N.load()
+    a, dt, p = _batch_args(a)
+    v = _c(v, dt)
+    nb, m, n = a.shape
+    out = np.empty_like(a)
+    info = N.SolveInfo()
+    rc = getattr(lib, f"scirs_gpu_{p}batch_add")(nb, m, n, _ptr(a), _ptr(v), int(axis), _ptr(out),
+                                                 ctypes.byref(info))
+    _check("batch_add", rc, info)
+    return out
+
+
+def batch_sum(a):
+    """The sum over the batch axis, in batch order (batch/mod.rs:326-336)."""
+    lib = N.load()
+    a, dt, p = _batch_args(a)
+    nb, m, n = a.shape
+    out = np.empty((m, n), dtype=dt)
+    info = N.SolveInfo()
+    rc = getattr(lib, f"scirs_gpu_{p}batch_sum")(nb, m, n, _ptr(a), _ptr(out), ctypes.byref(info))
+    _check("batch_sum", rc, info)
+    return out
+
+
 def gesv_ir(a, b, mode=N.MODE_AUTO, max_iter=0, max_gmres=0, tol=0.0):
     lib = N.load()
     a = _c(a, np.float64)

@@ -396,3 +396,44 @@ def test_lstsq_rank_deficient_reports_the_row():
     with pytest.raises(oracle.OracleError) as e:
         oracle.lstsq(a, np.ones(4))
     assert e.value.code == 2   # r_11 = 0; row 3 (bottom) passes, row 2 fails
+
+
+# ------------------------------------------------------------ batch ops ---
+# The batch operations are pinned by the reference's own tests
+# (batch/mod.rs:340-495: exact small answers), by exact integer arithmetic
+# (products and sums of small integers are exact in f64, so any order gives
+# the same bits: a dropped term or a transposed operand changes them), and by
+# numpy to rounding on random inputs.
+def test_batch_ops_reference_kats():
+    a = np.array([1.0, 2, 3, 4, 5, 6, 7, 8]).reshape(2, 2, 2)
+    assert oracle.batch_matmul(a, np.array([[1.0, 2], [3, 4]])).ravel().tolist() == [7, 10, 15, 22, 23, 34, 31, 46]
+    assert oracle.batch_matvec(a, np.array([1.0, 2])).ravel().tolist() == [5, 11, 17, 23]
+    assert oracle.batch_add(a, np.array([10.0, 20]), 1).ravel().tolist() == [11, 22, 13, 24, 15, 26, 17, 28]
+    assert oracle.batch_add(a, np.array([10.0, 20]), 0).ravel().tolist() == [11, 12, 23, 24, 15, 16, 27, 28]
+    assert oracle.batch_sum(a).ravel().tolist() == [6, 8, 10, 12]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_batch_ops_exact_integers(dt):
+    rng = np.random.default_rng(3)
+    a = rng.integers(-50, 50, (5, 7, 9)).astype(dt)
+    b = rng.integers(-50, 50, (9, 4)).astype(dt)
+    x = rng.integers(-50, 50, 9).astype(dt)
+    assert np.array_equal(oracle.batch_matmul(a, b), np.einsum("pik,kj->pij", a, b))
+    assert np.array_equal(oracle.batch_matvec(a, x), np.einsum("pij,j->pi", a, x))
+    assert np.array_equal(oracle.batch_add(a, x, 1), a + x[None, None, :])
+    assert np.array_equal(oracle.batch_add(a, x[:7], 0), a + x[:7][None, :, None])
+    assert np.array_equal(oracle.batch_sum(a), a.sum(axis=0))
+
+
+def test_batch_ops_random_to_rounding():
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((3, 20, 33))
+    b = rng.standard_normal((33, 11))
+    x = rng.standard_normal(33)
+    assert np.allclose(oracle.batch_matmul(a, b), a @ b, rtol=0, atol=1e-13)
+    assert np.allclose(oracle.batch_matvec(a, x), a @ x, rtol=0, atol=1e-13)
+    s = np.zeros((20, 33))
+    for p in range(3):
+        s = s + a[p]
+    assert np.array_equal(oracle.batch_sum(a), s)   # batch order, as ndarray's sum_axis
