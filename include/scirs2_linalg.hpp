@@ -14,6 +14,8 @@
 //                    scirs2-linalg/src/mixed_precision/adaptive.rs:49-151, 282-378
 //   compat_solve     scirs2-linalg/src/compat.rs:461-499
 //   qr, lstsq        scirs2-linalg/src/decomposition.rs:208-249, solve.rs:234-290 (rows >= cols)
+//   batch_matmul, batch_matvec, batch_add, batch_sum
+//                    scirs2-linalg/src/batch/mod.rs:62-336
 //   validation       scirs2-linalg/src/validation.rs:22-329
 //   LinalgError      scirs2-linalg/src/error.rs:8-150
 //
@@ -192,6 +194,31 @@ struct Array2 {
 
 template <class F>
 using Array1 = std::vector<F>;
+
+// ndarray's ArrayView3 / Array3 (any strides in the view; the owned array is C order)
+template <class F>
+struct ArrayView3 {
+    const F *data = nullptr;
+    std::size_t d0 = 0, d1 = 0, d2 = 0;
+    std::ptrdiff_t s0 = 0, s1 = 0, s2 = 1;
+    F operator()(std::size_t p, std::size_t i, std::size_t j) const
+    {
+        return data[(std::ptrdiff_t)p * s0 + (std::ptrdiff_t)i * s1 + (std::ptrdiff_t)j * s2];
+    }
+    std::tuple<std::size_t, std::size_t, std::size_t> dim() const { return {d0, d1, d2}; }
+};
+
+template <class F>
+struct Array3 {
+    std::size_t d0 = 0, d1 = 0, d2 = 0;
+    std::vector<F> data;
+    Array3() = default;
+    Array3(std::size_t a, std::size_t b, std::size_t c) : d0(a), d1(b), d2(c), data(a * b * c, F(0)) {}
+    ArrayView3<F> view() const
+    {
+        return {data.data(), d0, d1, d2, (std::ptrdiff_t)(d1 * d2), (std::ptrdiff_t)d2, 1};
+    }
+};
 
 template <class F>
 ArrayView1<F> view1(const std::vector<F> &v)
@@ -721,6 +748,132 @@ Result<LstsqResult<F>> lstsq(const ArrayView2<F> &a, const ArrayView1<F> &b, std
     if (rc > 0)
         return singular_matrix_with_suggestions("triangular system solve (back substitution)", {a.cols, a.cols}, 1e16);
     if (rc < 0) return detail::gpu_failure(rc);
+    return out;
+}
+
+// ------------------------------------------------------------------ batch (batch/mod.rs)
+namespace detail {
+template <class F>
+std::vector<F> standard3(const ArrayView3<F> &a)
+{
+    std::vector<F> out(a.d0 * a.d1 * a.d2);
+    std::size_t o = 0;
+    for (std::size_t p = 0; p < a.d0; ++p)
+        for (std::size_t i = 0; i < a.d1; ++i)
+            for (std::size_t j = 0; j < a.d2; ++j) out[o++] = a(p, i, j);
+    return out;
+}
+inline int batch_matmul(int nb, int m, int k, int n, const double *a, const double *b, double *c)
+{
+    return scirs_gpu_dbatch_matmul(nb, m, k, n, a, b, c, nullptr);
+}
+inline int batch_matmul(int nb, int m, int k, int n, const float *a, const float *b, float *c)
+{
+    return scirs_gpu_sbatch_matmul(nb, m, k, n, a, b, c, nullptr);
+}
+inline int batch_matvec(int nb, int m, int n, const double *a, const double *x, double *y)
+{
+    return scirs_gpu_dbatch_matvec(nb, m, n, a, x, y, nullptr);
+}
+inline int batch_matvec(int nb, int m, int n, const float *a, const float *x, float *y)
+{
+    return scirs_gpu_sbatch_matvec(nb, m, n, a, x, y, nullptr);
+}
+inline int batch_add(int nb, int m, int n, const double *a, const double *v, int axis, double *o)
+{
+    return scirs_gpu_dbatch_add(nb, m, n, a, v, axis, o, nullptr);
+}
+inline int batch_add(int nb, int m, int n, const float *a, const float *v, int axis, float *o)
+{
+    return scirs_gpu_sbatch_add(nb, m, n, a, v, axis, o, nullptr);
+}
+inline int batch_sum(int nb, int m, int n, const double *a, double *o)
+{
+    return scirs_gpu_dbatch_sum(nb, m, n, a, o, nullptr);
+}
+inline int batch_sum(int nb, int m, int n, const float *a, float *o)
+{
+    return scirs_gpu_sbatch_sum(nb, m, n, a, o, nullptr);
+}
+} // namespace detail
+
+// batch/mod.rs:62-107
+template <class F>
+Result<Array3<F>> batch_matmul(const ArrayView3<F> &batch_a, const ArrayView2<F> &b)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    const auto [nb, m, k1] = batch_a.dim();
+    if (k1 != b.rows)
+        return LinalgError(LinalgError::Kind::ShapeError, "Inner dimensions mismatch for batch_matmul: " +
+                                                              std::to_string(k1) + " vs " + std::to_string(b.rows));
+    Array3<F> out(nb, m, b.cols);
+    std::vector<F> ad = detail::standard3(batch_a), bd = detail::standard(b);
+    int rc = detail::batch_matmul((int)nb, (int)m, (int)k1, (int)b.cols, ad.data(), bd.data(), out.data.data());
+    if (rc < 0) return detail::gpu_failure(rc);
+    return out;
+}
+
+// batch/mod.rs:139-172
+template <class F>
+Result<Array2<F>> batch_matvec(const ArrayView3<F> &batch_a, const ArrayView1<F> &x)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    const auto [nb, m, n] = batch_a.dim();
+    if (n != x.len)
+        return LinalgError(LinalgError::Kind::ShapeError,
+                           "Dimension mismatch for batch_matvec: matrix width " + std::to_string(n) +
+                               " does not match vector length " + std::to_string(x.len));
+    Array2<F> out(nb, m);
+    std::vector<F> ad = detail::standard3(batch_a), xd(x.len);
+    for (std::size_t j = 0; j < x.len; ++j) xd[j] = x[j];
+    int rc = detail::batch_matvec((int)nb, (int)m, (int)n, ad.data(), xd.data(), out.data.data());
+    if (rc < 0) return detail::gpu_failure(rc);
+    return out;
+}
+
+// batch/mod.rs:220-291
+template <class F>
+Result<Array3<F>> batch_add(const ArrayView3<F> &batch_a, const ArrayView1<F> &v, std::size_t axis)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    const auto [nb, m, n] = batch_a.dim();
+    if (axis == 0) {
+        if (v.len != m)
+            return LinalgError(LinalgError::Kind::ShapeError,
+                               "Column-wise batch_add requires vector length " + std::to_string(m) +
+                                   " to match number of rows " + std::to_string(m) + ", got " + std::to_string(v.len));
+    } else if (axis == 1) {
+        if (v.len != n)
+            return LinalgError(LinalgError::Kind::ShapeError,
+                               "Row-wise batch_add requires vector length " + std::to_string(n) +
+                                   " to match number of columns " + std::to_string(n) + ", got " +
+                                   std::to_string(v.len));
+    } else {
+        return LinalgError(LinalgError::Kind::InvalidInputError,
+                           "Invalid axis " + std::to_string(axis) + ": must be 0 (column-wise) or 1 (row-wise)");
+    }
+    Array3<F> out(nb, m, n);
+    std::vector<F> ad = detail::standard3(batch_a), vd(v.len);
+    for (std::size_t j = 0; j < v.len; ++j) vd[j] = v[j];
+    int rc = detail::batch_add((int)nb, (int)m, (int)n, ad.data(), vd.data(), (int)axis, out.data.data());
+    if (rc < 0) return detail::gpu_failure(rc);
+    return out;
+}
+
+// batch/mod.rs:326-336 (infallible in the reference; a device failure aborts
+// here, as no error can be returned)
+template <class F>
+Array2<F> batch_sum(const ArrayView3<F> &batch_a)
+{
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
+    const auto [nb, m, n] = batch_a.dim();
+    Array2<F> out(m, n);
+    std::vector<F> ad = detail::standard3(batch_a);
+    int rc = detail::batch_sum((int)nb, (int)m, (int)n, ad.data(), out.data.data());
+    if (rc < 0) {
+        std::fprintf(stderr, "batch_sum: GPU failure %d: %s\n", rc, scirs_gpu_last_error());
+        std::abort();
+    }
     return out;
 }
 

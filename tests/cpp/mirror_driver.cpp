@@ -4,7 +4,9 @@
 // Input, whitespace separated, one case after another:
 //   <op> <type> <flag1> <flag2> <ar> <ac> <ar*ac values> <br> <bc> <br*bc values>
 //   op   : solve | solve_multiple | lu | cholesky | solve_triangular | det | inv |
-//          mixed_precision_solve | iterative_refinement_solve | compat_solve | qr | lstsq
+//          mixed_precision_solve | iterative_refinement_solve | compat_solve | qr | lstsq |
+//          batch_matmul | batch_matvec | batch_add | batch_sum (flag1 = batch count,
+//          A given as (batch*m) x k; flag2 = axis of batch_add)
 //   type : f64 | f32
 //   flags: lower / unit_diagonal for solve_triangular, ignored otherwise
 //   b    : the right-hand side (br = length, bc = 1 for a vector; 0 0 when unused)
@@ -133,6 +135,30 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
         if (r.is_err()) return print_err(r.error());
         std::printf("OK 1");
         put(r.value().rows, r.value().cols, r.value().data.data());
+    } else if (op == "batch_matmul" || op == "batch_matvec" || op == "batch_add" || op == "batch_sum") {
+        // f1 = batch count: A is given as (batch*m) x k; f2 = axis for batch_add
+        const std::size_t nb = (std::size_t)f1, m = nb ? ar / nb : 0;
+        ArrayView3<F> a3{a.data.data(), nb, m, ac, (std::ptrdiff_t)(m * ac), (std::ptrdiff_t)ac, 1};
+        if (op == "batch_matmul") {
+            auto r = batch_matmul<F>(a3, b.view());
+            if (r.is_err()) return print_err(r.error());
+            std::printf("OK 1");
+            put(r.value().d0 * r.value().d1, r.value().d2, r.value().data.data());
+        } else if (op == "batch_matvec") {
+            auto r = batch_matvec<F>(a3, view1(bvec));
+            if (r.is_err()) return print_err(r.error());
+            std::printf("OK 1");
+            put(r.value().rows, r.value().cols, r.value().data.data());
+        } else if (op == "batch_add") {
+            auto r = batch_add<F>(a3, view1(bvec), (std::size_t)f2);
+            if (r.is_err()) return print_err(r.error());
+            std::printf("OK 1");
+            put(r.value().d0 * r.value().d1, r.value().d2, r.value().data.data());
+        } else {
+            auto r = batch_sum<F>(a3);
+            std::printf("OK 1");
+            put(r.rows, r.cols, r.data.data());
+        }
     } else if (op == "qr" || op == "lstsq") {
         if constexpr (std::is_same<F, double>::value) {
             if (op == "qr") {

@@ -487,3 +487,50 @@ def test_qr_lstsq_results():
     assert np.allclose(got[2][1][0][:, 0], [3.0, 3.0], atol=1e-12)
     assert got[3] == ("ERR", "SingularMatrixError",
                       singular("triangular system solve (back substitution)", 3, 3, "1.00e16"))
+
+
+def test_batch_validation_texts():
+    """batch/mod.rs:70-75, 147-152, 230-262: shape and axis errors keep the
+    reference's kinds and texts (no device work)."""
+    a = np.ones((4, 3))   # two 2 x 3 matrices
+    got = run([
+        ("batch_matmul", "f64", 2, 0, a, np.ones((2, 2))),
+        ("batch_matvec", "f64", 2, 0, a, np.ones(2)),
+        ("batch_add", "f64", 2, 0, a, np.ones(3)),
+        ("batch_add", "f64", 2, 1, a, np.ones(2)),
+        ("batch_add", "f64", 2, 2, a, np.ones(3)),
+    ])
+    assert got[0] == ("ERR", "ShapeError", "Shape error: Inner dimensions mismatch for batch_matmul: 3 vs 2")
+    assert got[1] == ("ERR", "ShapeError",
+                      "Shape error: Dimension mismatch for batch_matvec: matrix width 3 does not match vector length 2")
+    assert got[2] == ("ERR", "ShapeError",
+                      "Shape error: Column-wise batch_add requires vector length 2 to match number of rows 2, got 3")
+    assert got[3] == ("ERR", "ShapeError",
+                      "Shape error: Row-wise batch_add requires vector length 3 to match number of columns 3, got 2")
+    assert got[4] == ("ERR", "InvalidInputError",
+                      "Invalid input error: Invalid axis 2: must be 0 (column-wise) or 1 (row-wise)")
+
+
+@pytest.mark.gpu
+def test_batch_results_bit_identical():
+    """The mirror's batch operations return the oracle's bits (batch/mod.rs:62-336)."""
+    from oracle import oracle
+    rng = np.random.default_rng(23)
+    a = rng.standard_normal((3, 17, 29))
+    b = rng.standard_normal((29, 13))
+    x = rng.standard_normal(29)
+    vr = rng.standard_normal(17)
+    flat = a.reshape(3 * 17, 29)
+    got = run([
+        ("batch_matmul", "f64", 3, 0, flat, b),
+        ("batch_matvec", "f64", 3, 0, flat, x),
+        ("batch_add", "f64", 3, 0, flat, vr),
+        ("batch_add", "f64", 3, 1, flat, x),
+        ("batch_sum", "f64", 3, 0, flat, None),
+    ])
+    assert all(g[0] == "OK" for g in got), got
+    assert np.array_equal(got[0][1][0], oracle.batch_matmul(a, b).reshape(51, 13))
+    assert np.array_equal(got[1][1][0], oracle.batch_matvec(a, x))
+    assert np.array_equal(got[2][1][0], oracle.batch_add(a, vr, 0).reshape(51, 29))
+    assert np.array_equal(got[3][1][0], oracle.batch_add(a, x, 1).reshape(51, 29))
+    assert np.array_equal(got[4][1][0], oracle.batch_sum(a))
