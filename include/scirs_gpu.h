@@ -124,6 +124,19 @@ int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, double *q, i
 int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda, const double *b, double *x,
                      double *residual, scirs_solve_info *info);
 
+/* ---- Krylov ------------------------------------------------------------------- */
+/* Replaces scirs2-linalg/src/solvers/iterative.rs:317-450 `gmres` for f64:
+ * restarted GMRES with the reference's classical Gram-Schmidt, breakdown test
+ * (h_{j+1,j} < eps ends the cycle without counting an iteration), cycle count
+ * (max_iter / restart).max(1) and Givens least-squares solve (:604-664).
+ * x0 may be NULL (zeros); restart <= 0 means min(n, 30).  Outputs the solution,
+ * the iteration count, ||b - A x||_2 and whether it fell below tol.
+ * Returns 0 (converged or not), or k > 0 when the least-squares solve meets
+ * |r_kk| < eps (the reference's SingularMatrixError). */
+int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const double *b, const double *x0, int max_iter,
+                     double tol, int restart, double *x, int *iterations, double *residual_norm, int *converged,
+                     scirs_solve_info *info);
+
 /* ---- batched array operations ------------------------------------------------ */
 /* Replace scirs2-linalg/src/batch/mod.rs for f64 / f32, bit-identical to its
  * loops (each element summed from +0 in the reference's order, no fused

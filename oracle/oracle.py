@@ -14,6 +14,7 @@ scirs2-linalg's CPU solve path:
 * ``lstsq``            -> scirs2-linalg/src/solve.rs:234-290, rows >= cols (f64)
 * ``batch_matmul``, ``batch_matvec``, ``batch_add``, ``batch_sum``
                        -> scirs2-linalg/src/batch/mod.rs:62-336
+* ``gmres``            -> scirs2-linalg/src/solvers/iterative.rs:317-450 (f64)
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import this module; it is the checker, never the product.
@@ -68,6 +69,7 @@ def _load():
         getattr(lib, f"oracle_batch_matvec_{s}").argtypes = [I, I, I, P, P, P]
         getattr(lib, f"oracle_batch_add_{s}").argtypes = [I, I, I, P, P, I, P]
         getattr(lib, f"oracle_batch_sum_{s}").argtypes = [I, I, I, P, P]
+    lib.oracle_gmres_d.argtypes = [I, P, L, P, P, I, ctypes.c_double, I, P, P, P, P]
     lib.oracle_qr_factor_d.argtypes = [I, I, P, L, P, P]
     lib.oracle_lstsq_d.argtypes = [I, I, P, L, P, P, P]
     _lib = lib
@@ -285,3 +287,25 @@ def batch_sum(a):
     out = np.empty((m, n), dtype=dt)
     getattr(_load(), f"oracle_batch_sum_{_sfx(dt)}")(p, m, n, _ptr(a), _ptr(out))
     return out
+
+
+def gmres(a, b, x0=None, max_iter=1000, tol=1e-10, restart=0):
+    """solvers/iterative.rs:317-450 (f64): (x, iterations, residual_norm, converged).
+
+    Raises OracleError('singular', k) where the least-squares solve of the
+    Hessenberg system meets |r_kk| < eps."""
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    n = a.shape[0]
+    x = np.empty(n)
+    it = np.zeros(1, dtype=np.int32)
+    conv = np.zeros(1, dtype=np.int32)
+    res = np.zeros(1)
+    x0p = None if x0 is None else _c(x0, np.float64)
+    rc = _load().oracle_gmres_d(n, _ptr(a), n, _ptr(b), None if x0p is None else _ptr(x0p), int(max_iter),
+                                float(tol), int(restart), _ptr(x), _ptr(it), _ptr(res), _ptr(conv))
+    if rc > 0:
+        raise OracleError("singular", rc)
+    if rc < 0:
+        raise OracleError("memory", rc)
+    return x, int(it[0]), float(res[0]), bool(conv[0])

@@ -19,6 +19,7 @@
  *   oracle_qr_factor_d        scirs2-linalg/src/lapack.rs:171-253
  *   oracle_lstsq_d            scirs2-linalg/src/solve.rs:234-290 (rows >= cols)
  *   oracle_batch_*            scirs2-linalg/src/batch/mod.rs:62-336
+ *   oracle_gmres_d            scirs2-linalg/src/solvers/iterative.rs:317-450, 604-664
  *
  * All matrices are dense row-major with an explicit leading dimension, the
  * layout of an ndarray in standard order.
@@ -369,6 +370,133 @@ void oracle_batch_sum_##S(int batch, int m, int n, const T *a, T *out)          
 
 ORACLE_BATCH(double, d)
 ORACLE_BATCH(float, s)
+
+/* solvers/iterative.rs:604-664 (solve_least_squares_gmres): Givens rotations
+ * on the (m x nc) Hessenberg h (row-major, ld nc), then back substitution.
+ * Returns 0, or i+1 when |r_ii| < eps (the reference's SingularMatrixError). */
+static int oracle_gmres_lsq(int m, int nc, const double *h, double beta, double *y)
+{
+    double *hc = (double *)malloc(sizeof(double) * (size_t)m * (nc > 0 ? nc : 1));
+    double *g = (double *)calloc((size_t)m, sizeof(double));
+    if (!hc || !g) { free(hc); free(g); return -3; }
+    memcpy(hc, h, sizeof(double) * (size_t)m * nc);
+    g[0] = beta;
+    for (int j = 0; j < nc; ++j)
+        for (int i = j + 1; i < m; ++i) {
+            if (fabs(hc[i * nc + j]) > DBL_EPSILON) {
+                const double a = hc[j * nc + j], b = hc[i * nc + j];
+                const double r = sqrt(a * a + b * b);
+                const double c = a / r, s = b / r;
+                for (int k = j; k < nc; ++k) {
+                    const double hjk = hc[j * nc + k], hik = hc[i * nc + k];
+                    hc[j * nc + k] = c * hjk + s * hik;
+                    hc[i * nc + k] = -s * hjk + c * hik;
+                }
+                const double gj = g[j], gi = g[i];
+                g[j] = c * gj + s * gi;
+                g[i] = -s * gj + c * gi;
+            }
+        }
+    int rc = 0;
+    for (int i = nc - 1; i >= 0; --i) {
+        double sum = g[i];
+        for (int j = i + 1; j < nc; ++j) sum -= hc[i * nc + j] * y[j];
+        if (fabs(hc[i * nc + i]) < DBL_EPSILON) { rc = i + 1; break; }
+        y[i] = sum / hc[i * nc + i];
+    }
+    free(hc); free(g);
+    return rc;
+}
+
+static double oracle_resid_norm(int n, const double *a, long lda, const double *b, const double *x, double *r)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double ax = 0.0;
+        for (int j = 0; j < n; ++j) ax += a[(size_t)i * lda + j] * x[j];
+        r[i] = b[i] - ax;
+    }
+    for (int i = 0; i < n; ++i) s += r[i] * r[i];
+    return sqrt(s);
+}
+
+/* solvers/iterative.rs:317-450 (gmres), f64: restarted GMRES with classical
+ * Gram-Schmidt against the unmodified w, a break (counting no iteration) when
+ * h[j+1][j] < eps, (max_iterations / restart).max(1) cycles.  x0 may be NULL
+ * (zeros); restart <= 0 means min(n, 30).  Returns 0, or k+1 > 0 when the
+ * least-squares solve is singular at row k+1. */
+int oracle_gmres_d(int n, const double *a, long lda, const double *b, const double *x0, int max_iter, double tol,
+                   int restart, double *x, int *iterations, double *residual_norm, int *converged)
+{
+    if (restart <= 0) restart = n < 30 ? n : 30;
+    double *v = (double *)calloc((size_t)(restart + 1) * n, sizeof(double));
+    double *h = (double *)calloc((size_t)(restart + 1) * restart, sizeof(double));
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    double *wo = (double *)malloc(sizeof(double) * (size_t)n);
+    double *r = (double *)malloc(sizeof(double) * (size_t)n);
+    double *y = (double *)malloc(sizeof(double) * (size_t)(restart + 1));
+    if (!v || !h || !w || !wo || !r || !y) { free(v); free(h); free(w); free(wo); free(r); free(y); return -3; }
+    for (int i = 0; i < n; ++i) x[i] = x0 ? x0[i] : 0.0;
+    int total = 0, rc = 0;
+    int cycles = max_iter / restart;
+    if (cycles < 1) cycles = 1;
+    *converged = 0;
+    for (int outer = 0; outer < cycles; ++outer) {
+        const double beta = oracle_resid_norm(n, a, lda, b, x, r);
+        if (beta < tol) {
+            *iterations = total; *residual_norm = beta; *converged = 1;
+            goto done;
+        }
+        for (int i = 0; i < n; ++i) v[i] = r[i] / beta;
+        memset(h, 0, sizeof(double) * (size_t)(restart + 1) * restart);
+        int j = 0;
+        while (j < restart && total < max_iter) {
+            const double *vj = v + (size_t)j * n;
+            for (int i = 0; i < n; ++i) {
+                double s = 0.0;
+                for (int q = 0; q < n; ++q) s += a[(size_t)i * lda + q] * vj[q];
+                w[i] = s;
+            }
+            memcpy(wo, w, sizeof(double) * (size_t)n);
+            for (int i = 0; i <= j; ++i) {
+                const double *vi = v + (size_t)i * n;
+                double d = 0.0;
+                for (int q = 0; q < n; ++q) d += w[q] * vi[q];
+                h[i * restart + j] = d;
+                for (int q = 0; q < n; ++q) wo[q] += -d * vi[q];
+            }
+            double nn = 0.0;
+            for (int q = 0; q < n; ++q) nn += wo[q] * wo[q];
+            h[(j + 1) * restart + j] = sqrt(nn);
+            if (h[(j + 1) * restart + j] < DBL_EPSILON) { j += 1; break; }
+            for (int q = 0; q < n; ++q) v[(size_t)(j + 1) * n + q] = wo[q] / h[(j + 1) * restart + j];
+            j += 1;
+            total += 1;
+        }
+        {
+            /* the leading (j+1) x j block of h, packed with ld j */
+            double *hp = (double *)malloc(sizeof(double) * (size_t)(j + 1) * (j > 0 ? j : 1));
+            if (!hp) { rc = -3; goto done; }
+            for (int i = 0; i <= j; ++i)
+                for (int c = 0; c < j; ++c) hp[i * j + c] = h[i * restart + c];
+            rc = j > 0 ? oracle_gmres_lsq(j + 1, j, hp, beta, y) : 0;
+            free(hp);
+            if (rc) goto done;
+        }
+        for (int i = 0; i < j; ++i)
+            for (int q = 0; q < n; ++q) x[q] += y[i] * v[(size_t)i * n + q];
+        const double rn = oracle_resid_norm(n, a, lda, b, x, r);
+        if (rn < tol) {
+            *iterations = total; *residual_norm = rn; *converged = 1;
+            goto done;
+        }
+    }
+    *iterations = total;
+    *residual_norm = oracle_resid_norm(n, a, lda, b, x, r);
+done:
+    free(v); free(h); free(w); free(wo); free(r); free(y);
+    return rc;
+}
 
 /* Version tag so a test can tell that the built library matches this source. */
 int oracle_version(void) { return 1; }

@@ -145,6 +145,25 @@ def lstsq(a, b):
     return x, res.value
 
 
+def gmres(a, b, x0=None, max_iter=1000, tol=1e-10, restart=0):
+    """Restarted GMRES as solvers/iterative.rs:317-450: (x, iterations, residual_norm, converged)."""
+    lib = N.load()
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    n = a.shape[0]
+    x = np.empty(n)
+    it = ctypes.c_int(0)
+    conv = ctypes.c_int(0)
+    res = ctypes.c_double(0.0)
+    x0p = None if x0 is None else _c(x0, np.float64)
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dgmres(n, _ptr(a), n, _ptr(b), None if x0p is None else _ptr(x0p), int(max_iter),
+                              float(tol), int(restart), _ptr(x), ctypes.byref(it), ctypes.byref(res),
+                              ctypes.byref(conv), ctypes.byref(info))
+    _check("gmres", rc, info)
+    return x, it.value, res.value, bool(conv.value)
+
+
 def _batch_args(a):
     a = np.asarray(a)
     dt = np.float32 if a.dtype == np.float32 else np.float64

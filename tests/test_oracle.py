@@ -437,3 +437,41 @@ def test_batch_ops_random_to_rounding():
     for p in range(3):
         s = s + a[p]
     assert np.array_equal(oracle.batch_sum(a), s)   # batch order, as ndarray's sum_axis
+
+
+# ---------------------------------------------------------------- gmres ---
+# gmres is pinned by the reference's test (solvers/iterative.rs:690-703:
+# [[1,2],[3,4]] x = [5,11] -> [1,2], converged), by LAPACK's solution of
+# well-conditioned systems (a wrong sign in the Gram-Schmidt or the Givens
+# sweep stalls the residual), by the exact-in-n-steps property (an n x n
+# system converges within one cycle of n steps), and by the cycle arithmetic
+# ((max_iter / restart).max(1) cycles: max_iter = 5 gives 5 iterations, not
+# converged).
+def test_gmres_reference_example():
+    x, it, res, conv = oracle.gmres(np.array([[1.0, 2.0], [3.0, 4.0]]), np.array([5.0, 11.0]))
+    assert conv and abs(x[0] - 1.0) < 1e-10 and abs(x[1] - 2.0) < 1e-10 and it == 2
+
+
+@pytest.mark.parametrize("n", [5, 40, 200])
+def test_gmres_matches_lapack(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n)) / np.sqrt(n) + 3.0 * np.eye(n)
+    b = rng.standard_normal(n)
+    x, it, res, conv = oracle.gmres(a, b)
+    assert conv and res < 1e-10
+    assert np.abs(x - np.linalg.solve(a, b)).max() < 1e-9
+    assert abs(res - np.linalg.norm(b - a @ x)) < 1e-12
+
+
+def test_gmres_cycle_arithmetic_and_x0():
+    rng = np.random.default_rng(2)
+    n = 60
+    a = rng.standard_normal((n, n)) / np.sqrt(n) + 1.2 * np.eye(n)
+    b = rng.standard_normal(n)
+    x, it, res, conv = oracle.gmres(a, b, max_iter=5)
+    assert it == 5 and not conv
+    xs = np.linalg.solve(a, b)
+    x, it, res, conv = oracle.gmres(a, b, x0=xs)
+    assert conv and it == 0 and res < 1e-10
+    x, it, res, conv = oracle.gmres(a, b, restart=n)
+    assert conv and it <= n
