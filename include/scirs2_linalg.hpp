@@ -14,6 +14,7 @@
 //                    scirs2-linalg/src/mixed_precision/adaptive.rs:49-151, 282-378
 //   compat_solve     scirs2-linalg/src/compat.rs:461-499
 //   qr, lstsq        scirs2-linalg/src/decomposition.rs:208-249, solve.rs:234-290 (rows >= cols)
+//   gmres            scirs2-linalg/src/solvers/iterative.rs:317-450
 //   batch_matmul, batch_matvec, batch_add, batch_sum
 //                    scirs2-linalg/src/batch/mod.rs:62-336
 //   validation       scirs2-linalg/src/validation.rs:22-329
@@ -748,6 +749,66 @@ Result<LstsqResult<F>> lstsq(const ArrayView2<F> &a, const ArrayView1<F> &b, std
     if (rc > 0)
         return singular_matrix_with_suggestions("triangular system solve (back substitution)", {a.cols, a.cols}, 1e16);
     if (rc < 0) return detail::gpu_failure(rc);
+    return out;
+}
+
+// ------------------------------------------------------------------ Krylov
+// solvers/iterative.rs:13-50 (IterativeSolverOptions, IterativeSolverResult)
+template <class A>
+struct IterativeSolverOptions {
+    std::size_t max_iterations = 1000;
+    A tolerance = A(1e-10);
+    bool verbose = false;
+    std::optional<std::size_t> restart;
+};
+
+template <class A>
+struct IterativeSolverResult {
+    Array1<A> solution;
+    std::size_t iterations;
+    A residual_norm;
+    bool converged;
+};
+
+// solvers/iterative.rs:317-450 (gmres), f64 on the GPU
+template <class A>
+Result<IterativeSolverResult<A>> gmres(const ArrayView2<A> &a, const ArrayView1<A> &b, const ArrayView1<A> *x0,
+                                       const IterativeSolverOptions<A> &options)
+{
+    static_assert(std::is_same<A, double>::value, "the GPU GMRES handles f64");
+    const std::size_t n = a.rows;
+    if (a.cols != n) return LinalgError(LinalgError::Kind::ShapeError, "Matrix A must be square");
+    if (b.len != n)
+        return LinalgError(LinalgError::Kind::ShapeError, "Vector b must have same length as matrix dimension");
+    std::vector<A> x0d;
+    if (x0) {
+        if (x0->len != n)
+            return LinalgError(LinalgError::Kind::ShapeError, "Initial guess x0 must have same length as b");
+        x0d.resize(n);
+        for (std::size_t i = 0; i < n; ++i) x0d[i] = (*x0)[i];
+    }
+    IterativeSolverResult<A> out{Array1<A>(n), 0, A(0), false};
+    if (n == 0) {
+        // the reference divides by restart = min(n, 30) = 0 here and panics;
+        // this returns the empty solution instead
+        out.converged = A(0) < options.tolerance;
+        return out;
+    }
+    std::vector<A> ad = detail::standard(a), bd(n);
+    for (std::size_t i = 0; i < n; ++i) bd[i] = b[i];
+    int it = 0, conv = 0;
+    double res = 0.0;
+    int rc = scirs_gpu_dgmres((int)n, ad.data(), (int64_t)n, bd.data(), x0 ? x0d.data() : nullptr,
+                              (int)options.max_iterations, (double)options.tolerance,
+                              options.restart ? (int)*options.restart : 0, out.solution.data(), &it, &res, &conv,
+                              nullptr);
+    if (rc > 0)
+        return LinalgError(LinalgError::Kind::SingularMatrixError, "Matrix is singular in GMRES least squares solve");
+    if (rc < 0) return detail::gpu_failure(rc);
+    if (options.verbose) std::printf("GMRES: residual = %g after %d iterations\n", res, it);
+    out.iterations = (std::size_t)it;
+    out.residual_norm = (A)res;
+    out.converged = conv != 0;
     return out;
 }
 

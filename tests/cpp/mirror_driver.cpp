@@ -135,6 +135,25 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
         if (r.is_err()) return print_err(r.error());
         std::printf("OK 1");
         put(r.value().rows, r.value().cols, r.value().data.data());
+    } else if (op == "gmres") {
+        // f1 = max_iterations (0: default 1000), f2 = restart (0: default); b = right-hand side
+        if constexpr (std::is_same<F, double>::value) {
+            IterativeSolverOptions<F> opt;
+            if (f1 > 0) opt.max_iterations = (std::size_t)f1;
+            if (f2 > 0) opt.restart = (std::size_t)f2;
+            auto r = gmres<F>(a.view(), view1(bvec), nullptr, opt);
+            if (r.is_err()) return print_err(r.error());
+            const auto &v = r.value();
+            const double itv = (double)v.iterations, cv = v.converged ? 1.0 : 0.0;
+            std::printf("OK 4");
+            put(v.solution.size(), 1, v.solution.data());
+            put(1, 1, &itv);
+            put(1, 1, &v.residual_norm);
+            put(1, 1, &cv);
+        } else {
+            std::printf("ERR Other gmres is f64 only\n");
+            return;
+        }
     } else if (op == "batch_matmul" || op == "batch_matvec" || op == "batch_add" || op == "batch_sum") {
         // f1 = batch count: A is given as (batch*m) x k; f2 = axis for batch_add
         const std::size_t nb = (std::size_t)f1, m = nb ? ar / nb : 0;

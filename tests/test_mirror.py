@@ -534,3 +534,32 @@ def test_batch_results_bit_identical():
     assert np.array_equal(got[2][1][0], oracle.batch_add(a, vr, 0).reshape(51, 29))
     assert np.array_equal(got[3][1][0], oracle.batch_add(a, x, 1).reshape(51, 29))
     assert np.array_equal(got[4][1][0], oracle.batch_sum(a))
+
+
+def test_gmres_validation_texts():
+    """gmres (solvers/iterative.rs:327-350): shape errors keep the reference's texts."""
+    got = run([
+        ("gmres", "f64", 0, 0, np.ones((2, 3)), np.ones(2)),
+        ("gmres", "f64", 0, 0, np.eye(3), np.ones(2)),
+    ])
+    assert got[0] == ("ERR", "ShapeError", "Shape error: Matrix A must be square")
+    assert got[1] == ("ERR", "ShapeError", "Shape error: Vector b must have same length as matrix dimension")
+
+
+@pytest.mark.gpu
+def test_gmres_results():
+    """gmres through the mirror: the oracle's iteration count, flag and x."""
+    from oracle import oracle
+    rng = np.random.default_rng(31)
+    n = 80
+    a = rng.standard_normal((n, n)) / np.sqrt(n) + 2.0 * np.eye(n)
+    b = rng.standard_normal(n)
+    got = run([("gmres", "f64", 0, 0, a, b), ("gmres", "f64", 7, 0, a, b), ("gmres", "f64", 0, 0,
+                                                                              [[1.0, 2.0], [3.0, 4.0]], [5.0, 11.0])])
+    for g, kw in ((got[0], {}), (got[1], {"max_iter": 7})):
+        assert g[0] == "OK", g
+        x, it, res, conv = g[1]
+        xo, ito, reso, convo = oracle.gmres(a, b, **kw)
+        assert int(it[0, 0]) == ito and bool(conv[0, 0]) == convo
+        assert np.abs(x[:, 0] - xo).max() <= 1e-9 * max(1.0, np.abs(xo).max())
+    assert np.allclose(got[2][1][0][:, 0], [1.0, 2.0], atol=1e-10)
