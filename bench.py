@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 LU comparison line")
     ap.add_argument("--cpu-n", type=int, default=4096, help="size of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "batched", "fp64"],
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "batched", "fp64", "next"],
                     help="mixed: the headline n=16384 mixed-precision solve; batched: 20,000 uneven "
                          "systems in one launch against a per-matrix launch loop")
     ap.add_argument("--batch", type=int, default=20000)
@@ -299,6 +299,83 @@ def bench_fp64(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_next(args):
+    """The widened rows beside the solve path: batched matmul (bit-identical to
+    batch/mod.rs), Householder QR / lstsq and restarted GMRES, each timed on the
+    GPU and, on a small sample, through the oracle's loops on one core."""
+    import ctypes
+    import numpy as np
+    import torch
+    from scirs_amd import _native as N
+    from scirs_amd import linalg as L
+    lib = N.load()
+    out = {"metric": "widened rows: batched matmul, Householder QR / lstsq, GMRES", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "data": "synthetic", "seed": args.seed}
+    # batched matmul, device-resident, HIP events on the stream it runs on
+    nb, m, k, nn = 64, 512, 512, 512
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    a = torch.randn(nb, m, k, dtype=torch.float64, device="cuda", generator=g)
+    b = torch.randn(k, nn, dtype=torch.float64, device="cuda", generator=g)
+    c = torch.empty(nb, m, nn, dtype=torch.float64, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def mm():
+        rc = lib.scirs_gpu_dbatch_matmul_dev(nb, m, k, nn, ctypes.c_void_p(a.data_ptr()),
+                                             ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(c.data_ptr()), st)
+        assert rc == 0, rc
+    for _ in range(max(1, args.warmup)):
+        mm()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        mm()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    fl = 2.0 * nb * m * k * nn
+    out["batch_matmul"] = {"shape": [nb, m, k, nn], "dtype": "f64", "ms": ms, "tflops": fl / ms / 1e9,
+                           "roofline": {"bound": "fp64 vector ALU, no FMA", "peak": 78.6 / 2, "unit": "TFLOP/s",
+                                        "frac": fl / ms / 1e9 / (78.6 / 2)}}
+    # QR and lstsq through the host entry points (copies included)
+    rng = np.random.default_rng(args.seed)
+    for name, shape in (("qr", (4096, 4096)), ("lstsq", (8192, 2048))):
+        A = rng.standard_normal(shape)
+        B = rng.standard_normal(shape[0])
+        fn = (lambda: L.qr(A)) if name == "qr" else (lambda: L.lstsq(A, B))
+        fn()
+        t0 = time.perf_counter()
+        fn()
+        out[name] = {"shape": list(shape), "ms_host_entry": (time.perf_counter() - t0) * 1e3}
+    # GMRES, n = 8192, default options (restart 30)
+    n = 8192
+    A = rng.standard_normal((n, n)) / np.sqrt(n) + 2.0 * np.eye(n)
+    B = rng.standard_normal(n)
+    L.gmres(A[:64, :64], B[:64])
+    t0 = time.perf_counter()
+    x, it, res, conv = L.gmres(A, B)
+    out["gmres"] = {"n": n, "ms_host_entry": (time.perf_counter() - t0) * 1e3, "iterations": it,
+                    "residual_norm": res, "converged": conv}
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        A = rng.standard_normal((2, 512, 512))
+        Bm = rng.standard_normal((512, 512))
+        t0 = time.perf_counter()
+        oracle.batch_matmul(A, Bm)
+        dt = time.perf_counter() - t0
+        A2 = rng.standard_normal((1024, 1024))
+        t1 = time.perf_counter()
+        oracle.qr_factor(A2)
+        dq = time.perf_counter() - t1
+        t0 = time.perf_counter()
+        L.qr(A2)
+        gq = time.perf_counter() - t0
+        out["cpu_baseline"] = {"kind": "port", "cores": 1,
+                               "batch_matmul_gflops": 2.0 * 2 * 512 ** 3 / dt / 1e9, "sample": "2 x 512^3 f64",
+                               "qr_1024_ms": dq * 1e3, "gpu_qr_1024_ms_host_entry": gq * 1e3}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -310,6 +387,9 @@ def main():
         return
     if args.workload == "fp64":
         bench_fp64(args)
+        return
+    if args.workload == "next":
+        bench_next(args)
         return
     import torch
     from scirs_amd import linalg as L

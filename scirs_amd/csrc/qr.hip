@@ -153,6 +153,22 @@ __global__ __launch_bounds__(256) void resid_sq_kernel(const double *__restrict_
     }
 }
 
+// out[i] = M[i * stride], i < count (a column or the diagonal of a row-major matrix)
+__global__ __launch_bounds__(256) void strided_gather_kernel(const double *__restrict__ M, long stride, int count,
+                                                             double *__restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) out[i] = M[(long)i * stride];
+}
+
+// M[i * stride] = in[i], i < count
+__global__ __launch_bounds__(256) void strided_scatter_kernel(const double *__restrict__ in, int count,
+                                                              double *__restrict__ M, long stride)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) M[(long)i * stride] = in[i];
+}
+
 __global__ __launch_bounds__(HT) void sum_kernel(const double *__restrict__ sq, int n, double *__restrict__ out)
 {
     __shared__ double red[HT / 64];
@@ -247,25 +263,28 @@ extern "C" int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda
     int *dflag = (int *)(dres + 1);
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
                                      hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR + cols, pr * sizeof(double), b, sizeof(double), sizeof(double), rows,
-                                     hipMemcpyHostToDevice, s));
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, pa * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
                                      hipMemcpyHostToDevice, s));
     SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, rows * sizeof(double), hipMemcpyHostToDevice, s));
+    // b into the extra column; strided columns move by kernels, only whole
+    // contiguous vectors cross PCIe
+    strided_scatter_kernel<<<(rows + 255) / 256, 256, 0, s>>>(db, rows, dR + cols, pr);
+    SCIRS_HIP_CHECK(hipGetLastError());
     if ((rc = householder_sweep(rows, cols, cols + 1, dR, pr, nullptr, 0, dv, dflag, s))) return finish(info, rc, t0);
     // the back substitution's |r_ii| < eps test (solve.rs:184-191), first row
     // in substitution order (bottom up)
     std::vector<double> diag(cols);
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(diag.data(), sizeof(double), dR, (pr + 1) * sizeof(double), sizeof(double), cols,
-                                     hipMemcpyDeviceToHost, s));
+    strided_gather_kernel<<<(cols + 255) / 256, 256, 0, s>>>(dR, pr + 1, cols, dsq);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    SCIRS_HIP_CHECK(hipMemcpyAsync(diag.data(), dsq, cols * sizeof(double), hipMemcpyDeviceToHost, s));
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     for (int i = cols - 1; i >= 0; --i)
         if (fabs(diag[i]) < DBL_EPSILON) {
             if (info) info->pivot_value = fabs(diag[i]);
             return finish(info, i + 1, t0);
         }
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dy, sizeof(double), dR + cols, pr * sizeof(double), sizeof(double), cols,
-                                     hipMemcpyDeviceToDevice, s));
+    strided_gather_kernel<<<(cols + 255) / 256, 256, 0, s>>>(dR + cols, pr, cols, dy);
+    SCIRS_HIP_CHECK(hipGetLastError());
     if ((rc = c.ensure_xtri(cols))) return finish(info, rc, t0);
     if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
     if ((rc = scirs_trsv_chain<double>(cols, 1, dR, pr, dy, 1, 0, 0, c.work, s))) return finish(info, rc, t0);
