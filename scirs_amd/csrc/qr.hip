@@ -9,13 +9,13 @@
 // normalised to unit length and H = I - 2 v v^T is applied to the trailing
 // columns of R and to the rows k.. of the accumulated Q^T; a column whose
 // ||x|| or ||v|| is not above machine epsilon is left as it is.  Per column
-// two launches: one workgroup forms v (two block reductions in a fixed order),
-// then every column of the trailing block is owned by one thread column of an
-// apply workgroup, which forms v^T c over its rows and subtracts 2 v (v^T c).
-// Row-major storage makes each 32-column strip of a row a 256-byte coalesced
-// segment.  This is the unblocked (level-2) form: bound by one pass over the
-// trailing block per column; the blocked (compact-WY, MFMA) form is the next
-// step for large matrices.
+// three launches: one workgroup forms v (two block reductions in a fixed
+// order); then the trailing block, cut into 32-column strips x 256-row chunks,
+// forms per-chunk partials of v^T c and subtracts 2 v (v^T c) after summing
+// the partials in chunk order.  Row-major storage makes each 32-column strip
+// of a row a 256-byte coalesced segment.  This is the unblocked (level-2)
+// form: bound by passes over the trailing block per column; the blocked
+// (compact-WY, MFMA) form is the next step for large matrices.
 #include "api_internal.hpp"
 #include <cfloat>
 #include <vector>
@@ -81,38 +81,74 @@ __global__ __launch_bounds__(HT) void house_kernel(const double *__restrict__ R,
 }
 
 // Apply H = I - 2 v v^T to rows k.. of two matrices: R's columns [rc0, rc1)
-// (workgroups [0, nbr)) and all qcols columns of Qt (the remaining workgroups).
-__global__ __launch_bounds__(AT) void house_apply_kernel(double *__restrict__ R, long ldr, int rc0, int rc1, int nbr,
-                                                         double *__restrict__ Qt, long ldq, int qcols, int rows,
-                                                         int k, const double *__restrict__ v,
-                                                         const int *__restrict__ flag)
-{
-    if (*flag == 0) return;
-    __shared__ double part[AG][AC];
-    const int cl = threadIdx.x % AC, rg = threadIdx.x / AC;
-    int b = blockIdx.x;
+// (column strips [0, nbr)) and all qcols columns of Qt (the remaining strips).
+// The trailing rows are split into chunks of RC rows (blockIdx.y), so the
+// update fills the chip at any shape: pass 1 writes each chunk's partial
+// v^T c per column, pass 2 sums the partials in chunk order (the same order
+// in every workgroup) and subtracts 2 v (v^T c) from its chunk.
+constexpr int RC = 256;
+
+struct StripCol {
     double *M;
     long ld;
-    int c, cend;
+    int c, cend, g;   // column in M, its end, column index in the partials
+};
+
+__device__ StripCol strip_col(double *R, long ldr, int rc0, int rc1, int nbr, double *Qt, long ldq, int qcols)
+{
+    const int cl = threadIdx.x % AC;
+    int b = blockIdx.x;
+    StripCol s;
+    s.g = b * AC + cl;
     if (b < nbr) {
-        M = R; ld = ldr; c = rc0 + b * AC + cl; cend = rc1;
+        s.M = R; s.ld = ldr; s.c = rc0 + b * AC + cl; s.cend = rc1;
     } else {
         b -= nbr;
-        M = Qt; ld = ldq; c = b * AC + cl; cend = qcols;
+        s.M = Qt; s.ld = ldq; s.c = b * AC + cl; s.cend = qcols;
     }
-    const bool on = c < cend;
-    const int len = rows - k;
-    double *base = M + (long)k * ld + c;
+    return s;
+}
+
+__global__ __launch_bounds__(AT) void house_dots_kernel(double *__restrict__ R, long ldr, int rc0, int rc1, int nbr,
+                                                        double *__restrict__ Qt, long ldq, int qcols, int rows, int k,
+                                                        const double *__restrict__ v, const int *__restrict__ flag,
+                                                        double *__restrict__ part, long ldp)
+{
+    if (*flag == 0) return;
+    __shared__ double red[AG][AC];
+    const StripCol sc = strip_col(R, ldr, rc0, rc1, nbr, Qt, ldq, qcols);
+    const int cl = threadIdx.x % AC, rg = threadIdx.x / AC;
+    const int len = rows - k, i0 = blockIdx.y * RC, i1 = min(len, i0 + RC);
     double acc = 0.0;
-    if (on)
-        for (int i = rg; i < len; i += AG) acc += v[i] * base[(long)i * ld];
-    part[rg][cl] = acc;
+    if (sc.c < sc.cend) {
+        const double *base = sc.M + (long)k * sc.ld + sc.c;
+        for (int i = i0 + rg; i < i1; i += AG) acc += v[i] * base[(long)i * sc.ld];
+    }
+    red[rg][cl] = acc;
     __syncthreads();
-    double dot = 0.0;
+    if (rg == 0) {
+        double d = 0.0;
 #pragma unroll
-    for (int g = 0; g < AG; ++g) dot += part[g][cl];
-    if (on)
-        for (int i = rg; i < len; i += AG) base[(long)i * ld] -= 2.0 * v[i] * dot;
+        for (int g = 0; g < AG; ++g) d += red[g][cl];
+        part[(long)blockIdx.y * ldp + sc.g] = d;
+    }
+}
+
+__global__ __launch_bounds__(AT) void house_update_kernel(double *__restrict__ R, long ldr, int rc0, int rc1,
+                                                          int nbr, double *__restrict__ Qt, long ldq, int qcols,
+                                                          int rows, int k, const double *__restrict__ v,
+                                                          const int *__restrict__ flag,
+                                                          const double *__restrict__ part, long ldp, int nrc)
+{
+    if (*flag == 0) return;
+    const StripCol sc = strip_col(R, ldr, rc0, rc1, nbr, Qt, ldq, qcols);
+    if (sc.c >= sc.cend) return;
+    const int rg = threadIdx.x / AC;
+    double dot = 0.0;
+    for (int q = 0; q < nrc; ++q) dot += part[(long)q * ldp + sc.g];
+    const int len = rows - k, i0 = blockIdx.y * RC, i1 = min(len, i0 + RC);
+    double *base = sc.M + (long)k * sc.ld + sc.c;
+    for (int i = i0 + rg; i < i1; i += AG) base[(long)i * sc.ld] -= 2.0 * v[i] * dot;
 }
 
 __global__ void eye_kernel(double *__restrict__ Q, long ldq, int n)
@@ -179,16 +215,28 @@ __global__ __launch_bounds__(HT) void sum_kernel(const double *__restrict__ sq, 
 }
 
 // Householder sweep over the first min(rows, cols) columns of R (rows x rcols,
-// ld ldr); Qt (rows x rows, ld ldq) accumulates Q^T when not null.
+// ld ldr); Qt (rows x rows, ld ldq) accumulates Q^T when not null.  part
+// holds partial_elems(rows, rcols, Qt != null) doubles.
+long partial_elems(int rows, int rcols, bool with_q)
+{
+    const long strips = (rcols + AC - 1) / AC + (with_q ? (rows + AC - 1) / AC : 0);
+    return ((rows + RC - 1) / RC) * strips * AC;
+}
+
 int householder_sweep(int rows, int cols, int rcols, double *R, long ldr, double *Qt, long ldq, double *v, int *flag,
-                      hipStream_t s)
+                      double *part, hipStream_t s)
 {
     const int kmax = std::min(rows, cols);
     for (int k = 0; k < kmax; ++k) {
         house_kernel<<<1, HT, 0, s>>>(R, ldr, rows, k, v, flag);
         const int nbr = (rcols - k + AC - 1) / AC;
         const int nbq = Qt ? (rows + AC - 1) / AC : 0;
-        house_apply_kernel<<<nbr + nbq, AT, 0, s>>>(R, ldr, k, rcols, nbr, Qt, ldq, rows, rows, k, v, flag);
+        const int nrc = (rows - k + RC - 1) / RC;
+        const long ldp = (long)(nbr + nbq) * AC;
+        dim3 grid(nbr + nbq, nrc);
+        house_dots_kernel<<<grid, AT, 0, s>>>(R, ldr, k, rcols, nbr, Qt, ldq, rows, rows, k, v, flag, part, ldp);
+        house_update_kernel<<<grid, AT, 0, s>>>(R, ldr, k, rcols, nbr, Qt, ldq, rows, rows, k, v, flag, part, ldp,
+                                                nrc);
     }
     SCIRS_HIP_CHECK(hipGetLastError());
     return 0;
@@ -215,15 +263,18 @@ extern "C" int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, d
     const long pr = round_up(cols, 64), pq = round_up(rows, 64);
     const size_t rbytes = (size_t)rows * pr * sizeof(double), qbytes = (size_t)rows * pq * sizeof(double);
     if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, rbytes))) return finish(info, rc, t0);
-    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, 2 * qbytes + (size_t)(rows + 64) * sizeof(double))))
+    const size_t npart = (size_t)partial_elems(rows, cols, true);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes,
+                       2 * qbytes + (size_t)(rows + 64) * sizeof(double) + npart * sizeof(double))))
         return finish(info, rc, t0);
     double *dR = (double *)c.scratch_a, *dQt = (double *)c.scratch_b, *dQ = dQt + (size_t)rows * pq;
     double *dv = dQ + (size_t)rows * pq;
     int *dflag = (int *)(dv + rows);
+    double *dpart = dv + rows + 64;
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
                                      hipMemcpyHostToDevice, s));
     eye_kernel<<<(unsigned)(((long)rows * rows + 255) / 256), 256, 0, s>>>(dQt, pq, rows);
-    if ((rc = householder_sweep(rows, cols, cols, dR, pr, dQt, pq, dv, dflag, s))) return finish(info, rc, t0);
+    if ((rc = householder_sweep(rows, cols, cols, dR, pr, dQt, pq, dv, dflag, dpart, s))) return finish(info, rc, t0);
     dim3 tg((rows + 31) / 32, (rows + 31) / 32);
     transpose_kernel<<<tg, 256, 0, s>>>(dQt, pq, rows, rows, dQ, pq);
     SCIRS_HIP_CHECK(hipGetLastError());
@@ -253,7 +304,8 @@ extern "C" int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda
     // column takes every reflector, which forms Q^T b without Q.
     const long pr = round_up(cols + 1, 64), pa = round_up(cols, 64);
     const size_t rbytes = (size_t)rows * pr * sizeof(double);
-    const size_t wbytes = ((size_t)rows * pa + 3 * (size_t)rows + 2 * (size_t)cols + 128) * sizeof(double);
+    const size_t npart = (size_t)partial_elems(rows, cols + 1, false);
+    const size_t wbytes = ((size_t)rows * pa + 3 * (size_t)rows + 2 * (size_t)cols + 128 + npart) * sizeof(double);
     if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, rbytes))) return finish(info, rc, t0);
     if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, wbytes))) return finish(info, rc, t0);
     double *dR = (double *)c.scratch_a;
@@ -261,6 +313,7 @@ extern "C" int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda
     double *dA = (double *)c.scratch_b, *dy = dA + (size_t)rows * pa, *db = dy + cols + 64, *dv = db + rows;
     double *dsq = dv + rows, *dres = dsq + rows;
     int *dflag = (int *)(dres + 1);
+    double *dpart = dres + 64;
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
                                      hipMemcpyHostToDevice, s));
     SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, pa * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
@@ -270,7 +323,8 @@ extern "C" int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda
     // contiguous vectors cross PCIe
     strided_scatter_kernel<<<(rows + 255) / 256, 256, 0, s>>>(db, rows, dR + cols, pr);
     SCIRS_HIP_CHECK(hipGetLastError());
-    if ((rc = householder_sweep(rows, cols, cols + 1, dR, pr, nullptr, 0, dv, dflag, s))) return finish(info, rc, t0);
+    if ((rc = householder_sweep(rows, cols, cols + 1, dR, pr, nullptr, 0, dv, dflag, dpart, s)))
+        return finish(info, rc, t0);
     // the back substitution's |r_ii| < eps test (solve.rs:184-191), first row
     // in substitution order (bottom up)
     std::vector<double> diag(cols);
