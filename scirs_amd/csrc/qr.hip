@@ -43,15 +43,26 @@ __device__ double block_sum(double v, double *red)
 }
 
 // v[0 .. rows-k) <- the unit reflector of column k of R; flag <- 1, or 0 when
-// the reference skips the column (lapack.rs:205-215).
+// the reference skips the column (lapack.rs:205-215).  The column is read
+// once: its first HC*HT entries stay in registers for the three passes.
+constexpr int HC = 16;
+
 __global__ __launch_bounds__(HT) void house_kernel(const double *__restrict__ R, long ldr, int rows, int k,
                                                    double *__restrict__ v, int *__restrict__ flag)
 {
     __shared__ double red[HT / 64];
     const int len = rows - k;
     const double *col = R + (long)k * ldr + k;
+    double xc[HC];
     double ss = 0.0;
-    for (int i = threadIdx.x; i < len; i += HT) {
+#pragma unroll
+    for (int u = 0; u < HC; ++u) {
+        const int i = threadIdx.x + u * HT;
+        xc[u] = i < len ? col[(long)i * ldr] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < HC; ++u) ss += xc[u] * xc[u];
+    for (int i = threadIdx.x + HC * HT; i < len; i += HT) {
         const double x = col[(long)i * ldr];
         ss += x * x;
     }
@@ -63,9 +74,12 @@ __global__ __launch_bounds__(HT) void house_kernel(const double *__restrict__ R,
     const double x0 = col[0];
     const double alpha = x0 >= 0.0 ? -xn : xn;
     const double v0 = x0 - alpha;
+    if (threadIdx.x == 0) xc[0] = v0;
     double vs = 0.0;
-    for (int i = threadIdx.x; i < len; i += HT) {
-        const double x = i == 0 ? v0 : col[(long)i * ldr];
+#pragma unroll
+    for (int u = 0; u < HC; ++u) vs += xc[u] * xc[u];
+    for (int i = threadIdx.x + HC * HT; i < len; i += HT) {
+        const double x = col[(long)i * ldr];
         vs += x * x;
     }
     const double vn = sqrt(block_sum(vs, red));
@@ -73,10 +87,12 @@ __global__ __launch_bounds__(HT) void house_kernel(const double *__restrict__ R,
         if (threadIdx.x == 0) *flag = 0;
         return;
     }
-    for (int i = threadIdx.x; i < len; i += HT) {
-        const double x = i == 0 ? v0 : col[(long)i * ldr];
-        v[i] = x / vn;
+#pragma unroll
+    for (int u = 0; u < HC; ++u) {
+        const int i = threadIdx.x + u * HT;
+        if (i < len) v[i] = xc[u] / vn;
     }
+    for (int i = threadIdx.x + HC * HT; i < len; i += HT) v[i] = col[(long)i * ldr] / vn;
     if (threadIdx.x == 0) *flag = 1;
 }
 
