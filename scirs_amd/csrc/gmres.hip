@@ -89,10 +89,10 @@ __global__ __launch_bounds__(NT) void gm_orth_kernel(int n, int k, const double 
 }
 
 // v = v / d (in place, true division)
-__global__ __launch_bounds__(NT) void gm_div_kernel(int n, double *__restrict__ v, const double *__restrict__ d)
+__global__ __launch_bounds__(NT) void gm_div_kernel(int n, double *__restrict__ v, double d)
 {
     const int q = blockIdx.x * NT + threadIdx.x;
-    if (q < n) v[q] = v[q] / *d;
+    if (q < n) v[q] = v[q] / d;
 }
 
 // dst = src / d
@@ -226,8 +226,7 @@ extern "C" int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const doubl
                 j += 1;
                 break;
             }
-            SCIRS_HIP_CHECK(hipMemcpyAsync(dsc + 1, &hcol[j + 1], sizeof(double), hipMemcpyHostToDevice, s));
-            gm_div_kernel<<<blocks(n, NT), NT, 0, s>>>(n, vn, dsc + 1);
+            gm_div_kernel<<<blocks(n, NT), NT, 0, s>>>(n, vn, hcol[j + 1]);
             j += 1;
             total += 1;
         }
