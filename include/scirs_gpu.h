@@ -136,6 +136,13 @@ int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda, const dou
 int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const double *b, const double *x0, int max_iter,
                      double tol, int restart, double *x, int *iterations, double *residual_norm, int *converged,
                      scirs_solve_info *info);
+/* The same on device-resident A (lda), b and x, on the caller's stream (NULL =
+ * the default stream): x holds x0 when use_x0 != 0 (else it is zeroed first)
+ * and the solution on return.  The host reads a few scalars per step (the
+ * reference's control flow), so the call returns after the work is done. */
+int scirs_gpu_dgmres_dev(int n, const double *dA, int64_t lda, const double *dB, double *dX, int use_x0,
+                         int max_iter, double tol, int restart, int *iterations, double *residual_norm,
+                         int *converged, void *stream, scirs_solve_info *info);
 
 /* ---- batched array operations ------------------------------------------------ */
 /* Replace scirs2-linalg/src/batch/mod.rs for f64 / f32, bit-identical to its

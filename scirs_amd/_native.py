@@ -83,6 +83,7 @@ SIGNATURES = {
     "scirs_gpu_dposv": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
     "scirs_gpu_dtrsv": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _INFO], _I),
     "scirs_gpu_dgmres": ([_I, _P, _L, _P, _P, _I, _D, _I, _P, _P, _P, _P, _INFO], _I),
+    "scirs_gpu_dgmres_dev": ([_I, _P, _L, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dbatch_matmul": ([_I, _I, _I, _I, _P, _P, _P, _INFO], _I),
     "scirs_gpu_sbatch_matmul": ([_I, _I, _I, _I, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dbatch_matvec": ([_I, _I, _I, _P, _P, _P, _INFO], _I),

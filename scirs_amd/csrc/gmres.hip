@@ -148,41 +148,25 @@ int gmres_lsq(int m, int nc, std::vector<double> hc, double beta, std::vector<do
 
 unsigned blocks(int n, int per) { return (unsigned)((n + per - 1) / per); }
 
-} // namespace
-
-extern "C" int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const double *b, const double *x0, int max_iter,
-                                double tol, int restart, double *x, int *iterations, double *residual_norm,
-                                int *converged, scirs_solve_info *info)
+// The iteration on device-resident A (lda), b and x (x holds x0 on entry, the
+// solution on return), on stream s; the Krylov basis and small vectors live in
+// the library's scratch.  Returns 0, k > 0 (least squares singular at row k)
+// or an SCIRS_ERR_* code.
+int gmres_device(int n, const double *dA, long lda, const double *db, double *dx, int max_iter, double tol,
+                 int restart, int *iterations, double *residual_norm, int *converged, hipStream_t s,
+                 scirs_solve_info *info)
 {
-    const double t0 = now_ms();
-    info_init(info, SCIRS_PATH_FP64_LU);
-    if (n <= 0 || max_iter < 0 || !a || !b || !x || lda < n || !iterations || !residual_norm || !converged) {
-        scirs_set_error("dgmres: invalid arguments (n=%d max_iter=%d)", n, max_iter);
-        return finish(info, SCIRS_ERR_ARG, t0);
-    }
-    if (restart <= 0) restart = std::min(n, 30);
-    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
     Context &c = ctx();
-    int rc = c.init(0);
-    if (rc) return finish(info, rc, t0);
-    hipStream_t s = c.stream;
+    int rc;
     const size_t nn = (size_t)n;
-    const size_t vbytes = (size_t)(restart + 1) * nn * sizeof(double);
-    const size_t wbytes = vbytes + (4 * nn + 2 * (size_t)(restart + 2) + 64) * sizeof(double);
-    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, nn * nn * sizeof(double)))) return finish(info, rc, t0);
-    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, wbytes))) return finish(info, rc, t0);
-    double *dA = (double *)c.scratch_a;
-    double *dV = (double *)c.scratch_b, *db = dV + (size_t)(restart + 1) * nn, *dx = db + nn, *dr = dx + nn,
-           *dw = dr + nn, *dh = dw + nn, *dy = dh + (restart + 2), *dsc = dy + (restart + 2);
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, nn * sizeof(double), a, lda * sizeof(double), nn * sizeof(double), n,
-                                     hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, nn * sizeof(double), hipMemcpyHostToDevice, s));
-    if (x0) SCIRS_HIP_CHECK(hipMemcpyAsync(dx, x0, nn * sizeof(double), hipMemcpyHostToDevice, s));
-    else SCIRS_HIP_CHECK(hipMemsetAsync(dx, 0, nn * sizeof(double), s));
+    const size_t wbytes = ((size_t)(restart + 1) * nn + 2 * nn + 2 * (size_t)(restart + 2) + 64) * sizeof(double);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, wbytes))) return rc;
+    double *dV = (double *)c.scratch_b, *dr = dV + (size_t)(restart + 1) * nn, *dw = dr + nn, *dh = dw + nn,
+           *dy = dh + (restart + 2), *dsc = dy + (restart + 2);
 
     // ||b - A x|| into the host; r stays in dr
     auto resid_norm = [&](double *out) -> int {
-        gm_resid_kernel<<<blocks(n, NT / 64), NT, 0, s>>>(n, dA, n, db, dx, dr);
+        gm_resid_kernel<<<blocks(n, NT / 64), NT, 0, s>>>(n, dA, lda, db, dx, dr);
         gm_dots_kernel<<<1, 1024, 0, s>>>(n, 1, dr, dr, dsc);
         SCIRS_HIP_CHECK(hipGetLastError());
         double v;
@@ -191,14 +175,12 @@ extern "C" int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const doubl
         *out = sqrt(v);
         return 0;
     };
-    auto finish_ok = [&](int total, double rn, int conv) -> int {
-        SCIRS_HIP_CHECK(hipMemcpyAsync(x, dx, nn * sizeof(double), hipMemcpyDeviceToHost, s));
-        SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    auto done = [&](int total, double rn, int conv) -> int {
         *iterations = total;
         *residual_norm = rn;
         *converged = conv;
         if (info) info->gmres_iters = total;
-        return finish(info, SCIRS_OK, t0);
+        return 0;
     };
 
     const int cycles = std::max(max_iter / restart, 1);
@@ -206,14 +188,14 @@ extern "C" int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const doubl
     std::vector<double> H((size_t)(restart + 1) * restart), hcol(restart + 2), y;
     for (int outer = 0; outer < cycles; ++outer) {
         double beta;
-        if ((rc = resid_norm(&beta))) return finish(info, rc, t0);
-        if (beta < tol) return finish_ok(total, beta, 1);
+        if ((rc = resid_norm(&beta))) return rc;
+        if (beta < tol) return done(total, beta, 1);
         gm_div_copy_kernel<<<blocks(n, NT), NT, 0, s>>>(n, dr, beta, dV);
         std::fill(H.begin(), H.end(), 0.0);
         int j = 0;
         while (j < restart && total < max_iter) {
             double *vj = dV + (size_t)j * nn, *vn = vj + nn;
-            gm_gemv_kernel<<<blocks(n, NT / 64), NT, 0, s>>>(n, dA, n, vj, dw);
+            gm_gemv_kernel<<<blocks(n, NT / 64), NT, 0, s>>>(n, dA, lda, vj, dw);
             gm_dots_kernel<<<j + 1, 1024, 0, s>>>(n, j + 1, dV, dw, dh);
             gm_orth_kernel<<<blocks(n, NT), NT, 0, s>>>(n, j + 1, dV, dh, dw, vn);
             gm_dots_kernel<<<1, 1024, 0, s>>>(n, 1, vn, vn, dh + j + 1);
@@ -238,17 +220,71 @@ extern "C" int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const doubl
             if (st) {
                 scirs_set_error("dgmres: least-squares solve singular at row %d", st);
                 if (info) info->gmres_iters = total;
-                return finish(info, st, t0);
+                return st;
             }
             SCIRS_HIP_CHECK(hipMemcpyAsync(dy, y.data(), j * sizeof(double), hipMemcpyHostToDevice, s));
             gm_combine_kernel<<<blocks(n, NT), NT, 0, s>>>(n, j, dV, dy, dx);
             SCIRS_HIP_CHECK(hipGetLastError());
         }
         double rn;
-        if ((rc = resid_norm(&rn))) return finish(info, rc, t0);
-        if (rn < tol) return finish_ok(total, rn, 1);
+        if ((rc = resid_norm(&rn))) return rc;
+        if (rn < tol) return done(total, rn, 1);
     }
     double rn;
-    if ((rc = resid_norm(&rn))) return finish(info, rc, t0);
-    return finish_ok(total, rn, 0);
+    if ((rc = resid_norm(&rn))) return rc;
+    return done(total, rn, 0);
+}
+
+} // namespace
+
+extern "C" int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const double *b, const double *x0, int max_iter,
+                                double tol, int restart, double *x, int *iterations, double *residual_norm,
+                                int *converged, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (n <= 0 || max_iter < 0 || !a || !b || !x || lda < n || !iterations || !residual_norm || !converged) {
+        scirs_set_error("dgmres: invalid arguments (n=%d max_iter=%d)", n, max_iter);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    if (restart <= 0) restart = std::min(n, 30);
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    Context &c = ctx();
+    int rc = c.init(0);
+    if (rc) return finish(info, rc, t0);
+    hipStream_t s = c.stream;
+    const size_t nn = (size_t)n;
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (nn * nn + 2 * nn) * sizeof(double))))
+        return finish(info, rc, t0);
+    double *dA = (double *)c.scratch_a, *db = dA + nn * nn, *dx = db + nn;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, nn * sizeof(double), a, lda * sizeof(double), nn * sizeof(double), n,
+                                     hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, nn * sizeof(double), hipMemcpyHostToDevice, s));
+    if (x0) SCIRS_HIP_CHECK(hipMemcpyAsync(dx, x0, nn * sizeof(double), hipMemcpyHostToDevice, s));
+    else SCIRS_HIP_CHECK(hipMemsetAsync(dx, 0, nn * sizeof(double), s));
+    rc = gmres_device(n, dA, n, db, dx, max_iter, tol, restart, iterations, residual_norm, converged, s, info);
+    if (rc) return finish(info, rc, t0);
+    SCIRS_HIP_CHECK(hipMemcpyAsync(x, dx, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    return finish(info, SCIRS_OK, t0);
+}
+
+extern "C" int scirs_gpu_dgmres_dev(int n, const double *dA, int64_t lda, const double *dB, double *dX, int use_x0,
+                                    int max_iter, double tol, int restart, int *iterations, double *residual_norm,
+                                    int *converged, void *stream, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (n <= 0 || max_iter < 0 || !dA || !dB || !dX || lda < n || !iterations || !residual_norm || !converged) {
+        scirs_set_error("dgmres_dev: invalid arguments (n=%d max_iter=%d)", n, max_iter);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    if (restart <= 0) restart = std::min(n, 30);
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    int rc = ctx().init(0);
+    if (rc) return finish(info, rc, t0);
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
+    if (!use_x0) SCIRS_HIP_CHECK(hipMemsetAsync(dX, 0, (size_t)n * sizeof(double), s));
+    rc = gmres_device(n, dA, lda, dB, dX, max_iter, tol, restart, iterations, residual_norm, converged, s, info);
+    return finish(info, rc, t0);
 }

@@ -164,6 +164,23 @@ def gmres(a, b, x0=None, max_iter=1000, tol=1e-10, restart=0):
     return x, it.value, res.value, bool(conv.value)
 
 
+def gmres_dev(a, b, x, use_x0=False, max_iter=1000, tol=1e-10, restart=0):
+    """gmres on device tensors (float64, cuda): x holds x0 when use_x0 and the
+    solution on return.  Returns (iterations, residual_norm, converged)."""
+    lib = N.load()
+    n = a.shape[0]
+    it = ctypes.c_int(0)
+    conv = ctypes.c_int(0)
+    res = ctypes.c_double(0.0)
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dgmres_dev(n, ctypes.c_void_p(a.data_ptr()), a.stride(0), ctypes.c_void_p(b.data_ptr()),
+                                  ctypes.c_void_p(x.data_ptr()), int(use_x0), int(max_iter), float(tol),
+                                  int(restart), ctypes.byref(it), ctypes.byref(res), ctypes.byref(conv),
+                                  _stream_ptr(), ctypes.byref(info))
+    _check("gmres_dev", rc, info)
+    return it.value, res.value, bool(conv.value)
+
+
 def _batch_args(a):
     a = np.asarray(a)
     dt = np.float32 if a.dtype == np.float32 else np.float64

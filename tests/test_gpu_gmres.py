@@ -63,3 +63,26 @@ def test_breakdown_ends_the_cycle(L):
     b[3] = 2.0
     x, it, res, conv = _check(L, np.eye(n), b)
     assert conv and it == 0
+
+
+def test_dev_entry_matches_host_entry(L):
+    """The device-resident entry point runs the same kernels: same iterations,
+    same bits of x, with and without x0 (lda larger than n covered)."""
+    import torch
+    rng = np.random.default_rng(12)
+    n = 500
+    a = rng.standard_normal((n, n)) / np.sqrt(n) + 2.0 * np.eye(n)
+    b = rng.standard_normal(n)
+    x_h, it_h, res_h, conv_h = L.gmres(a, b, restart=20)
+    big = np.zeros((n, n + 36))
+    big[:, :n] = a
+    ta = torch.from_numpy(big).cuda()[:, :n]
+    tb = torch.from_numpy(b).cuda()
+    tx = torch.full((n,), 7.0, dtype=torch.float64, device="cuda")
+    it, res, conv = L.gmres_dev(ta, tb, tx, restart=20)
+    assert (it, conv) == (it_h, conv_h)
+    assert np.array_equal(tx.cpu().numpy(), x_h) and res == res_h
+    x0 = np.linalg.solve(a, b)
+    tx = torch.from_numpy(x0).cuda()
+    it, res, conv = L.gmres_dev(ta, tb, tx, use_x0=True)
+    assert it == 0 and conv
