@@ -787,6 +787,10 @@ Result<IterativeSolverResult<A>> gmres(const ArrayView2<A> &a, const ArrayView1<
         x0d.resize(n);
         for (std::size_t i = 0; i < n; ++i) x0d[i] = (*x0)[i];
     }
+    if (options.restart && *options.restart == 0)
+        // the reference computes restart.min(n) = 0 and then divides by it
+        // (iterative.rs:361-366), a panic; here it is an error
+        return LinalgError(LinalgError::Kind::InvalidInputError, "GMRES restart must be positive");
     IterativeSolverResult<A> out{Array1<A>(n), 0, A(0), false};
     if (n == 0) {
         // the reference divides by restart = min(n, 30) = 0 here and panics;
@@ -805,7 +809,9 @@ Result<IterativeSolverResult<A>> gmres(const ArrayView2<A> &a, const ArrayView1<
     if (rc > 0)
         return LinalgError(LinalgError::Kind::SingularMatrixError, "Matrix is singular in GMRES least squares solve");
     if (rc < 0) return detail::gpu_failure(rc);
-    if (options.verbose) std::printf("GMRES: residual = %g after %d iterations\n", res, it);
+    // options.verbose is accepted and not acted on: the reference prints one
+    // "GMRES outer iteration {}: residual = {}" line per restart cycle
+    // (iterative.rs:424-429), which the GPU loop does not report back
     out.iterations = (std::size_t)it;
     out.residual_norm = (A)res;
     out.converged = conv != 0;
@@ -921,20 +927,19 @@ Result<Array3<F>> batch_add(const ArrayView3<F> &batch_a, const ArrayView1<F> &v
     return out;
 }
 
-// batch/mod.rs:326-336 (infallible in the reference; a device failure aborts
-// here, as no error can be returned)
+// batch/mod.rs:326-336.  The reference's batch_sum cannot fail and returns the
+// array itself; a GPU call can fail (no device, out of memory), so this one
+// returns a Result whose value is that array, and a device failure is a
+// ComputationError like every other entry point's.
 template <class F>
-Array2<F> batch_sum(const ArrayView3<F> &batch_a)
+Result<Array2<F>> batch_sum(const ArrayView3<F> &batch_a)
 {
     static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
     const auto [nb, m, n] = batch_a.dim();
     Array2<F> out(m, n);
     std::vector<F> ad = detail::standard3(batch_a);
     int rc = detail::batch_sum((int)nb, (int)m, (int)n, ad.data(), out.data.data());
-    if (rc < 0) {
-        std::fprintf(stderr, "batch_sum: GPU failure %d: %s\n", rc, scirs_gpu_last_error());
-        std::abort();
-    }
+    if (rc < 0) return detail::gpu_failure(rc);
     return out;
 }
 

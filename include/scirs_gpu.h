@@ -247,6 +247,25 @@ int scirs_gpu_prof_end(scirs_prof_stats *out);
 int scirs_gpu_backward_error_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
                                  const double *dB, int64_t ldb, void *stream, double *out);
 
+/* Tournament-pivoted FP32 LU (CALU) of the n x n matrix in HBM, in place (unit L
+ * below the diagonal, U on and above; lda a multiple of 4), with the trailing
+ * update in the arithmetic of `mode` (SCIRS_MODE_FP32, _BF16, _BF16X3, _BF16X6;
+ * AUTO = BF16X6), and the row permutation in perm (int64, row i of P A is row
+ * perm[i] of A).  No counterpart in the reference: this is the factorisation
+ * scirs_gpu_dgesv_ir refines (its pivots are chosen by a reduction tree over
+ * blocks of rows, not the column arg-max of lapack.rs:95-104), exported so its
+ * factors can be checked on their own.  Returns 0, k > 0 when |u_kk| < eps
+ * (FP32), or SCIRS_ERR_*. */
+int scirs_gpu_sgetrf_tp_dev(int n, float *dA, int64_t lda, int64_t *dperm, int mode, void *stream);
+
+/* Host arithmetic of the exchange tags (no device work, no reference
+ * counterpart; exported so the wrap-around rule can be tested on the host).
+ * A launch that uses `span` consecutive tags gets base .. base + span - 1 from
+ * the running count `next`; *next_out is the count after it.  Returns 1 when the
+ * count had to restart at 1 (the library then clears every tagged buffer
+ * first), 0 otherwise.  Tag 0 is never handed out. */
+int scirs_gpu_seq_plan(unsigned next, unsigned span, unsigned *base, unsigned *next_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,8 @@ _INFO = ctypes.POINTER(SolveInfo)
 
 SIGNATURES = {
     "scirs_gpu_init": ([_I], _I),
+    "scirs_gpu_sgetrf_tp_dev": ([_I, _P, _L, _P, _I, _P], _I),
+    "scirs_gpu_seq_plan": ([ctypes.c_uint, ctypes.c_uint, _P, _P], _I),
     "scirs_gpu_last_error": ([], ctypes.c_char_p),
     "scirs_gpu_dgetrf": ([_I, _I, _P, _L, _P, _INFO], _I),
     "scirs_gpu_sgetrf": ([_I, _I, _P, _L, _P, _INFO], _I),

@@ -140,7 +140,13 @@ int Context::init(int dev)
                  pairs_b = (size_t)work.max_panels * 4 * kbmax * sizeof(int),
                  npairs_b = (size_t)work.max_panels * sizeof(int),
                  linv_b = 2 * 128 * 128 * sizeof(float);   // two FP32 L11^-1 (double-buffered)
-    for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b, linv_b}) bytes += (b + 255) & ~(size_t)255;
+    const size_t tpc_b = (size_t)TSLU_MAXLEV * TSLU_MAXNODE * 65 * 8, tpn_b = (size_t)TSLU_MAXLEV * TSLU_MAXNODE * 4,
+                 tpu_b = 64 * 64 * sizeof(float), tpr_b = 64 * 64 * sizeof(float),
+                 tpt_b = 64 * (size_t)kbmax * sizeof(float), tpv_b = 65 * sizeof(int),
+                 tpm_b = (2 * (size_t)TSLU_MAP + 1) * sizeof(int);
+    for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b, linv_b, tpc_b, tpn_b, tpu_b, tpr_b, tpt_b, tpt_b,
+                     tpv_b, tpm_b})
+        bytes += (b + 255) & ~(size_t)255;
     char *base = nullptr;
     SCIRS_HIP_CHECK(hipMalloc((void **)&base, bytes));
     SCIRS_HIP_CHECK(hipMemset(base, 0, bytes));
@@ -149,6 +155,9 @@ int Context::init(int dev)
     work.xw = (unsigned long long *)take(xw_b);
     work.xrow = (unsigned long long *)take(xrow_b);
     work.xrj = (unsigned long long *)take(xrj_b);
+    work.xw_words = xw_b / 8;
+    work.xrow_words = xrow_b / 8;
+    work.xrj_words = xrj_b / 8;
     char *flags = take(flags_b);
     work.abort = (int *)(flags + 8);
     work.info = (int *)(flags + 12);
@@ -157,7 +166,16 @@ int Context::init(int dev)
     work.pairs = (int *)take(pairs_b);
     work.npairs = (int *)take(npairs_b);
     work.linv = take(linv_b);
-    work.seq_next = 0;
+    work.tp_cand = (unsigned long long *)take(tpc_b);
+    work.tp_cnt = (int *)take(tpn_b);
+    work.tp_uinv = (float *)take(tpu_b);
+    work.tp_urest = (float *)take(tpr_b);
+    work.tp_top = (float *)take(tpt_b);
+    work.tp_stage = (float *)take(tpt_b);
+    work.tp_vlist = (int *)take(tpv_b);
+    work.tp_map = (int *)take(tpm_b);
+    work.tslu = env_int("SCIRS_GPU_TSLU", 1);
+    work.seq_next = 1;
     work_base = base;
     SCIRS_HIP_CHECK(hipHostMalloc((void **)&host_flags, 64, hipHostMallocDefault));
     // panel widths: 8 .. 128 (the LDS budgets of the panel's neighbours — the
@@ -191,7 +209,12 @@ int Context::read_flags(hipStream_t s, int *info, double *info_val)
     memcpy(&v, (char *)host_flags + 8, 8);
     if (info_val) *info_val = v;
     if (abort_flag) {
-        scirs_set_error("a cooperative kernel (panel or chained triangular solve) timed out waiting for its exchange");
+        if (abort_flag > 0)
+            scirs_set_error("the LU panel's exchange timed out at column %d (another process may hold the device's "
+                            "compute units; SCIRS_GPU_SPIN_LIMIT bounds the wait)", abort_flag);
+        else
+            scirs_set_error("a chained triangular solve timed out waiting for its hand-over (SCIRS_GPU_SPIN_LIMIT "
+                            "bounds the wait)");
         return SCIRS_ERR_TIMEOUT;
     }
     return SCIRS_OK;
@@ -324,10 +347,9 @@ static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm
 {
     Context &c = ctx();
     int rc;
-    hipEvent_t e0, e1, e2;
-    SCIRS_HIP_CHECK(hipEventCreate(&e0));
-    SCIRS_HIP_CHECK(hipEventCreate(&e1));
-    SCIRS_HIP_CHECK(hipEventCreate(&e2));
+    scirs::CallEvents ev;
+    if (int erc = ev.create(3)) return erc;
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     SCIRS_HIP_CHECK(hipEventRecord(e0, s));
     if ((rc = c.reset_flags(s))) return rc;
     const int kb = c.lu_kb<T>(n);
@@ -350,9 +372,6 @@ static int getrf_solve_device(int n, int nrhs, T *dLU, long ldlu, int64_t *dPerm
         info->time_solve_ms = ms2;
         info->pivot_value = st ? pv : 0.0;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     return st;   // 0 or the 1-based singular column
 }
 
@@ -517,10 +536,9 @@ static int posv_device(int n, int nrhs, const double *dA, long lda, const double
 {
     Context &c = ctx();
     int rc;
-    hipEvent_t e0, e1, e2;
-    SCIRS_HIP_CHECK(hipEventCreate(&e0));
-    SCIRS_HIP_CHECK(hipEventCreate(&e1));
-    SCIRS_HIP_CHECK(hipEventCreate(&e2));
+    scirs::CallEvents ev;
+    if (int erc = ev.create(3)) return erc;
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     SCIRS_HIP_CHECK(hipEventRecord(e0, s));
     if ((rc = scirs_d2d_copy(n, n, dA, lda, dL, ldl, s))) return rc;
     if ((rc = c.reset_flags(s))) return rc;
@@ -551,9 +569,6 @@ static int posv_device(int n, int nrhs, const double *dA, long lda, const double
         info->time_solve_ms = ms2;
         info->pivot_value = st ? pv : 0.0;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     return st;
 }
 
@@ -665,4 +680,46 @@ extern "C" int scirs_gpu_backward_error_dev(int n, int nrhs, const double *dA, i
     SCIRS_ENTER(true);
     hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     return backward_error(n, nrhs, dA, lda, dX, ldx, dB, ldb, s, out);
+}
+
+// Clear every sequence-tagged exchange word (a kernel, so it is safe under capture).
+static int clear_tags(ScirsWork &w, hipStream_t s)
+{
+    struct Buf { unsigned long long *p; size_t words; };
+    const Buf bufs[] = {{w.xw, w.xw_words}, {w.xrow, w.xrow_words}, {w.xrj, w.xrj_words}, {w.xtri, w.xtri_words},
+                        {w.tp_cand, (size_t)TSLU_MAXLEV * TSLU_MAXNODE * 65}};
+    for (const Buf &b : bufs) {
+        if (!b.p || !b.words) continue;
+        for (size_t off = 0; off < 2 * b.words; off += (size_t)1 << 30) {
+            const size_t cnt = std::min((size_t)1 << 30, 2 * b.words - off);
+            if (int rc = scirs_fill_int((int *)b.p + off, (int)cnt, 0, s)) return rc;
+        }
+    }
+    return SCIRS_OK;
+}
+
+int scirs_next_seq(ScirsWork &w, unsigned span, hipStream_t s, unsigned *seq)
+{
+    unsigned next = w.seq_next, base = 0;
+    if (scirs_gpu_seq_plan(next, span, &base, &next)) {
+        if (int rc = clear_tags(w, s)) return rc;
+    }
+    w.seq_next = next;
+    *seq = base;
+    return SCIRS_OK;
+}
+
+extern "C" int scirs_gpu_seq_plan(unsigned next, unsigned span, unsigned *base, unsigned *next_out)
+{
+    // a launch tags its words with base .. base + span - 1 (never 0, the value of
+    // a cleared word); when that range would reach the top of the 32-bit count,
+    // the caller clears the tagged buffers and the count restarts at 1
+    int reset = 0;
+    if (next == 0u || span == 0u || next > 0xFFFFFFFFu - span) {
+        next = 1u;
+        reset = 1;
+    }
+    *base = next;
+    *next_out = next + (span ? span : 1u);
+    return reset;
 }

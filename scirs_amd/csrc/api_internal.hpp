@@ -55,6 +55,25 @@ int lu_substitute(int n, int nrhs, const T *dLU, long ldlu, T *dX, long ldx, hip
                   const T *dinv_l = nullptr, const T *dinv_u = nullptr, const T *dmt_l = nullptr,
                   const T *dmt_u = nullptr);
 
+// Timing events of one call, destroyed on every return path.
+struct CallEvents {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    int create(int n)
+    {
+        for (int i = 0; i < n; ++i)
+            if (hipEventCreate(&e[i]) != hipSuccess) {
+                scirs_set_error("hipEventCreate failed");
+                return SCIRS_ERR_HIP;
+            }
+        return SCIRS_OK;
+    }
+    ~CallEvents()
+    {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 Context &ctx();
 double now_ms();
 void info_init(scirs_solve_info *info, int path);

@@ -648,9 +648,9 @@ extern "C" int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *n
     if (wave_max < 0) {
         const int cand[3] = {0, 32, 64};
         float best = 1e30f;
-        hipEvent_t e0, e1;
-        SCIRS_HIP_CHECK(hipEventCreate(&e0));
-        SCIRS_HIP_CHECK(hipEventCreate(&e1));
+        scirs::CallEvents ev;
+        if (int erc = ev.create(2)) return erc;
+        hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
         for (int ci = 0; ci < 3; ++ci) {
             SCIRS_HIP_CHECK(hipEventRecord(e0, s));
             if ((rc = launch_batch(pl, dA, dB, dX, cand[ci], max_iter, tol_scale, s))) return rc;
@@ -660,8 +660,6 @@ extern "C" int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *n
             SCIRS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             if (ms < best) { best = ms; wave_max = cand[ci]; }
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         g_tune[pl.tune_key] = wave_max;
         tune_save();
     }
@@ -689,9 +687,9 @@ extern "C" int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *n
         SCIRS_HIP_CHECK(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
         pl.graph_wave_max = wave_max;
     }
-    hipEvent_t e0, e1;
-    SCIRS_HIP_CHECK(hipEventCreate(&e0));
-    SCIRS_HIP_CHECK(hipEventCreate(&e1));
+    scirs::CallEvents ev;
+    if (int erc = ev.create(2)) return erc;
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     SCIRS_HIP_CHECK(hipEventRecord(e0, s));
     {
         scirs::ProfScope prof(SCIRS_PROF_OTHER, 0.0, 0.0, s);
@@ -701,8 +699,6 @@ extern "C" int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *n
     SCIRS_HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;
     SCIRS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 
     std::vector<int> iters(batch);
     std::vector<double> bw(batch);

@@ -483,8 +483,7 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         panel::PanelArgs<T> pa;
         pa.A = A; pa.lda = lda; pa.k0 = k0; pa.m = mrows; pa.kb = kbw; pa.R = R; pa.G = G;
         pa.xw = w.xw; pa.xrow = w.xrow; pa.xrj = w.xrj; pa.nrep = w.panel_nrep;
-        pa.seq_base = w.seq_next;
-        w.seq_next += 512u;
+        if (int rc = scirs_next_seq(w, 512u, stream, &pa.seq_base)) return rc;
         pa.pairs = w.pairs + (size_t)p * 4 * SCIRS_KB_MAX; pa.npairs = w.npairs + p;
         pa.info = w.info; pa.info_val = w.info_val; pa.abort = w.abort; pa.spin_limit = w.spin_limit;
         pa.poll_sleep = w.poll_sleep;
@@ -493,7 +492,11 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
         const void *pk = w0 ? (const void *)panel::getf2_panel_kernel<T, true>
                                     : (const void *)panel::getf2_panel_kernel<T, false>;
         if (int rc = scirs_allow_lds(pk, lds)) return rc;
-        {
+        if (std::is_same<T, float>::value && w.tslu_active) {
+            // the mixed solve's FP32 factors: tournament pivoting (tslu.hip)
+            scirs::ProfScope ps(SCIRS_PROF_PANEL, (double)mrows * kbw * kbw, 2.0 * mrows * kbw * sizeof(T), stream);
+            if (int rc = scirs_tslu_panel((float *)A, lda, m, k0, kbw, p, w, stream)) return rc;
+        } else {
             scirs::ProfScope ps(SCIRS_PROF_PANEL, (double)mrows * kbw * kbw, 2.0 * mrows * kbw * sizeof(T), stream);
             if (G == 1) {
                 // one workgroup: nothing to exchange, 512 threads share the rows

@@ -145,7 +145,7 @@ struct IrState {
     const double *A; long lda;
     const double *B; long ldb;
     double *X; long ldx;
-    float *LU; int64_t *perm;
+    float *LU; long ld32; int64_t *perm;
     double *R;           // n x nrhs residual
     float *d32;          // n
     double *norms;       // device: [0..nrhs) r, [64..) x, [128..) b, [192] A
@@ -161,7 +161,7 @@ struct IrState {
 // d32 <- U^-1 L^-1 d32 with the FP32 factors
 int lu_solve_f32(IrState &st, float *d32)
 {
-    return scirs::lu_substitute<float>(st.n, 1, st.LU, st.n, d32, 1, st.s, st.dinv_l, st.dinv_u, st.dmt_l,
+    return scirs::lu_substitute<float>(st.n, 1, st.LU, st.ld32, d32, 1, st.s, st.dinv_l, st.dinv_u, st.dmt_l,
                                        st.dmt_u);
 }
 
@@ -346,7 +346,9 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     const int restart = std::min(n, std::max(1, atoi(getenv("SCIRS_GPU_GMRES_RESTART") ? getenv("SCIRS_GPU_GMRES_RESTART") : "50")));
 
     // scratch: A32 (n*n f32) | R (n*nrhs) | d32 (n) | norms (256) | rowsum (n) | V,Z ((restart+1)*n each) | w | h
-    const size_t a32 = (size_t)n * n * sizeof(float);
+    // the FP32 copy's rows start on 256-byte boundaries (16-byte loads of the tournament panel)
+    const long ld32 = ((long)n + 63) / 64 * 64;
+    const size_t a32 = (size_t)n * ld32 * sizeof(float);
     const size_t dinv_elems = (size_t)((n + 63) / 64) * 64 * 64;
     const size_t other = ((size_t)n * nrhs + (size_t)n + 256 + (size_t)n + 2 * (size_t)(restart + 1) * n + n + restart + 8) *
                          sizeof(double) + (size_t)n * sizeof(float) + (size_t)n * sizeof(int64_t) +
@@ -356,6 +358,7 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     IrState st;
     st.n = n; st.nrhs = nrhs; st.A = dA; st.lda = lda; st.B = dB; st.ldb = ldb; st.X = dX; st.ldx = ldx; st.s = s;
     st.LU = (float *)c.scratch_ir;
+    st.ld32 = ld32;
     char *p = (char *)c.scratch_ir2;
     auto take = [&](size_t bytes) { char *r = p; p += (bytes + 255) & ~(size_t)255; return r; };
     st.R = (double *)take((size_t)n * nrhs * sizeof(double));
@@ -371,15 +374,15 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     float *dinv_u = (float *)take(dinv_elems * sizeof(float));
     float *dmt_l = (float *)take(dinv_elems * sizeof(float));
     float *dmt_u = (float *)take(dinv_elems * sizeof(float));
-    st.kb = c.lu_kb<float>(n);
+    const bool tslu = c.work.tslu != 0;
+    st.kb = tslu ? c.kb_f32 : c.lu_kb<float>(n);
 
-    hipEvent_t e0, e1, e2;
-    SCIRS_HIP_CHECK(hipEventCreate(&e0));
-    SCIRS_HIP_CHECK(hipEventCreate(&e1));
-    SCIRS_HIP_CHECK(hipEventCreate(&e2));
+    scirs::CallEvents ev;
+    if (int erc = ev.create(3)) return erc;
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     SCIRS_HIP_CHECK(hipEventRecord(e0, s));
     // 1. FP32 copy of A and ||A||_inf; ||B_j||_inf
-    if ((rc = scirs_convert_rowsum(n, n, dA, lda, st.LU, n, st.rowsum, s))) return rc;
+    if ((rc = scirs_convert_rowsum(n, n, dA, lda, st.LU, ld32, st.rowsum, s))) return rc;
     if ((rc = scirs_rowsum_max(n, st.rowsum, st.norms + 192, s))) return rc;
     if ((rc = scirs_colmax_d(n, nrhs, dB, ldb, st.norms + 128, s))) return rc;
     // 2. low-precision LU
@@ -391,17 +394,19 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     }
     const int saved_mode = c.work.lowp_mode;
     c.work.lowp_mode = lowp;
-    rc = scirs_getrf_device<float>(n, n, st.LU, n, st.kb, c.work, s);
+    c.work.tslu_active = tslu;
+    rc = scirs_getrf_device<float>(n, n, st.LU, ld32, st.kb, c.work, s);
+    c.work.tslu_active = 0;
     c.work.lowp_mode = saved_mode;
     if (rc) return rc;
     if ((rc = scirs_perm_from_pairs(n, n, st.kb, c.work, st.perm, s))) return rc;
     // the FP32 solves apply inverted 64 x 64 diagonal blocks instead of the
     // dependent substitution inside each block (the factors only precondition)
     if (c.trsv_dinv) {
-        if ((rc = scirs_tri_block_inverse(n, st.LU, n, 1, 1, dinv_l, s))) return rc;
-        if ((rc = scirs_tri_block_inverse(n, st.LU, n, 0, 0, dinv_u, s))) return rc;
-        if ((rc = scirs_tri_block_coupling(n, st.LU, n, 1, dinv_l, dmt_l, s))) return rc;
-        if ((rc = scirs_tri_block_coupling(n, st.LU, n, 0, dinv_u, dmt_u, s))) return rc;
+        if ((rc = scirs_tri_block_inverse(n, st.LU, ld32, 1, 1, dinv_l, s))) return rc;
+        if ((rc = scirs_tri_block_inverse(n, st.LU, ld32, 0, 0, dinv_u, s))) return rc;
+        if ((rc = scirs_tri_block_coupling(n, st.LU, ld32, 1, dinv_l, dmt_l, s))) return rc;
+        if ((rc = scirs_tri_block_coupling(n, st.LU, ld32, 0, dinv_u, dmt_u, s))) return rc;
         st.dinv_l = dinv_l;
         st.dinv_u = dinv_u;
         st.dmt_l = dmt_l;
@@ -496,9 +501,6 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
         info->gmres_iters = gm_iters;
         info->backward_error = worst;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     return SCIRS_OK;
 }
 

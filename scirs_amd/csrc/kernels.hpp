@@ -16,6 +16,7 @@ struct ScirsWork {
     unsigned long long *xrj;   // [REP][2][KB_MAX*2]        panel row j
     int panel_nrep;            // replicas of the panel's published words (1..SCIRS_PANEL_REP_MAX)
     unsigned seq_next;         // sequence base of the next panel launch (host side)
+    size_t xw_words, xrow_words, xrj_words;
     int *abort;          // raised by a timed-out exchange
     int *pairs;          // [n_panels_max][2*KB_MAX][2] row gather lists (dst, src), absolute
     int *npairs;         // [n_panels_max]
@@ -37,7 +38,32 @@ struct ScirsWork {
     int poll_sleep;      // back-off of the panel's candidate poll (s_sleep(2) repetitions)
     hipStream_t side_stream;
     hipEvent_t ev_trsm, ev_upd, ev_start;
+    // tournament-pivoted panel (FP32 factorisation of the mixed solve)
+    int tslu;                  // the mixed solve's FP32 factorisation pivots by tournament (1) or exactly (0)
+    int tslu_active;           // set by the mixed solve around its factorisation (square, lda % 4 == 0)
+    unsigned long long *tp_cand;   // [TSLU_MAXLEV][TSLU_MAXNODE][64 + 1] candidate lists (sequence-tagged words)
+    int *tp_cnt;               // [TSLU_MAXLEV][TSLU_MAXNODE] arrival counters (zero between launches)
+    float *tp_uinv;            // 64 x 64 inverse of a sub-panel's U block
+    float *tp_urest;           // 64 x 64 the sub-panel's U rows on the panel's later columns
+    float *tp_top;             // 64 x SCIRS_KB_MAX final contents of the sub-panel's pivot rows
+    float *tp_stage;           // 64 x SCIRS_KB_MAX rows displaced from the sub-panel's top positions
+    int *tp_vlist;             // [64 + 1] positions the displaced rows move to, then their count
+    int *tp_map;               // [2 * TSLU_MAP] (position, original row) of the panel's moved rows, then a count
 };
+
+enum { TSLU_MAXLEV = 6, TSLU_MAXNODE = 512, TSLU_MAP = 512 };
+
+// Sequence number base for a launch that uses `span` consecutive tags of the
+// exchange buffers.  Before the 32-bit counter would wrap, every tagged buffer
+// is cleared on `s` and the count restarts at 1, so a stale word can never
+// carry a current tag.
+int scirs_next_seq(ScirsWork &w, unsigned span, hipStream_t s, unsigned *seq);
+
+// The tournament-pivoted factorisation of the kb-column panel at k0 of the n x n
+// FP32 matrix A (lda a multiple of 4): the panel's rows are permuted, the panel is
+// factored in place (unit L below, U on and above the diagonal) and w.pairs / w.npairs
+// receive its row interchanges, as the partial-pivot panel leaves them.
+int scirs_tslu_panel(float *A, long lda, int n, int k0, int kb, int panel_index, ScirsWork &w, hipStream_t s);
 
 enum {
     SCIRS_LOWP_FP32 = 0,   // FP32 operands (VALU FMA)

@@ -355,7 +355,10 @@ __global__ __launch_bounds__(NTK) void getf2_panel_kernel(PanelArgs<T> a)
             if (mine && lane == 0) {
                 s_p = ii; s_g = wgw; s_pv = vv;
                 if (timed_out) {
-                    __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // the 1-based column whose exchange timed out; info != 0 stops
+                    // every kernel queued after this one
+                    __hip_atomic_store(a.abort, a.k0 + j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(a.info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     s_stop = 1;
                 } else if (!(vv >= eps)) {   // |pivot| < eps: singular (lapack.rs:106-120)
                     s_stop = 1;

@@ -58,7 +58,8 @@ __device__ __forceinline__ double readlane<double>(double v, int k)
 // words per value).  Gives up (returns false) after spin_limit polls or when
 // another workgroup raised *abort.
 template <typename T, int QB>
-__device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB], unsigned spin_limit, int *abort)
+__device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB], unsigned spin_limit, int *abort,
+                          int *info)
 {
     constexpr int W = Tag<T>::W;
     unsigned spins = 0;
@@ -74,7 +75,12 @@ __device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB],
         if (__all(ok)) return true;
         ++spins;
         if (spins > spin_limit) {
-            if ((threadIdx.x & 63) == 0) __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // -1: a chained triangular solve timed out; info != 0 stops every
+            // kernel queued after this one
+            if ((threadIdx.x & 63) == 0) {
+                __hip_atomic_store(abort, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (info) __hip_atomic_store(info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             return false;
         }
         if ((spins & 255u) == 0 &&
@@ -87,7 +93,7 @@ __device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB],
 template <typename T, int QB, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A, long lda, T *X, long ldx, int n,
                                                          int nrhs, u64 *xch, unsigned seq, int *ticket, int *abort,
-                                                         unsigned spin_limit, const int *info,
+                                                         unsigned spin_limit, int *info,
                                                          const T *__restrict__ dinv, const T *__restrict__ dmt)
 {
     if (info && *info != 0) return;
@@ -137,7 +143,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int col = c0 + c + e;
-                ok = ok && poll_vals<T, QB>(xch + (size_t)col * QB * W, col < n, seq, xv[e], spin_limit, abort);
+                ok = ok && poll_vals<T, QB>(xch + (size_t)col * QB * W, col < n, seq, xv[e], spin_limit, abort, info);
             }
             if (!ok) { failed = true; break; }
 #pragma unroll
@@ -205,7 +211,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
             const int cb = LOWER ? b - 1 : nblk - b;
             const int c0 = cb * TB;
             T xk[QB];
-            if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort))
+            if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort, info))
                 return;
 #pragma unroll
             for (int k = 0; k < TB; ++k) {
@@ -221,7 +227,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         const int cb = LOWER ? b - 1 : nblk - b;
         const int c0 = cb * TB;
         T xk[QB];
-        if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort))
+        if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort, info))
             failed = true;
         if (failed) return;
 #pragma unroll
@@ -309,8 +315,8 @@ int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork 
     }
     int rc = scirs_fill_int(w.ticket, groups, 0, s);
     if (rc) return rc;
-    const unsigned seq = w.seq_next + 1u;
-    w.seq_next += 512u;
+    unsigned seq = 0;
+    if ((rc = scirs_next_seq(w, 512u, s, &seq))) return rc;
     const int nblk = (n + TB - 1) / TB;
     hipLaunchKernelGGL(kern, dim3(nblk, groups), dim3(NTC), lds, s, A, lda, X, ldx, n, nr, w.xtri, seq, w.ticket,
                        w.abort, w.spin_limit, w.info, dinv, dmt);

@@ -1,0 +1,763 @@
+// Tall-skinny LU with tournament pivoting (CALU: Grigori, Demmel & Xiang,
+// SIAM J. Matrix Anal. Appl. 32(4), 2011) for the FP32 factorisation of the
+// mixed-precision solve.
+//
+// The partial-pivot panel (panel.hpp) needs one device-wide exchange per
+// column, because the pivot of column j is the arg-max over all rows of the
+// panel (scirs2-linalg/src/lapack.rs:95-104).  The mixed solve refines its
+// result to FP64 and judges it by the backward error, so its FP32 factors may
+// use other pivots; the reference's pivot sequence is owed only by the FP64
+// `lu` / `solve` path.  Here a panel of kb <= 128 columns is factored as
+// sub-panels of TW = 64 columns.  Per sub-panel:
+//
+//   tour_kernel  — the rows below the sub-panel's diagonal are split into
+//                  blocks of 512 rows (one workgroup each).  Every block runs
+//                  Gaussian elimination with partial pivoting on its rows
+//                  (registers, two rows per thread) and keeps the TW rows it
+//                  picked.  The picks of up to 8 blocks (8 x 64 = 512 rows,
+//                  original values gathered from A) are eliminated again by
+//                  the last of those workgroups to finish — an arrival counter,
+//                  so no workgroup ever waits for one that may not be resident
+//                  — and so on up the tree until one workgroup holds the TW
+//                  pivot rows and, from its own elimination, their L11 \ U11.
+//                  That workgroup stages the rows the pivots displace, forms
+//                  U11^-1 and L11^-1, the pivot rows' U on the panel's later
+//                  columns (U12 = L11^-1 A12), and composes the panel's row
+//                  interchanges (the (destination, source) list of getrf).
+//   apply_kernel — every row of the sub-panel, in parallel: the pivot rows get
+//                  their final contents, every other row its multipliers
+//                  L21 = A21 U11^-1 and the update A22 -= L21 U12 of the
+//                  panel's later columns; displaced rows move to the positions
+//                  the pivots left.
+//
+// Two launches per sub-panel replace 64 device-wide exchanges.  The pivot
+// search keys are |a| with its 10 lowest bits replaced by the row index (the
+// lowest row wins among values equal in their top 21 bits); the pivot is
+// within 2^-13 relative of the column maximum, which is all the tournament
+// needs.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "api_internal.hpp"
+#include "prof.hpp"
+
+#include <algorithm>
+#include <type_traits>
+#include <utility>
+
+namespace tslu {
+
+typedef unsigned long long u64;
+constexpr int TW = 64;              // sub-panel width
+constexpr int NT = 256;             // threads per workgroup
+constexpr int RPT = 2;              // rows per thread in an elimination
+constexpr int NODE = NT * RPT;      // rows per tree node
+constexpr int FAN = NODE / TW;      // children per tree node
+constexpr int NW = NT / 64;         // waves per workgroup
+constexpr int KBT = 128;            // widest panel
+constexpr int PRP = KBT + 1;        // LDS pitch of a panel row
+constexpr int AR = 32;              // rows per apply workgroup
+
+static_assert(NW == 4, "the key gather reads four wave keys as one uint4");
+static_assert(NODE <= 1024, "row indexes live in the key's low 10 bits");
+
+__device__ __forceinline__ void st_agent(u64 *p, u64 v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 ld_agent(const u64 *p)
+{
+    return __hip_atomic_load(const_cast<u64 *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// search key of value x in local row r: larger |x| wins, then the lower row;
+// 0 is "no candidate"
+__device__ __forceinline__ unsigned mkkey(float x, int r)
+{
+    return (__float_as_uint(x) & 0x7FFFFC00u) | 0x200u | (unsigned)(NODE - 1 - r);
+}
+
+template <int CTRL, int RM>
+__device__ __forceinline__ unsigned dmax(unsigned v)
+{
+    return max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, RM, 0xf, false));
+}
+
+// wave-wide maximum (row shifts 1, 2, 4, 8, then the row broadcasts 15 and 31:
+// lane 63 ends with the maximum)
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
+{
+    v = dmax<0x111, 0xf>(v);
+    v = dmax<0x112, 0xf>(v);
+    v = dmax<0x114, 0xf>(v);
+    v = dmax<0x118, 0xf>(v);
+    v = dmax<0x142, 0xa>(v);
+    v = dmax<0x143, 0xc>(v);
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Rows of one elimination: thread t holds local rows t and t + NT in registers.
+struct Rows {
+    float v[RPT][TW];
+    int step[RPT];   // -1 not yet picked, >= 0 the step that picked it, -2 no row
+};
+
+// v <- columns c0 .. c0 + w - 1 of the given row of A (zero beyond w); 16-byte loads
+__device__ __forceinline__ void load_row(float (&v)[TW], const float *src, int w)
+{
+#pragma unroll
+    for (int c4 = 0; c4 < TW / 4; ++c4) {
+        if (4 * c4 + 3 < w) {
+            const float4 q = *reinterpret_cast<const float4 *>(src + 4 * c4);
+            v[4 * c4] = q.x; v[4 * c4 + 1] = q.y; v[4 * c4 + 2] = q.z; v[4 * c4 + 3] = q.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[4 * c4 + u] = (4 * c4 + u < w) ? src[4 * c4 + u] : 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ void rows_load(Rows &R, const float *A, long lda, int c0, int w, const int *rid, int nr)
+{
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int x = 0; x < RPT; ++x) {
+        const int i = t + x * NT;
+        if (i < nr) {
+            load_row(R.v[x], A + (long)rid[i] * lda + c0, w);
+            R.step[x] = -1;
+        } else {
+#pragma unroll
+            for (int c = 0; c < TW; ++c) R.v[x][c] = 0.0f;
+            R.step[x] = -2;
+        }
+    }
+}
+
+struct Xch {
+    unsigned key[2][NW];
+    float4 slot[2][NW][TW / 4];
+    int sel[TW];
+};
+
+// Gaussian elimination with partial pivoting on the rows in R (columns 0..w-1).
+// Step j picks the unpicked row with the largest key of column j (one barrier:
+// each wave publishes its best key and that row's columns j.., every thread
+// reads the four keys and the winner's row) and every unpicked row subtracts
+// its multiple of the pivot row from columns j+1.. .  The 64 steps are unrolled
+// (column j is a register, not an index) and never branched around: a step
+// with nothing to pick (past w, or no unpicked row left) runs with zero keys
+// and a zero multiplier.  Picked rows stop changing.  Returns the number of
+// picks; X.sel[j] is the local row picked at step j; with U != nullptr the
+// pivot rows' U (their values at their pick) is kept in U[j][j..TW-1].
+template <int J>
+__device__ __forceinline__ void gepp_step(Rows &R, int w, Xch &X, float (*U)[TW + 1], int &nsel)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int par = J & 1, J4 = J & ~3;
+    const bool live = J < w;
+    unsigned k[RPT];
+    unsigned kk = 0u;
+#pragma unroll
+    for (int x = 0; x < RPT; ++x) {
+        k[x] = (live && R.step[x] == -1) ? mkkey(R.v[x][J], t + x * NT) : 0u;
+        kk = max(kk, k[x]);
+    }
+    const unsigned wm = wave_max_u32(kk);
+    if (lane == 0) X.key[par][wave] = wm;
+    float4 *slot = X.slot[par][wave];
+#pragma unroll
+    for (int x = 0; x < RPT; ++x) {
+        if (wm != 0u && k[x] == wm) {
+#pragma unroll
+            for (int c4 = J4 / 4; c4 < TW / 4; ++c4)
+                slot[c4] = make_float4(R.v[x][4 * c4], R.v[x][4 * c4 + 1], R.v[x][4 * c4 + 2], R.v[x][4 * c4 + 3]);
+        }
+    }
+    __syncthreads();
+    const uint4 kv = *reinterpret_cast<const uint4 *>(X.key[par]);
+    const unsigned gm = (unsigned)__builtin_amdgcn_readfirstlane((int)max(max(kv.x, kv.y), max(kv.z, kv.w)));
+    const bool pick = gm != 0u;
+    if (!pick && J < nsel) nsel = J;   // no unpicked row left (or past w)
+    const int ww = __builtin_amdgcn_readfirstlane(kv.x == gm ? 0 : kv.y == gm ? 1 : kv.z == gm ? 2 : 3);
+    const int rr = pick ? NODE - 1 - (int)(gm & (NODE - 1)) : -1;
+    const float4 *ps = X.slot[par][ww];
+    float p[TW];
+#pragma unroll
+    for (int c4 = J4 / 4; c4 < TW / 4; ++c4) {
+        const float4 q = ps[c4];
+        p[4 * c4] = q.x; p[4 * c4 + 1] = q.y; p[4 * c4 + 2] = q.z; p[4 * c4 + 3] = q.w;
+    }
+    if (pick && U != nullptr && t >= J && t < TW) U[J][t] = reinterpret_cast<const float *>(ps)[t];
+    if (pick && t == 0) X.sel[J] = rr;
+#pragma unroll
+    for (int x = 0; x < RPT; ++x) R.step[x] = (t + x * NT == rr) ? J : R.step[x];
+    const float rp = (pick && p[J] != 0.0f) ? __builtin_amdgcn_rcpf(p[J]) : 0.0f;
+#pragma unroll
+    for (int x = 0; x < RPT; ++x) {
+        const bool un = R.step[x] == -1;
+        const float l = un ? R.v[x][J] * rp : 0.0f;
+        R.v[x][J] = un ? l : R.v[x][J];
+#pragma unroll
+        for (int c = J + 1; c < TW; ++c) R.v[x][c] = fmaf(-l, p[c], R.v[x][c]);
+    }
+}
+
+template <int J>
+__device__ __forceinline__ void gepp_steps(Rows &R, int w, Xch &X, float (*U)[TW + 1], int &nsel)
+{
+    if constexpr (J < TW) {
+        gepp_step<J>(R, w, X, U, nsel);
+        gepp_steps<J + 1>(R, w, X, U, nsel);
+    }
+}
+
+__device__ __forceinline__ int rows_gepp(Rows &R, int w, Xch &X, float (*U)[TW + 1])
+{
+    int nsel = w;
+    gepp_steps<0>(R, w, X, U, nsel);
+    return nsel;
+}
+
+// X = inverse of the upper-triangular TW x TW matrix U (entries (i, j) of U are
+// M[i * ldm + j], or M[j * ldm + i] when TRANS; UNIT: unit diagonal).  16 x 16
+// diagonal blocks by substitution (one wave each, lane = column), then the
+// blocks above the diagonal by increasing distance:
+// X_bc = -X_bb sum_{b<k<=c} U_bk X_kc.  X is TW x TW with pitch TW.
+template <bool TRANS, bool UNIT>
+__device__ void tri_inv_upper(const float *M, int ldm, float *X, float *T)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    auto U = [&](int i, int j) { return TRANS ? M[j * ldm + i] : M[i * ldm + j]; };
+    for (int q = t; q < TW * TW; q += NT) X[q] = 0.0f;
+    __syncthreads();
+    {
+        const int b0 = wave * 16;
+        if (lane < 16) {
+            float x[16];
+#pragma unroll
+            for (int i = 15; i >= 0; --i) {
+                float s = (i == lane) ? 1.0f : 0.0f;
+#pragma unroll
+                for (int k = i + 1; k < 16; ++k) s -= U(b0 + i, b0 + k) * x[k];
+                x[i] = UNIT ? s : s / U(b0 + i, b0 + i);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) X[(b0 + i) * TW + b0 + lane] = x[i];
+        }
+    }
+    __syncthreads();
+    for (int d = 1; d < 4; ++d) {
+        const int nblk = 4 - d;
+        // T_b = sum_{b<k<=c} U_bk X_kc for the blocks (b, c = b + d)
+        for (int q = t; q < nblk * 256; q += NT) {
+            const int b = q >> 8, e = q & 255, i = e >> 4, jj = e & 15, c = b + d;
+            float s = 0.0f;
+            for (int k = (b + 1) * 16; k < (c + 1) * 16; ++k) s += U(b * 16 + i, k) * X[k * TW + c * 16 + jj];
+            T[q] = s;
+        }
+        __syncthreads();
+        for (int q = t; q < nblk * 256; q += NT) {
+            const int b = q >> 8, e = q & 255, i = e >> 4, jj = e & 15, c = b + d;
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) s += X[(b * 16 + i) * TW + b * 16 + k] * T[(b << 8) + k * 16 + jj];
+            X[(b * 16 + i) * TW + c * 16 + jj] = -s;
+        }
+        __syncthreads();
+    }
+}
+
+struct TourArgs {
+    float *A; long lda;
+    int k0, kb;         // the panel: columns (and first row position) k0, width kb
+    int c0, w;          // this sub-panel: columns c0 .. c0 + w - 1, rows c0 .. n - 1
+    int n;
+    int nlev;           // tree levels above the leaves (0: one leaf, the root)
+    int lev_n[TSLU_MAXLEV + 1];
+    u64 *cand; int *cnt; unsigned seq;
+    float *uinv, *urest, *top, *stage;
+    int *vlist, *map;
+    int first, last;
+    int *pairs, *npairs;
+    int *info; double *info_val; int *abort;
+    unsigned spin_limit;
+};
+
+// LDS of the root's work (dynamic; the tree's workgroups carry it too)
+struct RootLds {
+    float pr[TW][PRP];      // pivot rows' panel rows; their current columns replaced by L11 \ U11
+    float u11[TW][TW + 1];  // the pivot rows' U (upper triangle) from the root's elimination
+    float l11[TW][TW + 1];  // unit lower L11 = A11 U11^-1
+    float uinv[TW * TW];
+    float linvT[TW * TW];   // (L11^-1)^T: the inverse of L11^T
+    float tmp[3 * 256];
+    int gp[TW];
+    int dlist[TW], vlist[TW];
+    int mpos[TSLU_MAP], mlab[TSLU_MAP];
+    int mv_dst[2 * TW], mv_src[2 * TW], mv_lab[2 * TW];
+    int nd, nmv, mcount, mcount0, bad, npairs;
+};
+
+__device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int *rid, Xch &X, RootLds &L)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int w = a.w, r0 = a.c0, cur0 = a.c0 - a.k0, rest0 = cur0 + w, nrest = a.kb - rest0;
+    // 1. pivot positions in pick order; U11 is in L.u11 (upper triangle)
+    if (t < nsel) L.gp[t] = rid[X.sel[t]];
+    if (t == 0) L.bad = nsel < w ? nsel : TW;
+    __syncthreads();
+    // 2. breakdown: |u_jj| < eps (the absolute test of lapack.rs:106-120); the
+    //    mixed solve then falls back to FP64 LU, which decides singularity
+    if (t < nsel && t < w && !(fabsf(L.u11[t][t]) >= FLT_EPSILON)) atomicMin(&L.bad, t);
+    __syncthreads();
+    if (L.bad < w) {
+        if (t == 0) {
+            *a.info = a.c0 + L.bad + 1;
+            *a.info_val = L.bad < nsel ? (double)fabsf(L.u11[L.bad][L.bad]) : 0.0;
+        }
+        return;
+    }
+    // U11 padded with the identity beyond w, zero below the diagonal
+    for (int q = t; q < TW * TW; q += NT) {
+        const int i = q / TW, c = q % TW;
+        if (c < i || i >= w || c >= w) L.u11[i][c] = (i == c) ? 1.0f : 0.0f;
+    }
+    // 3. positions: top r0 .. r0 + w - 1 receive the pivots in pick order; the
+    //    top rows that are not pivots (dlist) move to the positions of the
+    //    pivots from below (vlist)
+    if (wave == 0) {
+        const bool top_is_pivot = [&] {
+            if (lane >= w) return true;
+            for (int j = 0; j < w; ++j)
+                if (L.gp[j] == r0 + lane) return true;
+            return false;
+        }();
+        const unsigned long long bd = __ballot(lane < w && !top_is_pivot);
+        if (lane < w && !top_is_pivot) L.dlist[__popcll(bd & ((1ull << lane) - 1ull))] = r0 + lane;
+        const bool below = lane < w && L.gp[lane] >= r0 + w;
+        const unsigned long long bv = __ballot(below);
+        if (below) L.vlist[__popcll(bv & ((1ull << lane) - 1ull))] = L.gp[lane];
+        if (lane == 0) L.nd = __popcll(bd);
+    }
+    __syncthreads();
+    const int nd = L.nd;
+    // 4. the pivot rows' other columns (earlier sub-panels' L, later columns' A12),
+    //    and the displaced rows staged whole
+    for (int q = t; q < w * a.kb; q += NT) {
+        const int j = q / a.kb, c = q - j * a.kb;
+        L.pr[j][c] = a.A[(long)L.gp[j] * a.lda + a.k0 + c];
+    }
+    for (int q = t; q < nd * a.kb; q += NT) {
+        const int i = q / a.kb, c = q - i * a.kb;
+        a.stage[i * SCIRS_KB_MAX + c] = a.A[(long)L.dlist[i] * a.lda + a.k0 + c];
+    }
+    if (t < nd) a.vlist[t] = L.vlist[t];
+    if (t == 0) a.vlist[TW] = nd;
+    __syncthreads();
+    // 5. U11^-1; L11 = A11 U11^-1 (strictly lower part, from the pivot rows'
+    //    values; the elimination kept only their U); L11^-1
+    tri_inv_upper<false, false>(&L.u11[0][0], TW + 1, L.uinv, L.tmp);
+    {
+        const int ti = t >> 4, tj = t & 15;
+        float acc[4][4] = {};
+        for (int k = 0; k < w; ++k) {
+            float av[4], uv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = (4 * ti + u < w) ? L.pr[4 * ti + u][cur0 + k] : 0.0f;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) uv[v] = L.uinv[k * TW + 4 * tj + v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(av[u], uv[v], acc[u][v]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 4 * ti + u, c = 4 * tj + v;
+                L.l11[i][c] = (i == c || i >= w) ? (i == c ? 1.0f : 0.0f) : (c < i ? acc[u][v] : 0.0f);
+            }
+    }
+    __syncthreads();
+    tri_inv_upper<true, true>(&L.l11[0][0], TW + 1, L.linvT, L.tmp);
+    // the pivot rows' current columns: L11 below the diagonal, U11 on and above
+    for (int q = t; q < w * w; q += NT) {
+        const int i = q / w, c = q - i * w;
+        L.pr[i][cur0 + c] = c < i ? L.l11[i][c] : L.u11[i][c];
+    }
+    // 6. U12 = L11^-1 A12 on the later columns (4 x 4 register tiles)
+    float u12[4][4];
+    const int ti = t >> 4, tj = t & 15;   // rows 4 ti .., columns 4 tj ..
+    const bool have12 = 4 * tj < nrest && 4 * ti < w;
+    if (have12) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) u12[u][v] = 0.0f;
+        for (int k = 0; k < w; ++k) {
+            float lv[4], av[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) lv[u] = L.linvT[k * TW + 4 * ti + u];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) av[v] = (4 * tj + v < nrest) ? L.pr[k][rest0 + 4 * tj + v] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) u12[u][v] = fmaf(lv[u], av[v], u12[u][v]);
+        }
+    }
+    __syncthreads();
+    if (have12) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (4 * ti + u < w && 4 * tj + v < nrest) {
+                    L.pr[4 * ti + u][rest0 + 4 * tj + v] = u12[u][v];
+                    a.urest[(4 * ti + u) * TW + 4 * tj + v] = u12[u][v];
+                }
+    }
+    for (int q = t; q < TW * TW; q += NT) a.uinv[q] = L.uinv[q];
+    __syncthreads();
+    for (int q = t; q < w * a.kb; q += NT) {
+        const int j = q / a.kb, c = q - j * a.kb;
+        a.top[j * SCIRS_KB_MAX + c] = L.pr[j][c];
+    }
+    // 7. the panel's interchanges: (position, original row) for every moved row
+    if (t == 0) {
+        L.mcount = a.first ? 0 : a.map[2 * TSLU_MAP];
+        L.nmv = 0;
+    }
+    __syncthreads();
+    const int mc0 = L.mcount;
+    for (int q = t; q < mc0; q += NT) {
+        L.mpos[q] = a.map[q];
+        L.mlab[q] = a.map[TSLU_MAP + q];
+    }
+    if (t < w && L.gp[t] != r0 + t) {
+        const int m = atomicAdd(&L.nmv, 1);
+        L.mv_dst[m] = r0 + t;
+        L.mv_src[m] = L.gp[t];
+    } else if (t >= TW && t < TW + nd) {
+        const int m = atomicAdd(&L.nmv, 1);
+        L.mv_dst[m] = L.vlist[t - TW];
+        L.mv_src[m] = L.dlist[t - TW];
+    }
+    __syncthreads();
+    const int nmv = L.nmv;
+    if (t < nmv) {   // original row of each source
+        const int src = L.mv_src[t];
+        int lab = src;
+        for (int e = 0; e < mc0; ++e)
+            if (L.mpos[e] == src) lab = L.mlab[e];
+        L.mv_lab[t] = lab;
+    }
+    __syncthreads();
+    if (t < nmv) {
+        const int dst = L.mv_dst[t];
+        int at = -1;
+        for (int e = 0; e < mc0; ++e)
+            if (L.mpos[e] == dst) at = e;
+        if (at < 0) {
+            at = atomicAdd(&L.mcount, 1);
+            L.mpos[at] = dst;
+        }
+        L.mlab[at] = L.mv_lab[t];
+    }
+    __syncthreads();
+    const int mc = L.mcount;
+    if (a.last) {
+        if (t == 0) L.npairs = 0;
+        __syncthreads();
+        for (int e = t; e < mc; e += NT) {
+            if (L.mpos[e] != L.mlab[e]) {
+                const int q = atomicAdd(&L.npairs, 1);
+                a.pairs[2 * q] = L.mpos[e];
+                a.pairs[2 * q + 1] = L.mlab[e];
+            }
+        }
+        __syncthreads();
+        if (t == 0) *a.npairs = L.npairs;
+    } else {
+        for (int e = t; e < mc; e += NT) {
+            a.map[e] = L.mpos[e];
+            a.map[TSLU_MAP + e] = L.mlab[e];
+        }
+        if (t == 0) a.map[2 * TSLU_MAP] = mc;
+    }
+}
+
+__global__ __launch_bounds__(NT) void tour_kernel(TourArgs a)
+{
+    if (*a.info != 0) return;
+    __shared__ Xch X;
+    __shared__ int rid[NODE];
+    __shared__ int s_cnt[FAN], s_off[FAN + 1];
+    __shared__ int s_go;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    RootLds &L = *reinterpret_cast<RootLds *>(smem_raw);
+    const int t = threadIdx.x;
+    int node = blockIdx.x, lev = 0;
+    const int base = a.c0 + node * NODE;
+    int nr = min(NODE, a.n - base);
+    for (int i = t; i < nr; i += NT) rid[i] = base + i;
+    bool eliminate = true;
+    int nsel = 0;
+    while (true) {
+        if (eliminate) {
+            __syncthreads();
+            const bool root = lev == a.nlev;
+            if (root)
+                for (int q = t; q < TW * (TW + 1); q += NT) (&L.u11[0][0])[q] = 0.0f;
+            Rows R;
+            rows_load(R, a.A, a.lda, a.c0, a.w, rid, nr);
+            nsel = rows_gepp(R, a.w, X, root ? L.u11 : nullptr);
+        }
+        if (lev == a.nlev) break;
+        __syncthreads();
+        const int parent = node / FAN;
+        const int nch = min(FAN, a.lev_n[lev] - parent * FAN);
+        if (nch == 1) {
+            // a sole child's picks stand for its parent's: keep them, in pick order
+            const int r = t < nsel ? rid[X.sel[t]] : 0;
+            __syncthreads();
+            if (t < nsel) {
+                rid[t] = r;
+                X.sel[t] = t;
+            }
+            nr = nsel;
+            node = parent;
+            ++lev;
+            eliminate = lev == a.nlev;   // the root always eliminates (it keeps U)
+            continue;
+        }
+        // publish this node's picks (tagged words), then count the arrival
+        u64 *slot = a.cand + ((size_t)lev * TSLU_MAXNODE + node) * (TW + 1);
+        if (t <= nsel) {
+            const unsigned val = t == 0 ? (unsigned)nsel : (unsigned)rid[X.sel[t - 1]];
+            st_agent(slot + t, ((u64)a.seq << 32) | val);
+        }
+        __syncthreads();
+        if (t == 0) {
+            int *cnt = a.cnt + (lev + 1) * TSLU_MAXNODE + parent;
+            const int old = __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_go = old == nch - 1;
+            if (s_go) __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (!s_go) return;   // a sibling still works: the last to arrive merges
+        // the last arrival merges the children's picks: their counts, then rows
+        const u64 *cb = a.cand + ((size_t)lev * TSLU_MAXNODE + parent * FAN) * (TW + 1);
+        node = parent;
+        ++lev;
+        bool late = false;
+        if (t < nch) {
+            unsigned spins = 0;
+            u64 wv;
+            while (((wv = ld_agent(cb + t * (TW + 1))) >> 32) != a.seq)
+                if (++spins > a.spin_limit) { late = true; break; }
+            s_cnt[t] = late ? 0 : (int)(unsigned)wv;
+        }
+        if (__syncthreads_or(late)) {
+            if (t == 0) {
+                __hip_atomic_store(a.abort, a.c0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return;
+        }
+        if (t == 0) {
+            int o = 0;
+            for (int c = 0; c < nch; ++c) { s_off[c] = o; o += min(s_cnt[c], TW); }
+            s_off[nch] = o;
+        }
+        __syncthreads();
+        nr = s_off[nch];
+        for (int c = 0; c < nch; ++c) {
+            for (int j = t; j < s_off[c + 1] - s_off[c]; j += NT) {
+                unsigned spins = 0;
+                u64 wv;
+                while (((wv = ld_agent(cb + c * (TW + 1) + 1 + j)) >> 32) != a.seq)
+                    if (++spins > a.spin_limit) { late = true; break; }
+                rid[s_off[c] + j] = (int)(unsigned)wv;
+            }
+        }
+        if (__syncthreads_or(late)) {
+            if (t == 0) {
+                __hip_atomic_store(a.abort, a.c0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return;
+        }
+        eliminate = true;
+    }
+    __syncthreads();
+    root_work(a, nsel, rid, X, L);
+}
+
+struct ApplyArgs {
+    float *A; long lda;
+    int k0, kb, c0, w, n;
+    const float *uinv, *urest, *top, *stage;
+    const int *vlist;
+    const int *info;
+};
+
+// Rows c0 .. n-1 of the panel, AR per workgroup: the sub-panel's pivot rows
+// take their final contents; every other row (in place, or a displaced row
+// from the staging buffer at the position a pivot left) gets
+//   L21 = A21 U11^-1   (the sub-panel's columns)
+//   A22 -= L21 U12     (the panel's later columns).
+__global__ __launch_bounds__(NT) void apply_kernel(ApplyArgs a)
+{
+    if (*a.info != 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *Ui = reinterpret_cast<float *>(smem_raw);   // TW x TW
+    float *Ur = Ui + TW * TW;                          // TW x TW (pitch TW)
+    float *Ar = Ur + TW * TW;                          // AR x PRP
+    float *Lr = Ar + AR * PRP;                         // AR x (TW + 1)
+    __shared__ int sidx[AR];
+    const int t = threadIdx.x;
+    const int w = a.w, cur0 = a.c0 - a.k0, rest0 = cur0 + w, nrest = a.kb - rest0;
+    const int d0 = a.c0 + blockIdx.x * AR, top_end = a.c0 + w;
+    for (int q = t; q < TW * TW; q += NT) {
+        const int r = q / TW, c = q % TW;
+        Ui[q] = (r < w && c < w) ? a.uinv[q] : 0.0f;
+        Ur[q] = (r < w && c < nrest) ? a.urest[q] : 0.0f;
+    }
+    if (t < AR) sidx[t] = -1;
+    __syncthreads();
+    const int nv = a.vlist[TW];
+    for (int i = t; i < nv; i += NT) {
+        const int pos = a.vlist[i];
+        if (pos >= d0 && pos < d0 + AR) sidx[pos - d0] = i;
+    }
+    __syncthreads();
+    for (int q = t; q < AR * a.kb; q += NT) {
+        const int rho = q / a.kb, c = q - rho * a.kb, d = d0 + rho;
+        float v = 0.0f;
+        if (d < a.n) {
+            if (d < top_end) v = a.top[(d - a.c0) * SCIRS_KB_MAX + c];
+            else if (sidx[rho] >= 0) v = a.stage[sidx[rho] * SCIRS_KB_MAX + c];
+            else if (c >= cur0) v = a.A[(long)d * a.lda + a.k0 + c];
+        }
+        Ar[rho * PRP + c] = v;
+    }
+    __syncthreads();
+    const int rho = t >> 3, g = t & 7, d = d0 + rho;
+    const bool work = d >= top_end && d < a.n;
+    if (work) {
+        for (int c = g; c < w; c += 8) {
+            float s = 0.0f;
+            for (int k = 0; k <= c; ++k) s = fmaf(Ar[rho * PRP + cur0 + k], Ui[k * TW + c], s);
+            Lr[rho * (TW + 1) + c] = s;
+        }
+    }
+    __syncthreads();
+    if (work) {
+        for (int q = g; q < nrest; q += 8) {
+            float s = Ar[rho * PRP + rest0 + q];
+            for (int k = 0; k < w; ++k) s = fmaf(-Lr[rho * (TW + 1) + k], Ur[k * TW + q], s);
+            Ar[rho * PRP + rest0 + q] = s;
+        }
+        for (int c = g; c < w; c += 8) Ar[rho * PRP + cur0 + c] = Lr[rho * (TW + 1) + c];
+    }
+    __syncthreads();
+    for (int q = t; q < AR * a.kb; q += NT) {
+        const int r = q / a.kb, c = q - r * a.kb, dd = d0 + r;
+        if (dd < a.n && (c >= cur0 || dd < top_end || sidx[r] >= 0))
+            a.A[(long)dd * a.lda + a.k0 + c] = Ar[r * PRP + c];
+    }
+}
+
+} // namespace tslu
+
+int scirs_tslu_panel(float *A, long lda, int n, int k0, int kb, int panel_index, ScirsWork &w, hipStream_t s)
+{
+    using namespace tslu;
+    if (kb <= 0 || kb > KBT || (lda & 3) || (k0 & 3)) {
+        scirs_set_error("tslu panel: unsupported shape (kb=%d lda=%ld k0=%d)", kb, lda, k0);
+        return SCIRS_ERR_ARG;
+    }
+    const size_t lds_root = sizeof(RootLds);
+    if (int rc = scirs_allow_lds((const void *)tour_kernel, lds_root)) return rc;
+    const size_t lds_apply = ((size_t)2 * TW * TW + (size_t)AR * PRP + (size_t)AR * (TW + 1)) * sizeof(float);
+    if (int rc = scirs_allow_lds((const void *)apply_kernel, lds_apply)) return rc;
+    const int nsub = (kb + TW - 1) / TW;
+    for (int sp = 0; sp < nsub; ++sp) {
+        const int c0 = k0 + sp * TW;
+        const int wc = std::min(TW, k0 + kb - c0);
+        const int mrows = n - c0;
+        TourArgs ta{};
+        ta.A = A; ta.lda = lda; ta.k0 = k0; ta.kb = kb; ta.c0 = c0; ta.w = wc; ta.n = n;
+        ta.lev_n[0] = (mrows + NODE - 1) / NODE;
+        int nlev = 0;
+        while (ta.lev_n[nlev] > 1) {
+            if (nlev + 1 >= TSLU_MAXLEV || ta.lev_n[nlev] > TSLU_MAXNODE) {
+                scirs_set_error("tslu panel: %d rows exceed the tournament tree", mrows);
+                return SCIRS_ERR_UNSUPPORTED;
+            }
+            ta.lev_n[nlev + 1] = (ta.lev_n[nlev] + FAN - 1) / FAN;
+            ++nlev;
+        }
+        ta.nlev = nlev;
+        ta.cand = w.tp_cand; ta.cnt = w.tp_cnt;
+        if (int rc = scirs_next_seq(w, 1u, s, &ta.seq)) return rc;
+        ta.uinv = w.tp_uinv; ta.urest = w.tp_urest; ta.top = w.tp_top; ta.stage = w.tp_stage;
+        ta.vlist = w.tp_vlist; ta.map = w.tp_map;
+        ta.first = sp == 0; ta.last = sp == nsub - 1;
+        ta.pairs = w.pairs + (size_t)panel_index * 4 * SCIRS_KB_MAX;
+        ta.npairs = w.npairs + panel_index;
+        ta.info = w.info; ta.info_val = w.info_val; ta.abort = w.abort; ta.spin_limit = w.spin_limit;
+        hipLaunchKernelGGL(tour_kernel, dim3(ta.lev_n[0]), dim3(NT), lds_root, s, ta);
+        SCIRS_HIP_CHECK(hipGetLastError());
+        ApplyArgs aa{};
+        aa.A = A; aa.lda = lda; aa.k0 = k0; aa.kb = kb; aa.c0 = c0; aa.w = wc; aa.n = n;
+        aa.uinv = w.tp_uinv; aa.urest = w.tp_urest; aa.top = w.tp_top; aa.stage = w.tp_stage;
+        aa.vlist = w.tp_vlist; aa.info = w.info;
+        hipLaunchKernelGGL(apply_kernel, dim3((mrows + AR - 1) / AR), dim3(NT), lds_apply, s, aa);
+        SCIRS_HIP_CHECK(hipGetLastError());
+    }
+    return SCIRS_OK;
+}
+
+namespace {
+int lowp_of_mode(int mode)
+{
+    switch (mode) {
+    case SCIRS_MODE_BF16: return SCIRS_LOWP_BF16;
+    case SCIRS_MODE_FP16: return SCIRS_LOWP_FP16;
+    case SCIRS_MODE_BF16X3: return SCIRS_LOWP_BF16X3;
+    case SCIRS_MODE_FP32: return SCIRS_LOWP_FP32;
+    case SCIRS_MODE_BF16X6: return SCIRS_LOWP_BF16X6;
+    default: return -1;
+    }
+}
+} // namespace
+
+extern "C" int scirs_gpu_sgetrf_tp_dev(int n, float *dA, int64_t lda, int64_t *dperm, int mode, void *stream)
+{
+    using namespace scirs;
+    const int lowp = lowp_of_mode(mode == SCIRS_MODE_AUTO ? SCIRS_MODE_BF16X6 : mode);
+    if (n <= 0 || !dA || !dperm || lda < n || (lda & 3) || lowp < 0 || lowp == SCIRS_LOWP_FP16) {
+        scirs_set_error("sgetrf_tp_dev: invalid arguments (n=%d lda=%lld mode=%d; lda a multiple of 4, FP16 "
+                        "needs the mixed solve's scaling)", n, (long long)lda, mode);
+        return SCIRS_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    Context &c = ctx();
+    if (int rc = c.init(0)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = c.reset_flags(s)) return rc;
+    const int saved = c.work.lowp_mode;
+    c.work.lowp_mode = lowp;
+    c.work.tslu_active = 1;
+    int rc = scirs_getrf_device<float>(n, n, dA, lda, c.kb_f32, c.work, s);
+    c.work.tslu_active = 0;
+    c.work.lowp_mode = saved;
+    if (rc) return rc;
+    if ((rc = scirs_perm_from_pairs(n, n, c.kb_f32, c.work, dperm, s))) return rc;
+    int st = 0;
+    if ((rc = c.read_flags(s, &st, nullptr))) return rc;
+    return st;
+}

@@ -168,7 +168,13 @@ def gmres_dev(a, b, x, use_x0=False, max_iter=1000, tol=1e-10, restart=0):
     """gmres on device tensors (float64, cuda): x holds x0 when use_x0 and the
     solution on return.  Returns (iterations, residual_norm, converged)."""
     lib = N.load()
-    n = a.shape[0]
+    n = a.shape[0] if hasattr(a, "shape") and len(a.shape) == 2 else -1
+    if n <= 0 or a.shape[1] != n:
+        # the reference's shape text (solvers/iterative.rs:327-333)
+        raise ValueError(f"gmres_dev: Expected square matrix, got shape {tuple(getattr(a, 'shape', ()))}")
+    _dev_arg("gmres_dev", "a", a, n, n)
+    _dev_arg("gmres_dev", "b", b, n)
+    _dev_arg("gmres_dev", "x", x, n)
     it = ctypes.c_int(0)
     conv = ctypes.c_int(0)
     res = ctypes.c_double(0.0)
@@ -260,6 +266,28 @@ def _tp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _dev_arg(fn, name, t, rows, cols=None, dtype=None):
+    """A device operand as the C ABI reads it: a CUDA tensor of the given dtype
+    (float64 unless stated), `rows` rows (and `cols` columns when 2-D) with
+    unit column stride; a transposed or strided view would be read wrongly and
+    a short one past its end, so they are refused here."""
+    import torch
+    dtype = torch.float64 if dtype is None else dtype
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{fn}: {name} must be a CUDA tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{fn}: {name} must be {dtype}, got {t.dtype}")
+    if cols is None:
+        if t.dim() != 1 or t.shape[0] != rows or (rows > 1 and t.stride(0) != 1):
+            raise ValueError(f"{fn}: {name} must be a contiguous vector of length {rows}, got {tuple(t.shape)}")
+    else:
+        if t.dim() != 2 or t.shape[0] != rows or t.shape[1] != cols or (cols > 1 and t.stride(1) != 1) \
+                or t.stride(0) < max(cols, 1):
+            raise ValueError(f"{fn}: {name} must be {rows} x {cols} with unit column stride, "
+                             f"got shape {tuple(t.shape)} strides {t.stride()}")
+    return t
+
+
 def _stream_ptr():
     import torch
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -269,6 +297,14 @@ def gesv_dev(A, B, X, LU=None, perm=None):
     """A, B, X: contiguous float64 CUDA tensors (n x n, n x nrhs, n x nrhs)."""
     lib = N.load()
     n, nrhs = B.shape
+    _dev_arg("gesv_dev", "A", A, n, n)
+    _dev_arg("gesv_dev", "B", B, n, nrhs)
+    _dev_arg("gesv_dev", "X", X, n, nrhs)
+    if LU is not None:
+        _dev_arg("gesv_dev", "LU", LU, n, n)
+    if perm is not None:
+        import torch
+        _dev_arg("gesv_dev", "perm", perm, n, dtype=torch.int64)
     info = N.SolveInfo()
     rc = lib.scirs_gpu_dgesv_dev(n, nrhs, _tp(A), A.stride(0), _tp(B), B.stride(0), _tp(X), X.stride(0),
                                  _tp(LU) if LU is not None else None, LU.stride(0) if LU is not None else 0,
@@ -280,6 +316,11 @@ def gesv_dev(A, B, X, LU=None, perm=None):
 def posv_dev(A, B, X, L=None):
     lib = N.load()
     n, nrhs = B.shape
+    _dev_arg("posv_dev", "A", A, n, n)
+    _dev_arg("posv_dev", "B", B, n, nrhs)
+    _dev_arg("posv_dev", "X", X, n, nrhs)
+    if L is not None:
+        _dev_arg("posv_dev", "L", L, n, n)
     info = N.SolveInfo()
     rc = lib.scirs_gpu_dposv_dev(n, nrhs, _tp(A), A.stride(0), _tp(B), B.stride(0), _tp(X), X.stride(0),
                                  _tp(L) if L is not None else None, L.stride(0) if L is not None else 0,
@@ -291,6 +332,9 @@ def posv_dev(A, B, X, L=None):
 def gesv_ir_dev(A, B, X, mode=N.MODE_AUTO, max_iter=0, max_gmres=0, tol=0.0):
     lib = N.load()
     n, nrhs = B.shape
+    _dev_arg("gesv_ir_dev", "A", A, n, n)
+    _dev_arg("gesv_ir_dev", "B", B, n, nrhs)
+    _dev_arg("gesv_ir_dev", "X", X, n, nrhs)
     info = N.SolveInfo()
     rc = lib.scirs_gpu_dgesv_ir_dev(n, nrhs, _tp(A), A.stride(0), _tp(B), B.stride(0), _tp(X), X.stride(0),
                                     int(mode), int(max_iter), int(max_gmres), float(tol), _stream_ptr(),
@@ -302,11 +346,37 @@ def gesv_ir_dev(A, B, X, mode=N.MODE_AUTO, max_iter=0, max_gmres=0, tol=0.0):
 def backward_error_dev(A, X, B):
     lib = N.load()
     n, nrhs = B.shape
+    _dev_arg("backward_error_dev", "A", A, n, n)
+    _dev_arg("backward_error_dev", "X", X, n, nrhs)
+    _dev_arg("backward_error_dev", "B", B, n, nrhs)
     out = ctypes.c_double(0.0)
     rc = lib.scirs_gpu_backward_error_dev(n, nrhs, _tp(A), A.stride(0), _tp(X), X.stride(0), _tp(B), B.stride(0),
                                           _stream_ptr(), ctypes.byref(out))
     _check("backward_error_dev", rc, None)
     return out.value
+
+
+def sgetrf_tp_dev(A, perm, mode=N.MODE_FP32):
+    """Tournament-pivoted FP32 LU in place on the n x n float32 CUDA tensor A
+    (row stride a multiple of 4); perm (int64, n) receives the row permutation.
+    Returns the status (0, or k > 0 for |u_kk| < eps)."""
+    import torch
+    n = A.shape[0]
+    _dev_arg("sgetrf_tp_dev", "A", A, n, n, dtype=torch.float32)
+    _dev_arg("sgetrf_tp_dev", "perm", perm, n, dtype=torch.int64)
+    rc = N.load().scirs_gpu_sgetrf_tp_dev(n, _tp(A), A.stride(0), _tp(perm), int(mode), _stream_ptr())
+    if rc < 0:
+        _check("sgetrf_tp_dev", rc, None)
+    return rc
+
+
+def seq_plan(next_seq, span):
+    """The exchange-tag arithmetic: (reset, base, next)."""
+    base = ctypes.c_uint(0)
+    nxt = ctypes.c_uint(0)
+    reset = N.load().scirs_gpu_seq_plan(ctypes.c_uint(next_seq), ctypes.c_uint(span), ctypes.byref(base),
+                                        ctypes.byref(nxt))
+    return reset, base.value, nxt.value
 
 
 def prof_begin():
@@ -346,6 +416,14 @@ def gesv_batched_dev(n, nrhs, offA, offB, dA, dB, dX, spd=None, max_iter=0, tol_
     offA = np.ascontiguousarray(offA, dtype=np.int64)
     offB = np.ascontiguousarray(offB, dtype=np.int64)
     spd_arr = None if spd is None else np.ascontiguousarray(spd, dtype=np.uint8)
+    if not (len(nrhs) == len(offA) == len(offB) == batch) or (spd_arr is not None and len(spd_arr) != batch):
+        raise ValueError("gesv_batched_dev: n, nrhs, offA, offB (and spd) must have one entry per system")
+    need_a = int((offA + n.astype(np.int64) ** 2).max()) if batch else 0
+    need_b = int((offB + n.astype(np.int64) * nrhs).max()) if batch else 0
+    for name, t, need in (("dA", dA, need_a), ("dB", dB, need_b), ("dX", dX, need_b)):
+        _dev_arg("gesv_batched_dev", name, t.reshape(-1) if t.is_contiguous() else t, t.numel())
+        if t.numel() < need or not t.is_contiguous():
+            raise ValueError(f"gesv_batched_dev: {name} must be contiguous with at least {need} elements")
     status = np.zeros(batch, dtype=np.int32)
     bwd = np.zeros(batch, dtype=np.float64)
     info = N.BatchInfo()

@@ -141,6 +141,7 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
             IterativeSolverOptions<F> opt;
             if (f1 > 0) opt.max_iterations = (std::size_t)f1;
             if (f2 > 0) opt.restart = (std::size_t)f2;
+            else if (f2 < 0) opt.restart = (std::size_t)0;   // an explicit restart of 0
             auto r = gmres<F>(a.view(), view1(bvec), nullptr, opt);
             if (r.is_err()) return print_err(r.error());
             const auto &v = r.value();
@@ -175,8 +176,9 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
             put(r.value().d0 * r.value().d1, r.value().d2, r.value().data.data());
         } else {
             auto r = batch_sum<F>(a3);
+            if (r.is_err()) return print_err(r.error());
             std::printf("OK 1");
-            put(r.rows, r.cols, r.data.data());
+            put(r.value().rows, r.value().cols, r.value().data.data());
         }
     } else if (op == "qr" || op == "lstsq") {
         if constexpr (std::is_same<F, double>::value) {

@@ -48,3 +48,29 @@ def test_no_cpu_fallback_in_product_path():
             if f.endswith((".py", ".hip", ".hpp", ".cpp")):
                 text = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in text.lower() or f == "build.py", f
+
+
+def test_exchange_tag_wraparound():
+    """The 32-bit tag count of the exchange buffers: consecutive launches get
+    disjoint tag ranges, tag 0 (a cleared word) is never handed out, and a range
+    that would run past 2^32 - 1 restarts the count at 1 (the library clears every
+    tagged buffer first, so no word written before the restart can match)."""
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("library not built")
+    from scirs_amd import linalg
+    top = 0xFFFFFFFF
+    assert linalg.seq_plan(1, 512) == (0, 1, 513)
+    assert linalg.seq_plan(513, 1) == (0, 513, 514)
+    assert linalg.seq_plan(0, 512) == (1, 1, 513)           # never tag 0
+    assert linalg.seq_plan(top - 512, 512) == (0, top - 512, top)   # last full range
+    assert linalg.seq_plan(top - 511, 512) == (1, 1, 513)   # would wrap: restart
+    assert linalg.seq_plan(top, 1) == (1, 1, 2)
+    # a long run of launches never reuses a tag between two restarts
+    nxt, seen = top - 5000, set()
+    for span in [512, 1, 512, 7] * 40:
+        reset, base, nxt = linalg.seq_plan(nxt, span)
+        if reset:
+            seen.clear()
+        rng = set(range(base, base + span))
+        assert 0 not in rng and base + span - 1 <= top and not (rng & seen)
+        seen |= rng

@@ -546,6 +546,14 @@ def test_gmres_validation_texts():
     assert got[1] == ("ERR", "ShapeError", "Shape error: Vector b must have same length as matrix dimension")
 
 
+def test_gmres_restart_zero_is_an_error():
+    """restart = Some(0): the reference divides by restart.min(n) = 0 and panics
+    (solvers/iterative.rs:361-366); the mirror returns an error, before any
+    device work."""
+    got = run([("gmres", "f64", 0, -1, np.eye(3), np.ones(3))])
+    assert got[0] == ("ERR", "InvalidInputError", "Invalid input error: GMRES restart must be positive")
+
+
 @pytest.mark.gpu
 def test_gmres_results():
     """gmres through the mirror: the oracle's iteration count, flag and x."""
