@@ -47,18 +47,18 @@
 namespace tslu {
 
 typedef unsigned long long u64;
-constexpr int TW = 64;              // sub-panel width
+constexpr int TW = 32;              // sub-panel width
 constexpr int NT = 256;             // threads per workgroup
-constexpr int RPT = 2;              // rows per thread in an elimination
+constexpr int RPT = 4;              // rows per thread in an elimination
 constexpr int NODE = NT * RPT;      // rows per tree node
 constexpr int FAN = NODE / TW;      // children per tree node
 constexpr int NW = NT / 64;         // waves per workgroup
 constexpr int KBT = 128;            // widest panel
-constexpr int PRP = KBT + 1;        // LDS pitch of a panel row
 constexpr int AR = 32;              // rows per apply workgroup
 
 static_assert(NW == 4, "the key gather reads four wave keys as one uint4");
-static_assert(NODE <= 1024, "row indexes live in the key's low 10 bits");
+constexpr int IDXB = 10;            // bits of a node-local row index in a search key
+static_assert(NODE <= (1 << IDXB), "row indexes live in the key's low bits");
 
 __device__ __forceinline__ void st_agent(u64 *p, u64 v)
 {
@@ -73,7 +73,8 @@ __device__ __forceinline__ u64 ld_agent(const u64 *p)
 // 0 is "no candidate"
 __device__ __forceinline__ unsigned mkkey(float x, int r)
 {
-    return (__float_as_uint(x) & 0x7FFFFC00u) | 0x200u | (unsigned)(NODE - 1 - r);
+    constexpr unsigned low = (2u << IDXB) - 1u;   // the valid bit and the row index
+    return (__float_as_uint(x) & 0x7FFFFFFFu & ~low) | (1u << IDXB) | (unsigned)(NODE - 1 - r);
 }
 
 template <int CTRL, int RM>
@@ -101,34 +102,49 @@ struct Rows {
     int step[RPT];   // -1 not yet picked, >= 0 the step that picked it, -2 no row
 };
 
-// v <- columns c0 .. c0 + w - 1 of the given row of A (zero beyond w); 16-byte loads
-__device__ __forceinline__ void load_row(float (&v)[TW], const float *src, int w)
+// v <- columns c0 .. c0 + w - 1 of the given row of A (zero beyond w)
+__device__ __forceinline__ void load_row_masked(float (&v)[TW], const float *src, int w)
 {
 #pragma unroll
-    for (int c4 = 0; c4 < TW / 4; ++c4) {
-        if (4 * c4 + 3 < w) {
-            const float4 q = *reinterpret_cast<const float4 *>(src + 4 * c4);
-            v[4 * c4] = q.x; v[4 * c4 + 1] = q.y; v[4 * c4 + 2] = q.z; v[4 * c4 + 3] = q.w;
-        } else {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[4 * c4 + u] = (4 * c4 + u < w) ? src[4 * c4 + u] : 0.0f;
-        }
-    }
+    for (int c = 0; c < TW; ++c) v[c] = (c < w) ? src[c] : 0.0f;
 }
 
+// Both rows of the thread: a full-width sub-panel issues all 32 16-byte loads
+// before using any (one memory latency, not 32)
 __device__ __forceinline__ void rows_load(Rows &R, const float *A, long lda, int c0, int w, const int *rid, int nr)
 {
     const int t = threadIdx.x;
+    bool have[RPT];
+    const float *src[RPT];
 #pragma unroll
     for (int x = 0; x < RPT; ++x) {
         const int i = t + x * NT;
-        if (i < nr) {
-            load_row(R.v[x], A + (long)rid[i] * lda + c0, w);
-            R.step[x] = -1;
-        } else {
+        have[x] = i < nr;
+        src[x] = A + (long)(have[x] ? rid[i] : rid[0]) * lda + c0;
+        R.step[x] = have[x] ? -1 : -2;
+    }
+    if (w == TW) {
+        float4 q[RPT][TW / 4];
 #pragma unroll
-            for (int c = 0; c < TW; ++c) R.v[x][c] = 0.0f;
-            R.step[x] = -2;
+        for (int x = 0; x < RPT; ++x)
+#pragma unroll
+            for (int c4 = 0; c4 < TW / 4; ++c4) q[x][c4] = reinterpret_cast<const float4 *>(src[x])[c4];
+#pragma unroll
+        for (int x = 0; x < RPT; ++x)
+#pragma unroll
+            for (int c4 = 0; c4 < TW / 4; ++c4) {
+                R.v[x][4 * c4] = have[x] ? q[x][c4].x : 0.0f;
+                R.v[x][4 * c4 + 1] = have[x] ? q[x][c4].y : 0.0f;
+                R.v[x][4 * c4 + 2] = have[x] ? q[x][c4].z : 0.0f;
+                R.v[x][4 * c4 + 3] = have[x] ? q[x][c4].w : 0.0f;
+            }
+    } else {
+#pragma unroll
+        for (int x = 0; x < RPT; ++x) {
+            load_row_masked(R.v[x], src[x], w);
+            if (!have[x])
+#pragma unroll
+                for (int c = 0; c < TW; ++c) R.v[x][c] = 0.0f;
         }
     }
 }
@@ -147,10 +163,12 @@ struct Xch {
 // (column j is a register, not an index) and never branched around: a step
 // with nothing to pick (past w, or no unpicked row left) runs with zero keys
 // and a zero multiplier.  Picked rows stop changing.  Returns the number of
-// picks; X.sel[j] is the local row picked at step j; with U != nullptr the
-// pivot rows' U (their values at their pick) is kept in U[j][j..TW-1].
+// picks; X.sel[j] is the local row picked at step j.  A picked row holds its
+// multipliers (L) in columns < j and its U in columns >= j; the winner
+// publishes the whole row, and with U != nullptr it is kept in U[j][0..TW-1]:
+// L11 \ U11 of the picked rows in pick order.
 template <int J>
-__device__ __forceinline__ void gepp_step(Rows &R, int w, Xch &X, float (*U)[TW + 1], int &nsel)
+__device__ __forceinline__ void gepp_step(Rows &R, int w, Xch &X, float (*U)[TW + 4], int &nsel)
 {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     constexpr int par = J & 1, J4 = J & ~3;
@@ -168,6 +186,12 @@ __device__ __forceinline__ void gepp_step(Rows &R, int w, Xch &X, float (*U)[TW 
 #pragma unroll
     for (int x = 0; x < RPT; ++x) {
         if (wm != 0u && k[x] == wm) {
+            // the root keeps the whole row (its L too); elsewhere only columns >= J are read
+            if (U != nullptr) {
+#pragma unroll
+                for (int c4 = 0; c4 < J4 / 4; ++c4)
+                    slot[c4] = make_float4(R.v[x][4 * c4], R.v[x][4 * c4 + 1], R.v[x][4 * c4 + 2], R.v[x][4 * c4 + 3]);
+            }
 #pragma unroll
             for (int c4 = J4 / 4; c4 < TW / 4; ++c4)
                 slot[c4] = make_float4(R.v[x][4 * c4], R.v[x][4 * c4 + 1], R.v[x][4 * c4 + 2], R.v[x][4 * c4 + 3]);
@@ -187,7 +211,7 @@ __device__ __forceinline__ void gepp_step(Rows &R, int w, Xch &X, float (*U)[TW 
         const float4 q = ps[c4];
         p[4 * c4] = q.x; p[4 * c4 + 1] = q.y; p[4 * c4 + 2] = q.z; p[4 * c4 + 3] = q.w;
     }
-    if (pick && U != nullptr && t >= J && t < TW) U[J][t] = reinterpret_cast<const float *>(ps)[t];
+    if (pick && U != nullptr && t < TW) U[J][t] = reinterpret_cast<const float *>(ps)[t];
     if (pick && t == 0) X.sel[J] = rr;
 #pragma unroll
     for (int x = 0; x < RPT; ++x) R.step[x] = (t + x * NT == rr) ? J : R.step[x];
@@ -203,7 +227,7 @@ __device__ __forceinline__ void gepp_step(Rows &R, int w, Xch &X, float (*U)[TW 
 }
 
 template <int J>
-__device__ __forceinline__ void gepp_steps(Rows &R, int w, Xch &X, float (*U)[TW + 1], int &nsel)
+__device__ __forceinline__ void gepp_steps(Rows &R, int w, Xch &X, float (*U)[TW + 4], int &nsel)
 {
     if constexpr (J < TW) {
         gepp_step<J>(R, w, X, U, nsel);
@@ -211,60 +235,41 @@ __device__ __forceinline__ void gepp_steps(Rows &R, int w, Xch &X, float (*U)[TW
     }
 }
 
-__device__ __forceinline__ int rows_gepp(Rows &R, int w, Xch &X, float (*U)[TW + 1])
+__device__ __forceinline__ int rows_gepp(Rows &R, int w, Xch &X, float (*U)[TW + 4])
 {
     int nsel = w;
     gepp_steps<0>(R, w, X, U, nsel);
     return nsel;
 }
 
-// X = inverse of the upper-triangular TW x TW matrix U (entries (i, j) of U are
-// M[i * ldm + j], or M[j * ldm + i] when TRANS; UNIT: unit diagonal).  16 x 16
-// diagonal blocks by substitution (one wave each, lane = column), then the
-// blocks above the diagonal by increasing distance:
-// X_bc = -X_bb sum_{b<k<=c} U_bk X_kc.  X is TW x TW with pitch TW.
+// entry (i, j) of the upper-triangular matrix held in M (pitch ldm), or of the
+// transpose of the lower-triangular one (TRANS)
 template <bool TRANS, bool UNIT>
-__device__ void tri_inv_upper(const float *M, int ldm, float *X, float *T)
+__device__ __forceinline__ float tri_u(const float *M, int ldm, int i, int j)
 {
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    auto U = [&](int i, int j) { return TRANS ? M[j * ldm + i] : M[i * ldm + j]; };
-    for (int q = t; q < TW * TW; q += NT) X[q] = 0.0f;
-    __syncthreads();
-    {
-        const int b0 = wave * 16;
-        if (lane < 16) {
-            float x[16];
+    return TRANS ? M[j * ldm + i] : M[i * ldm + j];
+}
+
+// One wave: X = U^-1, lane c computing column c by right-looking substitution
+// (once x_i is final its multiples of column i of U leave the rows above it:
+// independent updates, and a reciprocal instead of a division for the
+// diagonal — these factors only precondition refinement).
+template <bool TRANS, bool UNIT>
+__device__ __forceinline__ void tri_inv_wave(const float *M, int ldm, float *X, int ldx)
+{
+    const int c = threadIdx.x & 63;
+    if (c >= TW) return;
+    float x[TW];
 #pragma unroll
-            for (int i = 15; i >= 0; --i) {
-                float s = (i == lane) ? 1.0f : 0.0f;
+    for (int i = 0; i < TW; ++i) x[i] = (i == c) ? 1.0f : 0.0f;
 #pragma unroll
-                for (int k = i + 1; k < 16; ++k) s -= U(b0 + i, b0 + k) * x[k];
-                x[i] = UNIT ? s : s / U(b0 + i, b0 + i);
-            }
+    for (int i = TW - 1; i >= 0; --i) {
+        if (!UNIT) x[i] *= __builtin_amdgcn_rcpf(tri_u<TRANS, UNIT>(M, ldm, i, i));
 #pragma unroll
-            for (int i = 0; i < 16; ++i) X[(b0 + i) * TW + b0 + lane] = x[i];
-        }
+        for (int k = 0; k < i; ++k) x[k] = fmaf(-tri_u<TRANS, UNIT>(M, ldm, k, i), x[i], x[k]);
     }
-    __syncthreads();
-    for (int d = 1; d < 4; ++d) {
-        const int nblk = 4 - d;
-        // T_b = sum_{b<k<=c} U_bk X_kc for the blocks (b, c = b + d)
-        for (int q = t; q < nblk * 256; q += NT) {
-            const int b = q >> 8, e = q & 255, i = e >> 4, jj = e & 15, c = b + d;
-            float s = 0.0f;
-            for (int k = (b + 1) * 16; k < (c + 1) * 16; ++k) s += U(b * 16 + i, k) * X[k * TW + c * 16 + jj];
-            T[q] = s;
-        }
-        __syncthreads();
-        for (int q = t; q < nblk * 256; q += NT) {
-            const int b = q >> 8, e = q & 255, i = e >> 4, jj = e & 15, c = b + d;
-            float s = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) s += X[(b * 16 + i) * TW + b * 16 + k] * T[(b << 8) + k * 16 + jj];
-            X[(b * 16 + i) * TW + c * 16 + jj] = -s;
-        }
-        __syncthreads();
-    }
+    for (int i = 0; i < TW; ++i) X[i * ldx + c] = x[i];
 }
 
 struct TourArgs {
@@ -283,55 +288,56 @@ struct TourArgs {
     unsigned spin_limit;
 };
 
+constexpr int LP = TW + 4;     // LDS pitch of the sub-panel-wide blocks (16-byte rows)
+constexpr int PP = KBT + 4;    // LDS pitch of the pivot rows' panel rows
+
 // LDS of the root's work (dynamic; the tree's workgroups carry it too)
 struct RootLds {
-    float pr[TW][PRP];      // pivot rows' panel rows; their current columns replaced by L11 \ U11
-    float u11[TW][TW + 1];  // the pivot rows' U (upper triangle) from the root's elimination
-    float l11[TW][TW + 1];  // unit lower L11 = A11 U11^-1
-    float uinv[TW * TW];
-    float linvT[TW * TW];   // (L11^-1)^T: the inverse of L11^T
-    float tmp[3 * 256];
+    float lu[TW][LP];       // L11 \ U11 of the pivot rows in pick order, from the root's elimination
+    float uinv[TW][LP];     // U11^-1
+    float linvT[TW][LP];    // (L11^-1)^T
+    float pr[TW][PP];       // the pivot rows' panel rows (earlier sub-panels' L, then U12 on the later columns)
     int gp[TW];
+    int mark[TW];
     int dlist[TW], vlist[TW];
     int mpos[TSLU_MAP], mlab[TSLU_MAP];
     int mv_dst[2 * TW], mv_src[2 * TW], mv_lab[2 * TW];
-    int nd, nmv, mcount, mcount0, bad, npairs;
+    int nd, nmv, mcount, bad, npairs;
 };
 
 __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int *rid, Xch &X, RootLds &L)
 {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int w = a.w, r0 = a.c0, cur0 = a.c0 - a.k0, rest0 = cur0 + w, nrest = a.kb - rest0;
-    // 1. pivot positions in pick order; U11 is in L.u11 (upper triangle)
+    const int w = a.w, r0 = a.c0, cur0 = a.c0 - a.k0, rest0 = cur0 + w, nrest = a.kb - rest0, kb = a.kb;
+    // 1. pivot positions in pick order; the breakdown test |u_jj| < eps (the
+    //    absolute test of lapack.rs:106-120: the mixed solve then falls back to
+    //    FP64 LU, which decides singularity)
     if (t < nsel) L.gp[t] = rid[X.sel[t]];
     if (t == 0) L.bad = nsel < w ? nsel : TW;
     __syncthreads();
-    // 2. breakdown: |u_jj| < eps (the absolute test of lapack.rs:106-120); the
-    //    mixed solve then falls back to FP64 LU, which decides singularity
-    if (t < nsel && t < w && !(fabsf(L.u11[t][t]) >= FLT_EPSILON)) atomicMin(&L.bad, t);
+    if (t < nsel && t < w && !(fabsf(L.lu[t][t]) >= FLT_EPSILON)) atomicMin(&L.bad, t);
     __syncthreads();
     if (L.bad < w) {
         if (t == 0) {
             *a.info = a.c0 + L.bad + 1;
-            *a.info_val = L.bad < nsel ? (double)fabsf(L.u11[L.bad][L.bad]) : 0.0;
+            *a.info_val = L.bad < nsel ? (double)fabsf(L.lu[L.bad][L.bad]) : 0.0;
         }
         return;
     }
-    // U11 padded with the identity beyond w, zero below the diagonal
-    for (int q = t; q < TW * TW; q += NT) {
-        const int i = q / TW, c = q % TW;
-        if (c < i || i >= w || c >= w) L.u11[i][c] = (i == c) ? 1.0f : 0.0f;
-    }
-    // 3. positions: top r0 .. r0 + w - 1 receive the pivots in pick order; the
+    if (w < TW)   // a ragged last sub-panel: the identity beyond w
+        for (int q = t; q < TW * TW; q += NT) {
+            const int i = q / TW, c = q % TW;
+            if (i >= w || c >= w) L.lu[i][c] = (i == c) ? 1.0f : 0.0f;
+        }
+    // 2. positions: top r0 .. r0 + w - 1 receive the pivots in pick order; the
     //    top rows that are not pivots (dlist) move to the positions of the
     //    pivots from below (vlist)
+    if (t < TW) L.mark[t] = 0;
+    __syncthreads();
+    if (t < w && L.gp[t] < r0 + w) L.mark[L.gp[t] - r0] = 1;
+    __syncthreads();
     if (wave == 0) {
-        const bool top_is_pivot = [&] {
-            if (lane >= w) return true;
-            for (int j = 0; j < w; ++j)
-                if (L.gp[j] == r0 + lane) return true;
-            return false;
-        }();
+        const bool top_is_pivot = lane >= w || L.mark[lane] != 0;
         const unsigned long long bd = __ballot(lane < w && !top_is_pivot);
         if (lane < w && !top_is_pivot) L.dlist[__popcll(bd & ((1ull << lane) - 1ull))] = r0 + lane;
         const bool below = lane < w && L.gp[lane] >= r0 + w;
@@ -341,90 +347,98 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
     }
     __syncthreads();
     const int nd = L.nd;
-    // 4. the pivot rows' other columns (earlier sub-panels' L, later columns' A12),
-    //    and the displaced rows staged whole
-    for (int q = t; q < w * a.kb; q += NT) {
-        const int j = q / a.kb, c = q - j * a.kb;
-        L.pr[j][c] = a.A[(long)L.gp[j] * a.lda + a.k0 + c];
-    }
-    for (int q = t; q < nd * a.kb; q += NT) {
-        const int i = q / a.kb, c = q - i * a.kb;
-        a.stage[i * SCIRS_KB_MAX + c] = a.A[(long)L.dlist[i] * a.lda + a.k0 + c];
+    // 3. the pivot rows' panel rows into LDS and the displaced rows staged whole:
+    //    16-byte loads, all issued before any is stored
+    {
+        constexpr int JOBS = 2 * TW * (KBT / 4) / NT;   // (row, 16-byte chunk) pairs per thread
+        float4 q[JOBS];
+#pragma unroll
+        for (int u = 0; u < JOBS; ++u) {
+            const int job = t + u * NT, r = job / (KBT / 4), c4 = job % (KBT / 4);
+            const bool piv = r < TW;
+            const int ri = piv ? r : r - TW;
+            const bool ok = (piv ? ri < w : ri < nd) && 4 * c4 < kb;
+            const int row = ok ? (piv ? L.gp[ri] : L.dlist[ri]) : r0;
+            const float *src = a.A + (long)row * a.lda + a.k0 + 4 * c4;
+            q[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (ok) {
+                if (4 * c4 + 3 < kb) {
+                    q[u] = *reinterpret_cast<const float4 *>(src);
+                } else {
+                    q[u].x = src[0];
+                    if (4 * c4 + 1 < kb) q[u].y = src[1];
+                    if (4 * c4 + 2 < kb) q[u].z = src[2];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < JOBS; ++u) {
+            const int job = t + u * NT, r = job / (KBT / 4), c4 = job % (KBT / 4);
+            const bool piv = r < TW;
+            const int ri = piv ? r : r - TW;
+            if ((piv ? ri < w : ri < nd) && 4 * c4 < kb) {
+                if (piv) *reinterpret_cast<float4 *>(&L.pr[ri][4 * c4]) = q[u];
+                else *reinterpret_cast<float4 *>(a.stage + ri * SCIRS_KB_MAX + 4 * c4) = q[u];
+            }
+        }
     }
     if (t < nd) a.vlist[t] = L.vlist[t];
     if (t == 0) a.vlist[TW] = nd;
     __syncthreads();
-    // 5. U11^-1; L11 = A11 U11^-1 (strictly lower part, from the pivot rows'
-    //    values; the elimination kept only their U); L11^-1
-    tri_inv_upper<false, false>(&L.u11[0][0], TW + 1, L.uinv, L.tmp);
-    {
-        const int ti = t >> 4, tj = t & 15;
-        float acc[4][4] = {};
-        for (int k = 0; k < w; ++k) {
-            float av[4], uv[4];
+    // 4. U11^-1 (wave 0) and (L11^-1)^T = the inverse of L11^T, upper with a
+    //    unit diagonal (wave 1), side by side
+    if (wave == 0) tri_inv_wave<false, false>(&L.lu[0][0], LP, &L.uinv[0][0], LP);
+    else if (wave == 1) tri_inv_wave<true, true>(&L.lu[0][0], LP, &L.linvT[0][0], LP);
+    __syncthreads();
+    // 5. U12 = L11^-1 A12 on the later columns (4 x 4 register tiles)
+    constexpr int TJ = KBT / 4;
+    static_assert((TW / 4) * TJ <= NT, "one U12 tile per thread");
+    const int ti = t / TJ, tj = t % TJ;   // rows 4 ti .., columns 4 tj ..
+    const bool have12 = ti < TW / 4 && 4 * tj < nrest && 4 * ti < w;
+    float u12[4][4] = {};
+    if (have12) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) av[u] = (4 * ti + u < w) ? L.pr[4 * ti + u][cur0 + k] : 0.0f;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) uv[v] = L.uinv[k * TW + 4 * tj + v];
+        for (int k = 0; k < TW; ++k) {
+            const float4 lv = *reinterpret_cast<const float4 *>(&L.linvT[k][4 * ti]);
+            const float4 av = *reinterpret_cast<const float4 *>(&L.pr[k][rest0 + 4 * tj]);
+            const float l4[4] = {lv.x, lv.y, lv.z, lv.w}, a4[4] = {av.x, av.y, av.z, av.w};
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(av[u], uv[v], acc[u][v]);
+                for (int v = 0; v < 4; ++v) u12[u][v] = fmaf(l4[u], a4[v], u12[u][v]);
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = 4 * ti + u, c = 4 * tj + v;
-                L.l11[i][c] = (i == c || i >= w) ? (i == c ? 1.0f : 0.0f) : (c < i ? acc[u][v] : 0.0f);
-            }
     }
     __syncthreads();
-    tri_inv_upper<true, true>(&L.l11[0][0], TW + 1, L.linvT, L.tmp);
+    if (have12) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = 4 * ti + u;
+            if (i >= w) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 4 * tj + v;
+                if (c < nrest) {
+                    L.pr[i][rest0 + c] = u12[u][v];
+                    a.urest[i * KBT + c] = u12[u][v];
+                }
+            }
+        }
+    }
     // the pivot rows' current columns: L11 below the diagonal, U11 on and above
     for (int q = t; q < w * w; q += NT) {
         const int i = q / w, c = q - i * w;
-        L.pr[i][cur0 + c] = c < i ? L.l11[i][c] : L.u11[i][c];
+        L.pr[i][cur0 + c] = L.lu[i][c];
     }
-    // 6. U12 = L11^-1 A12 on the later columns (4 x 4 register tiles)
-    float u12[4][4];
-    const int ti = t >> 4, tj = t & 15;   // rows 4 ti .., columns 4 tj ..
-    const bool have12 = 4 * tj < nrest && 4 * ti < w;
-    if (have12) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) u12[u][v] = 0.0f;
-        for (int k = 0; k < w; ++k) {
-            float lv[4], av[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) lv[u] = L.linvT[k * TW + 4 * ti + u];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) av[v] = (4 * tj + v < nrest) ? L.pr[k][rest0 + 4 * tj + v] : 0.0f;
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) u12[u][v] = fmaf(lv[u], av[v], u12[u][v]);
-        }
+    for (int q = t; q < TW * TW / 4; q += NT) {
+        const int i = q / (TW / 4), c4 = q % (TW / 4);
+        *reinterpret_cast<float4 *>(a.uinv + i * TW + 4 * c4) = *reinterpret_cast<const float4 *>(&L.uinv[i][4 * c4]);
     }
     __syncthreads();
-    if (have12) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                if (4 * ti + u < w && 4 * tj + v < nrest) {
-                    L.pr[4 * ti + u][rest0 + 4 * tj + v] = u12[u][v];
-                    a.urest[(4 * ti + u) * TW + 4 * tj + v] = u12[u][v];
-                }
-    }
-    for (int q = t; q < TW * TW; q += NT) a.uinv[q] = L.uinv[q];
-    __syncthreads();
-    for (int q = t; q < w * a.kb; q += NT) {
-        const int j = q / a.kb, c = q - j * a.kb;
+    for (int q = t; q < w * kb; q += NT) {
+        const int j = q / kb, c = q - j * kb;
         a.top[j * SCIRS_KB_MAX + c] = L.pr[j][c];
     }
-    // 7. the panel's interchanges: (position, original row) for every moved row
+    // 6. the panel's interchanges: (position, original row) for every moved row
     if (t == 0) {
         L.mcount = a.first ? 0 : a.map[2 * TSLU_MAP];
         L.nmv = 0;
@@ -446,22 +460,29 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
     }
     __syncthreads();
     const int nmv = L.nmv;
-    if (t < nmv) {   // original row of each source
-        const int src = L.mv_src[t];
-        int lab = src;
-        for (int e = 0; e < mc0; ++e)
-            if (L.mpos[e] == src) lab = L.mlab[e];
-        L.mv_lab[t] = lab;
+    // each wave looks its moves up 64 map entries at a time: the original row
+    // of every source, and the entry (if any) of every destination
+    for (int m = wave; m < nmv; m += NW) {
+        const int src = L.mv_src[m], dst = L.mv_dst[m];
+        int fs = -1, fd = -1;
+        for (int base = 0; base < mc0; base += 64) {
+            const int e = base + lane;
+            const int pos = e < mc0 ? L.mpos[e] : -1;
+            const unsigned long long bs = __ballot(pos == src), bd2 = __ballot(pos == dst);
+            if (fs < 0 && bs) fs = base + __ffsll(bs) - 1;
+            if (fd < 0 && bd2) fd = base + __ffsll(bd2) - 1;
+        }
+        if (lane == 0) {
+            L.mv_lab[m] = fs >= 0 ? L.mlab[fs] : src;
+            L.mv_src[m] = fd;   // reused: the destination's entry, -1 when new
+        }
     }
     __syncthreads();
     if (t < nmv) {
-        const int dst = L.mv_dst[t];
-        int at = -1;
-        for (int e = 0; e < mc0; ++e)
-            if (L.mpos[e] == dst) at = e;
+        int at = L.mv_src[t];
         if (at < 0) {
             at = atomicAdd(&L.mcount, 1);
-            L.mpos[at] = dst;
+            L.mpos[at] = L.mv_dst[t];
         }
         L.mlab[at] = L.mv_lab[t];
     }
@@ -509,10 +530,10 @@ __global__ __launch_bounds__(NT) void tour_kernel(TourArgs a)
             __syncthreads();
             const bool root = lev == a.nlev;
             if (root)
-                for (int q = t; q < TW * (TW + 1); q += NT) (&L.u11[0][0])[q] = 0.0f;
+                for (int q = t; q < TW * LP; q += NT) (&L.lu[0][0])[q] = 0.0f;
             Rows R;
             rows_load(R, a.A, a.lda, a.c0, a.w, rid, nr);
-            nsel = rows_gepp(R, a.w, X, root ? L.u11 : nullptr);
+            nsel = rows_gepp(R, a.w, X, root ? L.lu : nullptr);
         }
         if (lev == a.nlev) break;
         __syncthreads();
@@ -608,22 +629,37 @@ struct ApplyArgs {
 // from the staging buffer at the position a pivot left) gets
 //   L21 = A21 U11^-1   (the sub-panel's columns)
 //   A22 -= L21 U12     (the panel's later columns).
+// Thread (row rho, group g) computes 8 consecutive columns of each product;
+// rows move as 16-byte chunks, all loads issued before any is stored.
 __global__ __launch_bounds__(NT) void apply_kernel(ApplyArgs a)
 {
     if (*a.info != 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float *Ui = reinterpret_cast<float *>(smem_raw);   // TW x TW
-    float *Ur = Ui + TW * TW;                          // TW x TW (pitch TW)
-    float *Ar = Ur + TW * TW;                          // AR x PRP
-    float *Lr = Ar + AR * PRP;                         // AR x (TW + 1)
+    float(*Ui)[LP] = reinterpret_cast<float(*)[LP]>(smem_raw);   // TW x LP
+    float(*Ur)[PP] = reinterpret_cast<float(*)[PP]>(Ui + TW);    // TW x PP (the later columns)
+    float(*Ar)[PP] = Ur + TW;                                    // AR x PP
+    float(*Lr)[LP] = reinterpret_cast<float(*)[LP]>(Ar + AR);    // AR x LP
     __shared__ int sidx[AR];
     const int t = threadIdx.x;
-    const int w = a.w, cur0 = a.c0 - a.k0, rest0 = cur0 + w, nrest = a.kb - rest0;
+    const int w = a.w, cur0 = a.c0 - a.k0, rest0 = cur0 + w, nrest = a.kb - rest0, kb = a.kb;
     const int d0 = a.c0 + blockIdx.x * AR, top_end = a.c0 + w;
-    for (int q = t; q < TW * TW; q += NT) {
-        const int r = q / TW, c = q % TW;
-        Ui[q] = (r < w && c < w) ? a.uinv[q] : 0.0f;
-        Ur[q] = (r < w && c < nrest) ? a.urest[q] : 0.0f;
+    for (int q = t; q < TW * TW / 4; q += NT) {
+        const int r = q / (TW / 4), c4 = q % (TW / 4);
+        float4 u = *reinterpret_cast<const float4 *>(a.uinv + r * TW + 4 * c4);
+        float *uu = &u.x;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (r >= w || 4 * c4 + e >= w) uu[e] = 0.0f;
+        *reinterpret_cast<float4 *>(&Ui[r][4 * c4]) = u;
+    }
+    for (int q = t; q < TW * KBT / 4; q += NT) {
+        const int r = q / (KBT / 4), c4 = q % (KBT / 4);
+        float4 v = *reinterpret_cast<const float4 *>(a.urest + r * KBT + 4 * c4);
+        float *vv = &v.x;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (r >= w || 4 * c4 + e >= nrest) vv[e] = 0.0f;
+        *reinterpret_cast<float4 *>(&Ur[r][4 * c4]) = v;
     }
     if (t < AR) sidx[t] = -1;
     __syncthreads();
@@ -633,40 +669,100 @@ __global__ __launch_bounds__(NT) void apply_kernel(ApplyArgs a)
         if (pos >= d0 && pos < d0 + AR) sidx[pos - d0] = i;
     }
     __syncthreads();
-    for (int q = t; q < AR * a.kb; q += NT) {
-        const int rho = q / a.kb, c = q - rho * a.kb, d = d0 + rho;
-        float v = 0.0f;
-        if (d < a.n) {
-            if (d < top_end) v = a.top[(d - a.c0) * SCIRS_KB_MAX + c];
-            else if (sidx[rho] >= 0) v = a.stage[sidx[rho] * SCIRS_KB_MAX + c];
-            else if (c >= cur0) v = a.A[(long)d * a.lda + a.k0 + c];
+    // rows in: the top rows' final contents, staged rows, or the row in place
+    // (from the sub-panel's first column on)
+    constexpr int JOBS = AR * (KBT / 4) / NT;
+    float4 q[JOBS];
+#pragma unroll
+    for (int u = 0; u < JOBS; ++u) {
+        const int job = t + u * NT, rho = job / (KBT / 4), c4 = job % (KBT / 4), d = d0 + rho;
+        q[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (d < a.n && 4 * c4 < kb) {
+            const float *src = nullptr;
+            if (d < top_end) src = a.top + (d - a.c0) * SCIRS_KB_MAX;
+            else if (sidx[rho] >= 0) src = a.stage + sidx[rho] * SCIRS_KB_MAX;
+            else if (4 * c4 >= cur0) src = a.A + (long)d * a.lda + a.k0;
+            if (src) {
+                src += 4 * c4;
+                if (4 * c4 + 3 < kb) {
+                    q[u] = *reinterpret_cast<const float4 *>(src);
+                } else {
+                    q[u].x = src[0];
+                    if (4 * c4 + 1 < kb) q[u].y = src[1];
+                    if (4 * c4 + 2 < kb) q[u].z = src[2];
+                }
+            }
         }
-        Ar[rho * PRP + c] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < JOBS; ++u) {
+        const int job = t + u * NT, rho = job / (KBT / 4), c4 = job % (KBT / 4);
+        *reinterpret_cast<float4 *>(&Ar[rho][4 * c4]) = q[u];
     }
     __syncthreads();
+    // thread (row rho, group g): CL columns of L21 and CR of the later columns
+    constexpr int CL = TW / 8, CR = (KBT - TW) / 8;
+    static_assert(CL % 4 == 0 && CR % 4 == 0, "16-byte LDS reads");
     const int rho = t >> 3, g = t & 7, d = d0 + rho;
     const bool work = d >= top_end && d < a.n;
     if (work) {
-        for (int c = g; c < w; c += 8) {
-            float s = 0.0f;
-            for (int k = 0; k <= c; ++k) s = fmaf(Ar[rho * PRP + cur0 + k], Ui[k * TW + c], s);
-            Lr[rho * (TW + 1) + c] = s;
+        float l[CL] = {};
+#pragma unroll
+        for (int k = 0; k < TW; ++k) {
+            const float av = Ar[rho][cur0 + k];
+#pragma unroll
+            for (int e4 = 0; e4 < CL / 4; ++e4) {
+                const float4 u = *reinterpret_cast<const float4 *>(&Ui[k][CL * g + 4 * e4]);
+                l[4 * e4] = fmaf(av, u.x, l[4 * e4]);
+                l[4 * e4 + 1] = fmaf(av, u.y, l[4 * e4 + 1]);
+                l[4 * e4 + 2] = fmaf(av, u.z, l[4 * e4 + 2]);
+                l[4 * e4 + 3] = fmaf(av, u.w, l[4 * e4 + 3]);
+            }
         }
+#pragma unroll
+        for (int e4 = 0; e4 < CL / 4; ++e4)
+            *reinterpret_cast<float4 *>(&Lr[rho][CL * g + 4 * e4]) =
+                make_float4(l[4 * e4], l[4 * e4 + 1], l[4 * e4 + 2], l[4 * e4 + 3]);
     }
     __syncthreads();
     if (work) {
-        for (int q = g; q < nrest; q += 8) {
-            float s = Ar[rho * PRP + rest0 + q];
-            for (int k = 0; k < w; ++k) s = fmaf(-Lr[rho * (TW + 1) + k], Ur[k * TW + q], s);
-            Ar[rho * PRP + rest0 + q] = s;
+        if (CR * g < nrest) {
+            float r[CR];
+#pragma unroll
+            for (int e = 0; e < CR; ++e) r[e] = Ar[rho][rest0 + CR * g + e];
+#pragma unroll
+            for (int k = 0; k < TW; ++k) {
+                const float lv = Lr[rho][k];
+#pragma unroll
+                for (int e4 = 0; e4 < CR / 4; ++e4) {
+                    const float4 u = *reinterpret_cast<const float4 *>(&Ur[k][CR * g + 4 * e4]);
+                    r[4 * e4] = fmaf(-lv, u.x, r[4 * e4]);
+                    r[4 * e4 + 1] = fmaf(-lv, u.y, r[4 * e4 + 1]);
+                    r[4 * e4 + 2] = fmaf(-lv, u.z, r[4 * e4 + 2]);
+                    r[4 * e4 + 3] = fmaf(-lv, u.w, r[4 * e4 + 3]);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < CR; ++e) Ar[rho][rest0 + CR * g + e] = r[e];
         }
-        for (int c = g; c < w; c += 8) Ar[rho * PRP + cur0 + c] = Lr[rho * (TW + 1) + c];
+#pragma unroll
+        for (int e = 0; e < CL; ++e) Ar[rho][cur0 + CL * g + e] = Lr[rho][CL * g + e];
     }
     __syncthreads();
-    for (int q = t; q < AR * a.kb; q += NT) {
-        const int r = q / a.kb, c = q - r * a.kb, dd = d0 + r;
-        if (dd < a.n && (c >= cur0 || dd < top_end || sidx[r] >= 0))
-            a.A[(long)dd * a.lda + a.k0 + c] = Ar[r * PRP + c];
+#pragma unroll
+    for (int u = 0; u < JOBS; ++u) {
+        const int job = t + u * NT, rho2 = job / (KBT / 4), c4 = job % (KBT / 4), dd = d0 + rho2;
+        if (dd >= a.n || 4 * c4 >= kb) continue;
+        if (!(4 * c4 >= cur0 || dd < top_end || sidx[rho2] >= 0)) continue;
+        float *dst = a.A + (long)dd * a.lda + a.k0 + 4 * c4;
+        const float4 v = *reinterpret_cast<const float4 *>(&Ar[rho2][4 * c4]);
+        if (4 * c4 + 3 < kb) {
+            *reinterpret_cast<float4 *>(dst) = v;
+        } else {
+            dst[0] = v.x;
+            if (4 * c4 + 1 < kb) dst[1] = v.y;
+            if (4 * c4 + 2 < kb) dst[2] = v.z;
+        }
     }
 }
 
@@ -681,7 +777,7 @@ int scirs_tslu_panel(float *A, long lda, int n, int k0, int kb, int panel_index,
     }
     const size_t lds_root = sizeof(RootLds);
     if (int rc = scirs_allow_lds((const void *)tour_kernel, lds_root)) return rc;
-    const size_t lds_apply = ((size_t)2 * TW * TW + (size_t)AR * PRP + (size_t)AR * (TW + 1)) * sizeof(float);
+    const size_t lds_apply = ((size_t)TW * LP + (size_t)TW * PP + (size_t)AR * PP + (size_t)AR * LP) * sizeof(float);
     if (int rc = scirs_allow_lds((const void *)apply_kernel, lds_apply)) return rc;
     const int nsub = (kb + TW - 1) / TW;
     for (int sp = 0; sp < nsub; ++sp) {

@@ -12,9 +12,12 @@ satisfy, computed on the host in FP64:
 * an exactly zero column k stops the factorisation with status k + 1, the
   absolute test |u_kk| < eps of lapack.rs:106-120.
 
-Sizes cover a single workgroup (n <= 512 rows per sub-panel), a two-level tree,
-a parent with a sole child (9 leaves: 8 + 1), ragged sub-panels (n = 333 leaves
-a 13-column sub-panel) and row strides that are not multiples of 64.
+Sizes cover a single workgroup (n <= 1024 rows per sub-panel), a two-level
+tree, ragged sub-panels (n = 333 leaves a 13-column sub-panel) and row strides
+that are not multiples of 64.  At n = 33000 the first panels have 33 blocks of
+1024 rows (a three-level tree whose last block is its parent's sole child);
+there P A = L U is checked on random vectors (O(n^2) work, in FP64 with torch
+on the GPU).
 """
 import numpy as np
 import pytest
@@ -81,3 +84,26 @@ def test_tournament_lu_rejects_unaligned_stride(L):
     perm = torch.empty(10, dtype=torch.int64, device="cuda")
     with pytest.raises(L.GpuSolveError):
         L.sgetrf_tp_dev(A, perm, MODE_FP32)
+
+
+def test_tournament_lu_three_levels(L):
+    import torch
+    n = 33000
+    g = torch.Generator(device="cuda").manual_seed(12)
+    a = torch.randn(n, n, dtype=torch.float32, device="cuda", generator=g)
+    A = a.clone()
+    perm = torch.empty(n, dtype=torch.int64, device="cuda")
+    print(f"tslu n={n} (three-level tree)", flush=True)
+    assert L.sgetrf_tp_dev(A, perm, MODE_FP32) == 0
+    p = perm.cpu().numpy()
+    assert np.array_equal(np.sort(p), np.arange(n)), "perm is not a permutation"
+    x = torch.randn(n, 4, dtype=torch.float64, device="cuda", generator=g)
+    low = torch.tril(A, -1).double()
+    low.diagonal().fill_(1.0)
+    up = torch.triu(A).double()
+    assert low.abs().max().item() <= 32
+    lhs = a.double()[perm] @ x
+    rhs = low @ (up @ x)
+    scale = (low.abs() @ (up.abs() @ x.abs())).max().item()
+    err = (lhs - rhs).abs().max().item() / scale
+    assert err <= 8 * n * U32, err
