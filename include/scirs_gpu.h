@@ -247,6 +247,20 @@ int scirs_gpu_prof_end(scirs_prof_stats *out);
 int scirs_gpu_backward_error_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
                                  const double *dB, int64_t ldb, void *stream, double *out);
 
+/* The same backward error with no allocation: its scratch (the residual, the
+ * row sums of |A| and the norms: n * nrhs + n + 200 doubles) is a fixed 8 MiB
+ * array in the library's code object instead of memory the library allocates,
+ * so the call makes no hipMalloc / hipFree or other allocating call.  No
+ * counterpart in the reference; it is for callers that cannot allocate (code
+ * being captured into a HIP graph, tight serving loops, harnesses that forbid
+ * allocation).  Limits: n * nrhs + n + 200 <= 1048576 (else SCIRS_ERR_ARG);
+ * scirs_gpu_init must have run (it resolves the array's device address;
+ * otherwise SCIRS_ERR_UNSUPPORTED).  Every call on the device shares the one
+ * array: calls that overlap on the device race (the library's lock serialises
+ * the calls of one process, each of which waits for its stream). */
+int scirs_gpu_backward_error_dev_noalloc(int n, int nrhs, const double *dA, int64_t lda, const double *dX,
+                                         int64_t ldx, const double *dB, int64_t ldb, void *stream, double *out);
+
 /* Tournament-pivoted FP32 LU (CALU) of the n x n matrix in HBM, in place (unit L
  * below the diagonal, U on and above; lda a multiple of 4), with the trailing
  * update in the arithmetic of `mode` (SCIRS_MODE_FP32, _BF16, _BF16X3, _BF16X6;

@@ -106,6 +106,8 @@ SIGNATURES = {
     "scirs_gpu_prof_begin": ([], _I),
     "scirs_gpu_prof_end": ([ctypes.POINTER(ProfStats)], _I),
     "scirs_gpu_backward_error_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, ctypes.POINTER(ctypes.c_double)], _I),
+    "scirs_gpu_backward_error_dev_noalloc": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, ctypes.POINTER(ctypes.c_double)],
+                                             _I),
 }
 
 _lib = None

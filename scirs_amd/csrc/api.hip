@@ -61,6 +61,13 @@ extern "C" const char *scirs_gpu_last_error(void)
     return "";
 }
 
+// Scratch of scirs_gpu_backward_error_dev_noalloc: a fixed array in the code
+// object, so that entry point calls no allocator.  Host code must use the
+// address hipGetSymbolAddress returns for it (Context::noalloc_scratch), never
+// the array's name, which on the host is a stand-in and not a device address.
+#define SCIRS_NOALLOC_SCRATCH_DOUBLES (1u << 20)   /* 8 MiB */
+static __device__ __align__(256) double scratch[SCIRS_NOALLOC_SCRATCH_DOUBLES];
+
 namespace scirs {
 
 static Context g_ctx;
@@ -178,6 +185,8 @@ int Context::init(int dev)
     work.seq_next = 1;
     work_base = base;
     SCIRS_HIP_CHECK(hipHostMalloc((void **)&host_flags, 64, hipHostMallocDefault));
+    // this device's copy of the allocation-free entry point's scratch
+    SCIRS_HIP_CHECK(hipGetSymbolAddress((void **)&noalloc_scratch, HIP_SYMBOL(scratch)));
     // panel widths: 8 .. 128 (the LDS budgets of the panel's neighbours — the
     // next-block interchange + TRSM kernel, the TRSM and Cholesky kernels — are
     // sized for kb <= 128)
@@ -244,14 +253,20 @@ int finish(scirs_solve_info *info, int rc, double t0)
 }
 
 // normwise backward error  max_j ||R_j||_inf / (||A||_inf ||X_j||_inf + ||B_j||_inf)
-int backward_error(int n, int nrhs, const double *dA, long lda, const double *dX, long ldx, const double *dB,
-                   long ldb, hipStream_t s, double *out)
+// Doubles of scratch the backward error uses: R (n x nrhs), row sums (n), norms.
+static size_t backward_error_scratch(int n, int nrhs)
 {
-    Context &c = ctx();
+    return (size_t)n * nrhs + (size_t)n + 3 * 64 + 8;
+}
+
+// The backward error with its scratch (backward_error_scratch(n, nrhs) doubles)
+// supplied.  No part of it needs to start at zero: every value read back is
+// written by a kernel of this call first.
+static int backward_error_in(int n, int nrhs, const double *dA, long lda, const double *dX, long ldx,
+                             const double *dB, long ldb, hipStream_t s, double *scratch, double *out)
+{
     int rc;
-    size_t need = ((size_t)n * nrhs + (size_t)n + 3 * 64 + 8) * sizeof(double);
-    if ((rc = c.ensure(&c.scratch_bwd, &c.scratch_bwd_bytes, need))) return rc;
-    double *R = (double *)c.scratch_bwd;
+    double *R = scratch;
     double *rowsum = R + (size_t)n * nrhs;
     double *norms = rowsum + n;   // [0..63] r, [64..127] x, [128..191] b, [192] A
     if ((rc = scirs_residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, nrhs, s))) return rc;
@@ -274,6 +289,15 @@ int backward_error(int n, int nrhs, const double *dA, long lda, const double *dX
     }
     *out = worst;
     return SCIRS_OK;
+}
+
+int backward_error(int n, int nrhs, const double *dA, long lda, const double *dX, long ldx, const double *dB,
+                   long ldb, hipStream_t s, double *out)
+{
+    Context &c = ctx();
+    if (int rc = c.ensure(&c.scratch_bwd, &c.scratch_bwd_bytes, backward_error_scratch(n, nrhs) * sizeof(double)))
+        return rc;
+    return backward_error_in(n, nrhs, dA, lda, dX, ldx, dB, ldb, s, (double *)c.scratch_bwd, out);
 }
 
 } // namespace scirs
@@ -680,6 +704,31 @@ extern "C" int scirs_gpu_backward_error_dev(int n, int nrhs, const double *dA, i
     SCIRS_ENTER(true);
     hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     return backward_error(n, nrhs, dA, lda, dX, ldx, dB, ldb, s, out);
+}
+
+extern "C" int scirs_gpu_backward_error_dev_noalloc(int n, int nrhs, const double *dA, int64_t lda,
+                                                    const double *dX, int64_t ldx, const double *dB, int64_t ldb,
+                                                    void *stream, double *out)
+{
+    if (n <= 0 || nrhs <= 0 || !dA || !dX || !dB || !out || lda < n || ldx < nrhs || ldb < nrhs) {
+        scirs_set_error("backward_error_dev_noalloc: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        return SCIRS_ERR_ARG;
+    }
+    if (backward_error_scratch(n, nrhs) > SCIRS_NOALLOC_SCRATCH_DOUBLES) {
+        scirs_set_error("backward_error_dev_noalloc: n * nrhs + n + 200 = %zu doubles exceed its fixed scratch of %u "
+                        "(use scirs_gpu_backward_error_dev)", backward_error_scratch(n, nrhs),
+                        (unsigned)SCIRS_NOALLOC_SCRATCH_DOUBLES);
+        return SCIRS_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    Context &c = ctx();
+    // initialising the library allocates, so it must have happened before
+    if (!c.initialised || !c.noalloc_scratch) {
+        scirs_set_error("backward_error_dev_noalloc: call scirs_gpu_init first (it sets up the fixed scratch)");
+        return SCIRS_ERR_UNSUPPORTED;
+    }
+    SCIRS_HIP_CHECK(hipSetDevice(c.device));
+    return backward_error_in(n, nrhs, dA, lda, dX, ldx, dB, ldb, (hipStream_t)stream, c.noalloc_scratch, out);
 }
 
 // Clear every sequence-tagged exchange word (a kernel, so it is safe under capture).

@@ -38,6 +38,7 @@ struct Context {
     void *scratch_t = nullptr; size_t scratch_t_bytes = 0;   // L^T of a Cholesky solve
     void *xtri_mem = nullptr; size_t xtri_bytes = 0;
     void *colexp_mem = nullptr; size_t colexp_bytes = 0;
+    double *noalloc_scratch = nullptr;   // device address of the fixed scratch array (set at init)
 
     int init(int dev);
     int ensure(void **p, size_t *have, size_t bytes);

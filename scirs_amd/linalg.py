@@ -356,6 +356,21 @@ def backward_error_dev(A, X, B):
     return out.value
 
 
+def backward_error_dev_noalloc(A, X, B):
+    """backward_error_dev through the allocation-free entry point (fixed scratch;
+    scirs_gpu_init must have run)."""
+    lib = N.load()
+    n, nrhs = B.shape
+    _dev_arg("backward_error_dev_noalloc", "A", A, n, n)
+    _dev_arg("backward_error_dev_noalloc", "X", X, n, nrhs)
+    _dev_arg("backward_error_dev_noalloc", "B", B, n, nrhs)
+    out = ctypes.c_double(0.0)
+    rc = lib.scirs_gpu_backward_error_dev_noalloc(n, nrhs, _tp(A), A.stride(0), _tp(X), X.stride(0), _tp(B),
+                                                  B.stride(0), _stream_ptr(), ctypes.byref(out))
+    _check("backward_error_dev_noalloc", rc, None)
+    return out.value
+
+
 def sgetrf_tp_dev(A, perm, mode=N.MODE_FP32):
     """Tournament-pivoted FP32 LU in place on the n x n float32 CUDA tensor A
     (row stride a multiple of 4); perm (int64, n) receives the row permutation.

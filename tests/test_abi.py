@@ -74,3 +74,20 @@ def test_exchange_tag_wraparound():
         rng = set(range(base, base + span))
         assert 0 not in rng and base + span - 1 <= top and not (rng & seen)
         seen |= rng
+
+
+def test_noalloc_entry_point_needs_init():
+    """The allocation-free backward error never initialises the library itself
+    (that allocates): before scirs_gpu_init it returns SCIRS_ERR_UNSUPPORTED
+    without touching a device (so this runs without a GPU), and it refuses
+    sizes beyond its fixed scratch before anything else."""
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("library not built")
+    import ctypes
+    lib = _native.load()
+    out = ctypes.c_double(0.0)
+    p = ctypes.c_void_p(16)   # never dereferenced on these paths
+    rc = lib.scirs_gpu_backward_error_dev_noalloc(10, 1, p, 10, p, 1, p, 1, None, ctypes.byref(out))
+    assert rc == _native.SCIRS_ERR_UNSUPPORTED and "scirs_gpu_init" in _native.last_error()
+    rc = lib.scirs_gpu_backward_error_dev_noalloc(1024, 1024, p, 1024, p, 1024, p, 1024, None, ctypes.byref(out))
+    assert rc == _native.SCIRS_ERR_ARG and "fixed scratch" in _native.last_error()
