@@ -414,12 +414,15 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
         for (int u = 0; u < 4; ++u) {
             const int i = 4 * ti + u;
             if (i >= w) continue;
+            if (4 * tj + 3 < nrest)
+                *reinterpret_cast<float4 *>(a.urest + i * KBT + 4 * tj) =
+                    make_float4(u12[u][0], u12[u][1], u12[u][2], u12[u][3]);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int c = 4 * tj + v;
                 if (c < nrest) {
                     L.pr[i][rest0 + c] = u12[u][v];
-                    a.urest[i * KBT + c] = u12[u][v];
+                    if (4 * tj + 3 >= nrest) a.urest[i * KBT + c] = u12[u][v];
                 }
             }
         }
@@ -434,9 +437,11 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
         *reinterpret_cast<float4 *>(a.uinv + i * TW + 4 * c4) = *reinterpret_cast<const float4 *>(&L.uinv[i][4 * c4]);
     }
     __syncthreads();
-    for (int q = t; q < w * kb; q += NT) {
-        const int j = q / kb, c = q - j * kb;
-        a.top[j * SCIRS_KB_MAX + c] = L.pr[j][c];
+    for (int q = t; q < w * (KBT / 4); q += NT) {
+        const int j = q / (KBT / 4), c4 = q % (KBT / 4);
+        if (4 * c4 < kb)
+            *reinterpret_cast<float4 *>(a.top + j * SCIRS_KB_MAX + 4 * c4) =
+                *reinterpret_cast<const float4 *>(&L.pr[j][4 * c4]);
     }
     // 6. the panel's interchanges: (position, original row) for every moved row
     if (t == 0) {
@@ -594,8 +599,10 @@ __global__ __launch_bounds__(NT) void tour_kernel(TourArgs a)
         }
         __syncthreads();
         nr = s_off[nch];
-        for (int c = 0; c < nch; ++c) {
-            for (int j = t; j < s_off[c + 1] - s_off[c]; j += NT) {
+        // every (child, pick) word at once: one round trip, not one per child
+        for (int q = t; q < nch * TW; q += NT) {
+            const int c = q / TW, j = q - c * TW;
+            if (j < s_off[c + 1] - s_off[c]) {
                 unsigned spins = 0;
                 u64 wv;
                 while (((wv = ld_agent(cb + c * (TW + 1) + 1 + j)) >> 32) != a.seq)
