@@ -370,6 +370,19 @@ int potrf(int n, F *a, scirs_solve_info *info)
     if constexpr (std::is_same<F, double>::value) return scirs_gpu_dpotrf(n, a, n, info);
     else return scirs_gpu_spotrf(n, a, n, info);
 }
+template <class F>
+int trsv(int n, const F *a, const F *b, F *x, bool lower, bool unit_diagonal, scirs_solve_info *info)
+{
+    if constexpr (std::is_same<F, double>::value)
+        return scirs_gpu_dtrsv(n, 1, a, n, b, 1, x, 1, lower, unit_diagonal, info);
+    else return scirs_gpu_strsv(n, 1, a, n, b, 1, x, 1, lower, unit_diagonal, info);
+}
+template <class F>
+int posv(int n, int nrhs, const F *a, const F *b, F *x, scirs_solve_info *info)
+{
+    if constexpr (std::is_same<F, double>::value) return scirs_gpu_dposv(n, nrhs, a, n, b, nrhs, x, nrhs, info);
+    else return scirs_gpu_sposv(n, nrhs, a, n, b, nrhs, x, nrhs, info);
+}
 } // namespace detail
 
 template <class F>
@@ -436,12 +449,11 @@ Result<Array1<F>> solve_triangular(const ArrayView2<F> &a, const ArrayView1<F> &
     if (auto e = validate_matrix_vector_dimensions(a, b, op)) return *e;
     if (auto e = validate_finite_matrix(a, op)) return *e;
     if (auto e = validate_finite_vector(b, op)) return *e;
+    static_assert(detail::supported<F>, "the GPU path handles f32 and f64");
     const std::size_t n = a.rows;
-    std::vector<double> ad(n * n), bd(n), xd(n);
-    for (std::size_t i = 0; i < n; ++i) {
-        bd[i] = (double)b[i];
-        for (std::size_t j = 0; j < n; ++j) ad[i * n + j] = (double)a(i, j);
-    }
+    // the substitution runs in F, as the reference's loop does
+    std::vector<F> ad = detail::standard(a), bd(n), xd(n);
+    for (std::size_t i = 0; i < n; ++i) bd[i] = b[i];
     // The reference stops at the first diagonal with |a_ii| < F::epsilon() in
     // substitution order (solve.rs:166-173, 186-193); only the error comes back,
     // so the test runs here, before any device work.
@@ -454,14 +466,14 @@ Result<Array1<F>> solve_triangular(const ArrayView2<F> &a, const ArrayView1<F> &
                                                         a.dim(), 1e16);
         }
     scirs_solve_info info;
-    int rc = scirs_gpu_dtrsv((int)n, 1, ad.data(), (int64_t)n, bd.data(), 1, xd.data(), 1, lower, unit_diagonal, &info);
+    int rc = detail::trsv<F>((int)n, ad.data(), bd.data(), xd.data(), lower, unit_diagonal, &info);
     if (rc > 0)
         return singular_matrix_with_suggestions(lower ? "triangular system solve (forward substitution)"
                                                       : "triangular system solve (back substitution)",
                                                 a.dim(), 1e16);
     if (rc < 0) return detail::gpu_failure(rc);
     Array1<F> x(n);
-    for (std::size_t i = 0; i < n; ++i) x[i] = (F)xd[i];
+    for (std::size_t i = 0; i < n; ++i) x[i] = xd[i];
     return x;
 }
 
@@ -972,18 +984,14 @@ Result<Array2<F>> compat_solve(const ArrayView2<F> &a, const ArrayView2<F> &b, b
     if (!pos) return solve_multiple(a, b, std::nullopt);
     if (auto e = validate_multiple_linear_systems(a, b, "Multiple linear systems solve")) return *e;
     const std::size_t n = a.rows, nrhs = b.cols;
-    std::vector<double> ad(n * n), bd(n * nrhs), xd(n * nrhs);
-    for (std::size_t i = 0; i < n; ++i) {
-        for (std::size_t j = 0; j < n; ++j) ad[i * n + j] = static_cast<double>(a(i, j));
-        for (std::size_t j = 0; j < nrhs; ++j) bd[i * nrhs + j] = static_cast<double>(b(i, j));
-    }
+    // factor and substitutions in F (f32 stays f32)
+    std::vector<F> ad = detail::standard(a), bd = detail::standard(b), xd(n * nrhs);
     scirs_solve_info info;
-    int rc = scirs_gpu_dposv((int)n, (int)nrhs, ad.data(), (int64_t)n, bd.data(), (int64_t)nrhs, xd.data(),
-                             (int64_t)nrhs, &info);
+    int rc = detail::posv<F>((int)n, (int)nrhs, ad.data(), bd.data(), xd.data(), &info);
     if (rc > 0) return non_positive_definite_with_suggestions("Cholesky decomposition", a.dim(), std::nullopt);
     if (rc < 0) return detail::gpu_failure(rc);
     Array2<F> x(n, nrhs);
-    for (std::size_t i = 0; i < n * nrhs; ++i) x.data[i] = static_cast<F>(xd[i]);
+    x.data = std::move(xd);
     return x;
 }
 

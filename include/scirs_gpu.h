@@ -100,12 +100,20 @@ int scirs_gpu_spotrf(int n, float *a, int64_t lda, scirs_solve_info *info);
  * solve_triangular(L^T) (decomposition.rs:44, solve.rs:139): X = A^-1 B. */
 int scirs_gpu_dposv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
                     double *x, int64_t ldx, scirs_solve_info *info);
+/* The same composition for F = f32 (decomposition.rs:44 and solve.rs:139-202
+ * instantiated at f32): factor and both substitutions compute in f32. */
+int scirs_gpu_sposv(int n, int nrhs, const float *a, int64_t lda, const float *b, int64_t ldb,
+                    float *x, int64_t ldx, scirs_solve_info *info);
 
 /* ---- triangular ------------------------------------------------------------ */
 /* Replaces scirs2-linalg/src/solve.rs:139-202 `solve_triangular` for nrhs
  * right-hand sides.  Returns i+1 when a non-unit diagonal has |a_ii| < eps. */
 int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb,
                     double *x, int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info);
+/* solve.rs:139-202 `solve_triangular` for F = f32: substitution in f32, and the
+ * singular test |a_ii| < f32::EPSILON (solve.rs:166, 186). */
+int scirs_gpu_strsv(int n, int nrhs, const float *a, int64_t lda, const float *b, int64_t ldb,
+                    float *x, int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info);
 
 /* ---- QR and least squares --------------------------------------------------- */
 /* Replaces scirs2-linalg/src/lapack.rs:171-253 `qr_factor` (behind
@@ -224,6 +232,16 @@ int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *nrhs, const 
                                 const double *dA, const int64_t *offA, const double *dB, const int64_t *offB,
                                 double *dX, int *status, double *bwd, int max_iter, double tol_scale, void *stream,
                                 scirs_batch_info *info);
+/* The same batched solve on host memory (the entry a Rust `solve_batched` binds
+ * without HIP calls of its own): A[i] (n_i x n_i, row stride lda[i]), B[i] and
+ * X[i] (n_i x nrhs_i, row strides ldb[i], ldx[i]) are host pointers.  The
+ * systems travel in one pinned copy each way; status, bwd and info as above
+ * (info->total_ms includes both copies).  X[i] is written for every system;
+ * it holds no solution where status[i] != 0. */
+int scirs_gpu_dgesv_batched(int batch, const int *n, const int *nrhs, const unsigned char *spd,
+                            const double *const *A, const int64_t *lda, const double *const *B, const int64_t *ldb,
+                            double *const *X, const int64_t *ldx, int *status, double *bwd, int max_iter,
+                            double tol_scale, scirs_batch_info *info);
 
 /* ---- diagnostics -------------------------------------------------------------- */
 /* Kernel-class timing with HIP events on the launching stream (no reference

@@ -84,6 +84,8 @@ SIGNATURES = {
     "scirs_gpu_spotrf": ([_I, _P, _L, _INFO], _I),
     "scirs_gpu_dposv": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
     "scirs_gpu_dtrsv": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _INFO], _I),
+    "scirs_gpu_strsv": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _INFO], _I),
+    "scirs_gpu_sposv": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
     "scirs_gpu_dgmres": ([_I, _P, _L, _P, _P, _I, _D, _I, _P, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dgmres_dev": ([_I, _P, _L, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dbatch_matmul": ([_I, _I, _I, _I, _P, _P, _P, _INFO], _I),
@@ -103,6 +105,8 @@ SIGNATURES = {
     "scirs_gpu_dgesv_ir_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _I, _I, _I, _D, _P, _INFO], _I),
     "scirs_gpu_dgesv_batched_dev": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _D, _P,
                                      ctypes.POINTER(BatchInfo)], _I),
+    "scirs_gpu_dgesv_batched": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _D,
+                                 ctypes.POINTER(BatchInfo)], _I),
     "scirs_gpu_prof_begin": ([], _I),
     "scirs_gpu_prof_end": ([ctypes.POINTER(ProfStats)], _I),
     "scirs_gpu_backward_error_dev": ([_I, _I, _P, _L, _P, _L, _P, _L, _P, ctypes.POINTER(ctypes.c_double)], _I),

@@ -555,8 +555,9 @@ extern "C" int scirs_gpu_spotrf(int n, float *a, int64_t lda, scirs_solve_info *
     return host_potrf<float>(n, a, lda, info);
 }
 
-static int posv_device(int n, int nrhs, const double *dA, long lda, const double *dB, long ldb, double *dX, long ldx,
-                       double *dL, long ldl, hipStream_t s, scirs_solve_info *info)
+template <typename T>
+static int posv_device(int n, int nrhs, const T *dA, long lda, const T *dB, long ldb, T *dX, long ldx, T *dL,
+                       long ldl, hipStream_t s, scirs_solve_info *info)
 {
     Context &c = ctx();
     int rc;
@@ -566,19 +567,19 @@ static int posv_device(int n, int nrhs, const double *dA, long lda, const double
     SCIRS_HIP_CHECK(hipEventRecord(e0, s));
     if ((rc = scirs_d2d_copy(n, n, dA, lda, dL, ldl, s))) return rc;
     if ((rc = c.reset_flags(s))) return rc;
-    if ((rc = scirs_potrf_device<double>(n, dL, ldl, c.kb_chol, c.work, s))) return rc;
+    if ((rc = scirs_potrf_device<T>(n, dL, ldl, c.kb_chol, c.work, s))) return rc;
     SCIRS_HIP_CHECK(hipEventRecord(e1, s));
     if ((rc = scirs_d2d_copy(n, nrhs, dB, ldb, dX, ldx, s))) return rc;
     if (c.chol_chain) {
         // L y = b and L^T x = y as chained solves (true division by the diagonal,
         // solve.rs:159-199), the second on an explicit row-major L^T
-        if ((rc = c.ensure(&c.scratch_t, &c.scratch_t_bytes, (size_t)n * n * sizeof(double)))) return rc;
-        double *dU = (double *)c.scratch_t;
-        if ((rc = scirs_transpose_lower<double>(n, dL, ldl, dU, n, s))) return rc;
+        if ((rc = c.ensure(&c.scratch_t, &c.scratch_t_bytes, (size_t)n * n * sizeof(T)))) return rc;
+        T *dU = (T *)c.scratch_t;
+        if ((rc = scirs_transpose_lower<T>(n, dL, ldl, dU, n, s))) return rc;
         if ((rc = c.ensure_xtri(n))) return rc;
-        if ((rc = scirs_trsv_chain<double>(n, nrhs, dL, ldl, dX, ldx, 1, 0, c.work, s))) return rc;
-        if ((rc = scirs_trsv_chain<double>(n, nrhs, dU, n, dX, ldx, 0, 0, c.work, s))) return rc;
-    } else if ((rc = scirs_potrs_device<double>(n, nrhs, dL, ldl, dX, ldx, c.work.info, s))) {
+        if ((rc = scirs_trsv_chain<T>(n, nrhs, dL, ldl, dX, ldx, 1, 0, c.work, s))) return rc;
+        if ((rc = scirs_trsv_chain<T>(n, nrhs, dU, n, dX, ldx, 0, 0, c.work, s))) return rc;
+    } else if ((rc = scirs_potrs_device<T>(n, nrhs, dL, ldl, dX, ldx, c.work.info, s))) {
         return rc;
     }
     SCIRS_HIP_CHECK(hipEventRecord(e2, s));
@@ -596,36 +597,49 @@ static int posv_device(int n, int nrhs, const double *dA, long lda, const double
     return st;
 }
 
-extern "C" int scirs_gpu_dposv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb, double *x,
-                               int64_t ldx, scirs_solve_info *info)
+template <typename T>
+static int host_posv(int n, int nrhs, const T *a, int64_t lda, const T *b, int64_t ldb, T *x, int64_t ldx,
+                     scirs_solve_info *info)
 {
     const double t0 = now_ms();
     info_init(info, SCIRS_PATH_FP64_CHOL);
     if (n <= 0 || nrhs <= 0 || !a || !b || !x || lda < n || ldb < nrhs || ldx < nrhs) {
-        scirs_set_error("dposv: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        scirs_set_error("%cposv: invalid arguments (n=%d nrhs=%d)", sizeof(T) == 8 ? 'd' : 's', n, nrhs);
         return finish(info, SCIRS_ERR_ARG, t0);
     }
     SCIRS_ENTER(true);
     Context &c = ctx();
     hipStream_t s = c.stream;
     int rc;
-    const size_t abytes = (size_t)n * n * sizeof(double), bbytes = (size_t)n * nrhs * sizeof(double);
+    const size_t abytes = (size_t)n * n * sizeof(T), bbytes = (size_t)n * nrhs * sizeof(T);
     if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, 2 * abytes))) return finish(info, rc, t0);
     if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, 2 * bbytes))) return finish(info, rc, t0);
-    double *dA = (double *)c.scratch_a, *dL = dA + (size_t)n * n;
-    double *dB = (double *)c.scratch_b, *dX = dB + (size_t)n * nrhs;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n,
+    T *dA = (T *)c.scratch_a, *dL = dA + (size_t)n * n;
+    T *dB = (T *)c.scratch_b, *dX = dB + (size_t)n * nrhs;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n,
                                      hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dB, nrhs * sizeof(double), b, ldb * sizeof(double), nrhs * sizeof(double), n,
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dB, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n,
                                      hipMemcpyHostToDevice, s));
-    int st = posv_device(n, nrhs, dA, n, dB, nrhs, dX, nrhs, dL, n, s, info);
+    int st = posv_device<T>(n, nrhs, dA, n, dB, nrhs, dX, nrhs, dL, n, s, info);
     if (st < 0) return finish(info, st, t0);
     if (st == 0) {
-        SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(double), dX, nrhs * sizeof(double), nrhs * sizeof(double), n,
+        SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n,
                                          hipMemcpyDeviceToHost, s));
         SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     }
     return finish(info, st, t0);
+}
+
+extern "C" int scirs_gpu_dposv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb, double *x,
+                               int64_t ldx, scirs_solve_info *info)
+{
+    return host_posv<double>(n, nrhs, a, lda, b, ldb, x, ldx, info);
+}
+
+extern "C" int scirs_gpu_sposv(int n, int nrhs, const float *a, int64_t lda, const float *b, int64_t ldb, float *x,
+                               int64_t ldx, scirs_solve_info *info)
+{
+    return host_posv<float>(n, nrhs, a, lda, b, ldb, x, ldx, info);
 }
 
 extern "C" int scirs_gpu_dposv_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dB, int64_t ldb,
@@ -647,27 +661,28 @@ extern "C" int scirs_gpu_dposv_dev(int n, int nrhs, const double *dA, int64_t ld
         dL = (double *)c.scratch_a;
         ldl = n;
     }
-    int st = posv_device(n, nrhs, dA, lda, dB, ldb, dX, ldx, dL, ldl, s, info);
+    int st = posv_device<double>(n, nrhs, dA, lda, dB, ldb, dX, ldx, dL, ldl, s, info);
     return finish(info, st, t0);
 }
 
 // ------------------------------------------------------------------ triangular
-extern "C" int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb, double *x,
-                               int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info)
+template <typename T>
+static int host_trsv(int n, int nrhs, const T *a, int64_t lda, const T *b, int64_t ldb, T *x, int64_t ldx, int lower,
+                     int unit_diagonal, scirs_solve_info *info)
 {
     const double t0 = now_ms();
     info_init(info, SCIRS_PATH_FP64_LU);
     if (n <= 0 || nrhs <= 0 || !a || !b || !x || lda < n || ldb < nrhs || ldx < nrhs) {
-        scirs_set_error("dtrsv: invalid arguments (n=%d nrhs=%d)", n, nrhs);
+        scirs_set_error("%ctrsv: invalid arguments (n=%d nrhs=%d)", sizeof(T) == 8 ? 'd' : 's', n, nrhs);
         return finish(info, SCIRS_ERR_ARG, t0);
     }
-    // |a_ii| < eps on a non-unit diagonal is singular (solve.rs:163-170, 184-191);
+    // |a_ii| < eps (of T) on a non-unit diagonal is singular (solve.rs:163-170, 184-191);
     // the first such row in substitution order is reported.
     if (!unit_diagonal) {
         for (int k = 0; k < n; ++k) {
             int i = lower ? k : n - 1 - k;
-            double d = a[(size_t)i * lda + i];
-            if (fabs(d) < DBL_EPSILON) {
+            const double d = (double)a[(size_t)i * lda + i];
+            if (fabs(d) < (sizeof(T) == 8 ? DBL_EPSILON : (double)FLT_EPSILON)) {
                 if (info) info->pivot_value = fabs(d);
                 return finish(info, i + 1, t0);
             }
@@ -677,23 +692,35 @@ extern "C" int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, co
     Context &c = ctx();
     hipStream_t s = c.stream;
     int rc;
-    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(double)))) return finish(info, rc, t0);
-    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, (size_t)n * nrhs * sizeof(double)))) return finish(info, rc, t0);
-    double *dA = (double *)c.scratch_a, *dX = (double *)c.scratch_b;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n,
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(T)))) return finish(info, rc, t0);
+    if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, (size_t)n * nrhs * sizeof(T)))) return finish(info, rc, t0);
+    T *dA = (T *)c.scratch_a, *dX = (T *)c.scratch_b;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n,
                                      hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dX, nrhs * sizeof(double), b, ldb * sizeof(double), nrhs * sizeof(double), n,
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dX, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n,
                                      hipMemcpyHostToDevice, s));
     if ((rc = c.ensure_xtri(n))) return finish(info, rc, t0);
     if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
-    if ((rc = scirs_trsv_chain<double>(n, nrhs, dA, n, dX, nrhs, lower, unit_diagonal, c.work, s)))
+    if ((rc = scirs_trsv_chain<T>(n, nrhs, dA, n, dX, nrhs, lower, unit_diagonal, c.work, s)))
         return finish(info, rc, t0);
     int st = 0;
     if ((rc = c.read_flags(s, &st, nullptr))) return finish(info, rc, t0);
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(double), dX, nrhs * sizeof(double), nrhs * sizeof(double), n,
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n,
                                      hipMemcpyDeviceToHost, s));
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
+}
+
+extern "C" int scirs_gpu_dtrsv(int n, int nrhs, const double *a, int64_t lda, const double *b, int64_t ldb, double *x,
+                               int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info)
+{
+    return host_trsv<double>(n, nrhs, a, lda, b, ldb, x, ldx, lower, unit_diagonal, info);
+}
+
+extern "C" int scirs_gpu_strsv(int n, int nrhs, const float *a, int64_t lda, const float *b, int64_t ldb, float *x,
+                               int64_t ldx, int lower, int unit_diagonal, scirs_solve_info *info)
+{
+    return host_trsv<float>(n, nrhs, a, lda, b, ldb, x, ldx, lower, unit_diagonal, info);
 }
 
 // ------------------------------------------------------------------ diagnostics

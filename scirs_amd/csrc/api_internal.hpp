@@ -95,6 +95,7 @@ int scirs_rowsum_max(int n, const double *rowsum, double *out, hipStream_t s);
 int scirs_d2s(int rows, int cols, const double *src, long lds, float *dst, long ldd, hipStream_t s);
 int scirs_s2d(int rows, int cols, const float *src, long lds, double *dst, long ldd, hipStream_t s);
 int scirs_d2d_copy(int rows, int cols, const double *src, long lds, double *dst, long ldd, hipStream_t s);
+int scirs_d2d_copy(int rows, int cols, const float *src, long lds, float *dst, long ldd, hipStream_t s);
 int scirs_add_s_to_d(int rows, int cols, const float *D, long ldd, double *X, long ldx, hipStream_t s);
 int scirs_add_d_to_d(int rows, int cols, const double *D, long ldd, double *X, long ldx, hipStream_t s);
 int scirs_fill_int(int *p, int count, int value, hipStream_t s);

@@ -724,3 +724,94 @@ extern "C" int scirs_gpu_dgesv_batched_dev(int batch, const int *n, const int *n
     }
     return SCIRS_OK;
 }
+
+// ---------------------------------------------------------------- host memory
+// Host-memory batched solve: the systems are packed into pinned staging, sent
+// in one copy, solved by the device entry point above, and the solutions come
+// back in one copy.  Staging (pinned and device) is kept between calls and
+// grown on demand, so a repeated batch of the same shape replays the cached
+// graph on the same device buffers.
+namespace {
+struct HostBatch {
+    double *h = nullptr;   // pinned: packed A | packed B (X on the way back)
+    size_t hcap = 0;
+    double *d = nullptr;   // device: packed A | packed B | packed X
+    size_t dcap = 0;
+};
+HostBatch g_hb;
+
+int grow_pinned(double **p, size_t *cap, size_t count)
+{
+    if (*cap >= count && *p) return SCIRS_OK;
+    if (*p) {
+        SCIRS_HIP_CHECK(hipDeviceSynchronize());
+        SCIRS_HIP_CHECK(hipHostFree(*p));
+        *p = nullptr;
+    }
+    if (hipHostMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        scirs_set_error("batched: pinned allocation of %zu bytes failed", count * sizeof(double));
+        *cap = 0;
+        return SCIRS_ERR_NOMEM;
+    }
+    *cap = count;
+    return SCIRS_OK;
+}
+} // namespace
+
+extern "C" int scirs_gpu_dgesv_batched(int batch, const int *n, const int *nrhs, const unsigned char *spd,
+                                       const double *const *A, const int64_t *lda, const double *const *B,
+                                       const int64_t *ldb, double *const *X, const int64_t *ldx, int *status,
+                                       double *bwd, int max_iter, double tol_scale, scirs_batch_info *info)
+{
+    using namespace scirs;
+    const double t0 = now_ms();
+    if (info) memset(info, 0, sizeof(*info));
+    if (batch <= 0 || !n || !nrhs || !A || !lda || !B || !ldb || !X || !ldx || !status) {
+        scirs_set_error("dgesv_batched: invalid arguments");
+        return SCIRS_ERR_ARG;
+    }
+    std::vector<int64_t> offA(batch), offB(batch);
+    size_t na = 0, nb = 0;
+    for (int i = 0; i < batch; ++i) {
+        if (n[i] < 1 || n[i] > NMAX || nrhs[i] < 1 || nrhs[i] > 16 || !A[i] || !B[i] || !X[i] || lda[i] < n[i] ||
+            ldb[i] < nrhs[i] || ldx[i] < nrhs[i]) {
+            scirs_set_error("dgesv_batched: system %d has n=%d nrhs=%d lda=%lld ldb=%lld ldx=%lld or a null pointer "
+                            "(1 <= n <= %d, 1 <= nrhs <= 16, leading dimensions >= the row length)",
+                            i, n[i], nrhs[i], (long long)lda[i], (long long)ldb[i], (long long)ldx[i], NMAX);
+            return SCIRS_ERR_ARG;
+        }
+        offA[i] = (int64_t)na;
+        offB[i] = (int64_t)nb;
+        na += (size_t)n[i] * n[i];
+        nb += (size_t)n[i] * nrhs[i];
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    Context &c = ctx();
+    if (int rc = c.init(0)) return rc;
+    hipStream_t s = c.stream;
+    HostBatch &hb = g_hb;
+    if (int rc = grow_pinned(&hb.h, &hb.hcap, na + nb)) return rc;
+    if (int rc = grow(&hb.d, &hb.dcap, na + 2 * nb)) return rc;
+    double *hA = hb.h, *hB = hb.h + na;
+    for (int i = 0; i < batch; ++i) {
+        const int ni = n[i], ri = nrhs[i];
+        for (int r = 0; r < ni; ++r) {
+            memcpy(hA + offA[i] + (size_t)r * ni, A[i] + (size_t)r * lda[i], sizeof(double) * ni);
+            memcpy(hB + offB[i] + (size_t)r * ri, B[i] + (size_t)r * ldb[i], sizeof(double) * ri);
+        }
+    }
+    double *dA = hb.d, *dB = hb.d + na, *dX = hb.d + na + nb;
+    SCIRS_HIP_CHECK(hipMemcpyAsync(dA, hA, sizeof(double) * (na + nb), hipMemcpyHostToDevice, s));
+    int rc = scirs_gpu_dgesv_batched_dev(batch, n, nrhs, spd, dA, offA.data(), dB, offB.data(), dX, status, bwd,
+                                         max_iter, tol_scale, s, info);
+    if (rc) return rc;
+    SCIRS_HIP_CHECK(hipMemcpyAsync(hB, dX, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < batch; ++i) {
+        const int ni = n[i], ri = nrhs[i];
+        for (int r = 0; r < ni; ++r)
+            memcpy(X[i] + (size_t)r * ldx[i], hB + offB[i] + (size_t)r * ri, sizeof(double) * ri);
+    }
+    if (info) info->total_ms = now_ms() - t0;
+    return SCIRS_OK;
+}

@@ -186,6 +186,168 @@ __global__ __launch_bounds__(NTHREADS, CPRE ? 1 : 2) void gemm_sub_kernel(GemmAr
     }
 }
 
+// FP64, op(B) = B, K = KS * 16 (the LU's rank-128 update: KS = 8), two
+// workgroups per CU.  C is never held beside the accumulators: A is negated on
+// its way into LDS, so the MFMAs accumulate -A B, and each wave folds C in one
+// block row at a time — the loads of block row i are issued after slab 2i's
+// operand loads and added during slab 2i + 1, so they overlap the matrix-core
+// work instead of preceding it (the memory counter retires loads in order: a
+// wait for the slab operands never waits for C).  Operands come in as 16-byte
+// loads; B goes to LDS as 16-byte stores.
+template <int KS, bool VEC, int OCC, bool TRANS_B>
+__global__ __launch_bounds__(NTHREADS, OCC) void dgemm_fold_kernel(GemmArgs<double> g)
+{
+    static_assert(KS >= 8 && KS % 8 == 0, "eight half block rows of C folded over the slabs");
+    if (g.info && *g.info != 0) return;
+    int tm, tn;
+    scirs_tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n, tm, tn);
+    if (tm >= g.tiles_m || tn >= g.tiles_n) return;
+    const int row0 = tm * BM, col0 = tn * BN;
+    if (g.lower && col0 > row0 + BM - 1) return;   // SYRK: tiles on or below the diagonal
+
+    __shared__ double As[2][BK * LDA_S];
+    __shared__ double Bs[2][BK * LDB_S];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int a_r = t >> 1, a_k = (t & 1) * 8;
+    const int b_k = t >> 4, b_c = (t & 15) * 8;
+    // Rows of A past M and columns of B past N only feed outputs that are never
+    // stored, so their loads are clamped to a valid address instead of
+    // predicated: no branch around a load, so the memory-counter waits stay exact.
+    const double *pa0 = g.A + (long)min(row0 + a_r, g.M - 1) * g.lda + a_k;
+    // op(B) = B^T: B is read like A, row col0 + a_r, eight consecutive k
+    const double *pbt0 = g.B + (long)min(col0 + a_r, g.N - 1) * g.ldb + a_k;
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    typedef double d8 __attribute__((ext_vector_type(8)));
+    i4 bcol;   // first column of each of the thread's four B column pairs
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int c = col0 + b_c + 2 * i;
+        bcol[i] = VEC ? (c + 1 < g.N ? c : g.N - 2) : c;
+    }
+    d8 ra, rb;   // vector types: the staging stays in registers
+
+    auto load_slab = [&](int k0) {
+        const double *pa = pa0 + k0;
+        const double *pb = g.B + (long)(k0 + b_k) * g.ldb;
+        const double *pbt = pbt0 + k0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double2 va, vb;
+            if (VEC) {
+                va = *reinterpret_cast<const double2 *>(pa + 2 * i);
+                vb = TRANS_B ? *reinterpret_cast<const double2 *>(pbt + 2 * i)
+                             : *reinterpret_cast<const double2 *>(pb + bcol[i]);
+            } else {
+                va = double2{pa[2 * i], pa[2 * i + 1]};
+                vb = TRANS_B ? double2{pbt[2 * i], pbt[2 * i + 1]}
+                             : double2{pb[min(bcol[i], g.N - 1)], pb[min(bcol[i] + 1, g.N - 1)]};
+            }
+            ra[2 * i] = va.x;
+            ra[2 * i + 1] = va.y;
+            rb[2 * i] = vb.x;
+            rb[2 * i + 1] = vb.y;
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) As[buf][(a_k + i) * LDA_S + a_r] = -ra[i];
+        if (TRANS_B) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) Bs[buf][(a_k + i) * LDB_S + a_r] = rb[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                *reinterpret_cast<double2 *>(&Bs[buf][b_k * LDB_S + b_c + 2 * i]) =
+                    double2{rb[2 * i], rb[2 * i + 1]};
+        }
+    };
+
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0, 0, 0, 0};
+
+    const int lr = lane & 15;
+    const int lk = lane >> 4;
+    // lane l, register q of a 16 x 16 f64 block: column l % 16, row l / 16 + 4 q
+    // C arrives in eight half block rows h = 0..7 (block row h / 2, column
+    // blocks 2 (h % 2) and 2 (h % 2) + 1), one per slab
+    double cb[2][4];
+    auto load_c = [&](int h) {
+        const int i = h >> 1;
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            const int c = min(col0 + wn * 64 + (2 * (h & 1) + jj) * 16 + lr, g.N - 1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = min(row0 + wm * 64 + i * 16 + lk + 4 * q, g.M - 1);
+                cb[jj][q] = g.C[(long)r * g.ldc + c];
+            }
+        }
+    };
+    auto fold_c = [&](int h) {
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[h >> 1][2 * (h & 1) + jj][q] += cb[jj][q];
+    };
+
+    load_slab(0);
+    store_slab(0);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < KS) load_slab((s + 1) * BK);
+        // C half row h: loaded during slab h * KS / 8, folded in the next slab
+        constexpr int STEP = KS / 8;
+        if (s >= STEP && s % STEP == 0) fold_c(s / STEP - 1);
+        if (s % STEP == 0) load_c(s / STEP);
+        // keep the loads ahead of the matrix-core work and the LDS stores behind it
+        __builtin_amdgcn_sched_barrier(0);
+        const double *as = As[buf];
+        const double *bs = Bs[buf];
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = as[(kk + lk) * LDA_S + wm * 64 + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = bs[(kk + lk) * LDB_S + wn * 64 + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma16x16x4(af[i], bf[j], acc[i][j]);
+            // one k-step of fragments in registers at a time (the other wave on
+            // the SIMD covers the LDS latency)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (s + 1 < KS) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+    fold_c(7);
+    // epilogue: C = C - A B, already in the accumulators
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = col0 + wn * 64 + j * 16 + lr;
+            if (c >= g.N) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = row0 + wm * 64 + i * 16 + lk + 4 * q;
+                if (r < g.M) g.C[(long)r * g.ldc + c] = acc[i][j][q];
+            }
+        }
+    }
+}
+
 template <typename T>
 int launch_gemm_sub(int M, int N, int K, const T *A, long lda, const T *B, long ldb, int trans_b, T *C,
                     long ldc, int lower, const int *info, hipStream_t stream)
@@ -205,7 +367,29 @@ int launch_gemm_sub(int M, int N, int K, const T *A, long lda, const T *B, long 
         const char *v = getenv("SCIRS_GPU_DGEMM_CPRE");
         return !(v && *v && atoi(v) == 0);
     }();
-    if (trans_b)
+    static const int fold = [] {
+        const char *v = getenv("SCIRS_GPU_DGEMM_FOLD");
+        return v && *v ? atoi(v) : 1;
+    }();
+    if (sizeof(T) == 8 && fold && K == 8 * BK) {
+        // the rank-128 update (LU) and SYRK (Cholesky): C folded into the
+        // accumulators, two workgroups per CU
+        const GemmArgs<double> &gd = reinterpret_cast<const GemmArgs<double> &>(g);
+        // 16-byte operand loads: aligned rows and (B not transposed) column pairs
+        // that are both in range or both out of it
+        const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0 &&
+                         (trans_b || N % 2 == 0);
+        if (trans_b) {
+            if (vec)
+                hipLaunchKernelGGL((dgemm_fold_kernel<8, true, 2, true>), dim3(nwg), dim3(NTHREADS), 0, stream, gd);
+            else
+                hipLaunchKernelGGL((dgemm_fold_kernel<8, false, 1, true>), dim3(nwg), dim3(NTHREADS), 0, stream, gd);
+        } else if (vec) {
+            hipLaunchKernelGGL((dgemm_fold_kernel<8, true, 2, false>), dim3(nwg), dim3(NTHREADS), 0, stream, gd);
+        } else {
+            hipLaunchKernelGGL((dgemm_fold_kernel<8, false, 1, false>), dim3(nwg), dim3(NTHREADS), 0, stream, gd);
+        }
+    } else if (trans_b)
         hipLaunchKernelGGL((gemm_sub_kernel<T, true, false>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
     else if (cpre && sizeof(T) == 8)
         hipLaunchKernelGGL((gemm_sub_kernel<T, false, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);

@@ -253,6 +253,15 @@ int scirs_d2d_copy(int rows, int cols, const double *src, long lds, double *dst,
     return SCIRS_OK;
 }
 
+int scirs_d2d_copy(int rows, int cols, const float *src, long lds, float *dst, long ldd, hipStream_t s)
+{
+    long count = (long)rows * cols;
+    hipLaunchKernelGGL((convert_kernel<float, float>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                       count, cols, src, lds, dst, ldd);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+
 int scirs_add_s_to_d(int rows, int cols, const float *D, long ldd, double *X, long ldx, hipStream_t s)
 {
     long count = (long)rows * cols;

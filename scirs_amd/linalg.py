@@ -89,31 +89,37 @@ def potrf(a):
     return l, info
 
 
-def posv(a, b):
+def posv(a, b, dtype=np.float64):
+    """Cholesky solve in f64 (scirs_gpu_dposv) or f32 (scirs_gpu_sposv)."""
     lib = N.load()
-    a = _c(a, np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    dt = np.dtype(dtype)
+    a = _c(a, dt)
+    b = np.asarray(b, dtype=dt)
     vec = b.ndim == 1
-    b2 = _c(b.reshape(b.shape[0], -1), np.float64)
+    b2 = _c(b.reshape(b.shape[0], -1), dt)
     n, nrhs = b2.shape
     x = np.empty_like(b2)
     info = N.SolveInfo()
-    rc = lib.scirs_gpu_dposv(n, nrhs, _ptr(a), n, _ptr(b2), nrhs, _ptr(x), nrhs, ctypes.byref(info))
+    fn = lib.scirs_gpu_sposv if dt == np.float32 else lib.scirs_gpu_dposv
+    rc = fn(n, nrhs, _ptr(a), n, _ptr(b2), nrhs, _ptr(x), nrhs, ctypes.byref(info))
     _check("posv", rc, info, chol=True)
     return (x[:, 0] if vec else x), info
 
 
-def trsv(a, b, lower, unit_diagonal):
+def trsv(a, b, lower, unit_diagonal, dtype=np.float64):
+    """Triangular solve in f64 (scirs_gpu_dtrsv) or f32 (scirs_gpu_strsv)."""
     lib = N.load()
-    a = _c(a, np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    dt = np.dtype(dtype)
+    a = _c(a, dt)
+    b = np.asarray(b, dtype=dt)
     vec = b.ndim == 1
-    b2 = _c(b.reshape(b.shape[0], -1), np.float64)
+    b2 = _c(b.reshape(b.shape[0], -1), dt)
     n, nrhs = b2.shape
     x = np.empty_like(b2)
     info = N.SolveInfo()
-    rc = lib.scirs_gpu_dtrsv(n, nrhs, _ptr(a), n, _ptr(b2), nrhs, _ptr(x), nrhs, int(lower), int(unit_diagonal),
-                             ctypes.byref(info))
+    fn = lib.scirs_gpu_strsv if dt == np.float32 else lib.scirs_gpu_dtrsv
+    rc = fn(n, nrhs, _ptr(a), n, _ptr(b2), nrhs, _ptr(x), nrhs, int(lower), int(unit_diagonal),
+            ctypes.byref(info))
     _check("trsv", rc, info)
     return (x[:, 0] if vec else x), info
 
@@ -448,6 +454,43 @@ def gesv_batched_dev(n, nrhs, offA, offB, dA, dB, dX, spd=None, max_iter=0, tol_
                                          ctypes.byref(info))
     _check("gesv_batched_dev", rc, None)
     return status, bwd, info
+
+
+def gesv_batched_host(As, Bs, spd=None, max_iter=0, tol_scale=0.0):
+    """Batched solve through the host-memory entry point scirs_gpu_dgesv_batched:
+    lists of f64 numpy systems (any row strides) in, list of solutions out.
+    Returns (xs, status, bwd, info)."""
+    lib = N.load()
+    batch = len(As)
+    if len(Bs) != batch or (spd is not None and len(spd) != batch):
+        raise ValueError("gesv_batched_host: one B (and spd flag) per A")
+    As = [np.asarray(a, dtype=np.float64) for a in As]
+    Bs = [np.asarray(b, dtype=np.float64) for b in Bs]
+    Bs = [b.reshape(-1, 1) if b.ndim == 1 else b for b in Bs]
+    for i, (a, b) in enumerate(zip(As, Bs)):
+        if a.ndim != 2 or a.shape[0] != a.shape[1] or b.ndim != 2 or b.shape[0] != a.shape[0]:
+            raise ValueError(f"gesv_batched_host: system {i} has A {a.shape} and B {b.shape}")
+    As = [a if a.strides[1] == 8 and a.strides[0] > 0 else np.ascontiguousarray(a) for a in As]
+    Bs = [b if b.strides[1] == 8 and b.strides[0] > 0 else np.ascontiguousarray(b) for b in Bs]
+    n = np.array([a.shape[0] for a in As], dtype=np.int32)
+    nrhs = np.array([b.shape[1] for b in Bs], dtype=np.int32)
+    xs = [np.zeros((int(n[i]), int(nrhs[i]))) for i in range(batch)]
+    PA = (ctypes.c_void_p * batch)(*[a.ctypes.data for a in As])
+    PB = (ctypes.c_void_p * batch)(*[b.ctypes.data for b in Bs])
+    PX = (ctypes.c_void_p * batch)(*[x.ctypes.data for x in xs])
+    lda = np.array([a.strides[0] // 8 for a in As], dtype=np.int64)
+    ldb = np.array([b.strides[0] // 8 for b in Bs], dtype=np.int64)
+    ldx = np.array([x.strides[0] // 8 for x in xs], dtype=np.int64)
+    spd_arr = None if spd is None else np.ascontiguousarray(spd, dtype=np.uint8)
+    status = np.zeros(batch, dtype=np.int32)
+    bwd = np.zeros(batch, dtype=np.float64)
+    info = N.BatchInfo()
+    rc = lib.scirs_gpu_dgesv_batched(batch, _ptr(n), _ptr(nrhs), _ptr(spd_arr) if spd_arr is not None else None,
+                                     ctypes.cast(PA, ctypes.c_void_p), _ptr(lda), ctypes.cast(PB, ctypes.c_void_p),
+                                     _ptr(ldb), ctypes.cast(PX, ctypes.c_void_p), _ptr(ldx), _ptr(status),
+                                     _ptr(bwd), int(max_iter), float(tol_scale), ctypes.byref(info))
+    _check("gesv_batched", rc, None)
+    return xs, status, bwd, info
 
 
 def gesv_batched(As, Bs, spd=None, max_iter=0, tol_scale=0.0):

@@ -102,3 +102,40 @@ def test_batched_graph_replay_is_identical(L):
     assert np.array_equal(st1, st2) and np.array_equal(st2, st3)
     assert torch.equal(x1, dX1)
     assert i1.failed == 0
+
+
+def test_host_entry_point_matches_device_entry(L):
+    """scirs_gpu_dgesv_batched (host pointers, per-system strides) gives the
+    device entry point's solutions bit for bit: the same packed systems reach
+    the same kernels.  Strided views (lda > n) and a repeated call (graph
+    replay on the kept buffers) are covered."""
+    As, Bs, spd = make_batch(60, 8, 200, 11)
+    wide = [np.pad(a, ((0, 0), (0, 5)))[:, :a.shape[0]] for a in As]   # row stride n + 5
+    for a, w in zip(As, wide):
+        assert np.array_equal(a, w) and w.strides[0] == 8 * (a.shape[0] + 5)
+    xs_d, st_d, eta_d, _ = L.gesv_batched(As, Bs, spd)
+    for call in range(2):
+        xs_h, st_h, eta_h, info = L.gesv_batched_host(wide, Bs, spd)
+        assert info.failed == 0 and np.array_equal(st_h, st_d)
+        for i, (a, b) in enumerate(zip(As, Bs)):
+            assert np.array_equal(xs_h[i], xs_d[i]), (call, i)
+            assert bwd(a, xs_h[i], b) <= 8 * a.shape[0] * U64
+        assert info.total_ms >= info.time_ms > 0
+
+
+def test_host_entry_point_rejects_bad_strides(L):
+    a = np.eye(4)
+    with pytest.raises(ValueError):
+        L.gesv_batched_host([a[:, :3]], [np.ones(4)])   # not square
+    import ctypes
+    from scirs_amd import _native as N
+    lib = N.load()
+    n = np.array([4], np.int32); r = np.array([1], np.int32)
+    lda = np.array([3], np.int64); ld1 = np.array([1], np.int64)
+    A = (ctypes.c_void_p * 1)(a.ctypes.data); B = (ctypes.c_void_p * 1)(a.ctypes.data)
+    st = np.zeros(1, np.int32)
+    rc = lib.scirs_gpu_dgesv_batched(1, n.ctypes.data, r.ctypes.data, None, ctypes.cast(A, ctypes.c_void_p),
+                                     lda.ctypes.data, ctypes.cast(B, ctypes.c_void_p), ld1.ctypes.data,
+                                     ctypes.cast(B, ctypes.c_void_p), ld1.ctypes.data, st.ctypes.data, None, 0, 0.0,
+                                     None)
+    assert rc == N.SCIRS_ERR_ARG and "lda=3" in N.last_error()

@@ -290,6 +290,18 @@ def test_mirror_compute_matches_oracle():
             xo = oracle.solve_triangular(t, b, bool(lower), bool(unit))
             assert np.abs(g[1][0].ravel() - xo).max() <= 1e-12 * np.abs(xo).max()
         add(("solve_triangular", "f64", lower, unit, t, b), chk_t)
+        # f32: the mirror hands f32 to scirs_gpu_strsv (no upcast); the oracle's
+        # f32 loop within 8 n 2^-24 kappa
+        t32, b32 = t.astype(np.float32), b.astype(np.float32)
+
+        def chk_t32(g, t32=t32, b32=b32, lower=lower, unit=unit):
+            xo = oracle.solve_triangular(t32, b32, bool(lower), bool(unit))
+            te = t32.astype(np.float64)
+            if unit:
+                np.fill_diagonal(te, 1.0)
+            bound = 8 * 50 * 2.0 ** -24 * np.linalg.cond(te, np.inf)
+            assert np.abs(g[1][0].ravel() - xo).max() <= bound * np.abs(xo).max()
+        add(("solve_triangular", "f32", lower, unit, t32, b32), chk_t32)
 
     a6 = rng.standard_normal((6, 6))
     add(("det", "f64", 0, 0, a6, None),
@@ -430,7 +442,10 @@ def test_compat_solve_general_and_pos():
         kappa = np.linalg.cond(m, np.inf)
         assert np.abs(x - xo).max() <= 8 * n * U64 * kappa * np.abs(xo).max(), i
     x32 = got[2][1][0]
-    assert np.abs(x32 - np.linalg.solve(s, b)).max() <= 1e-5
+    s32, b32 = s.astype(np.float32), b.astype(np.float32)
+    xo32 = oracle.cholesky_solve(s32, b32)   # the composition in f32
+    kappa = np.linalg.cond(s, np.inf)
+    assert np.abs(x32 - xo32).max() <= 8 * n * 2.0 ** -24 * kappa * np.abs(xo32).max()
     assert got[3] == ("ERR", "NonPositiveDefiniteError", not_pd("Cholesky decomposition", n, n))
 
 
