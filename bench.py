@@ -9,7 +9,7 @@ low-precision LU + FP64 refinement to the FP64 backward-error target, or, for
 the comparison line, FP64 LU + substitution.  TFLOP/s uses the 2n^3/3
 convention.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--mode bf16x6|bf16x3|fp16|bf16|fp32]
+    python bench.py [--steps K] [--warmup W] [--n N] [--mode bf16x6|bf16x3|fp16x3|fp16|bf16|fp32]
     python bench.py --workload batched        # 20,000 uneven systems in one launch
 
 Prints one JSON line.  One GPU only: --gpus above 1 exits with a message.
@@ -27,8 +27,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_MFMA_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
 PEAK_HBM_GBS = 8000.0
-MODES = {"bf16": 1, "fp16": 2, "bf16x3": 3, "fp32": 4, "bf16x6": 5}
-MODE_PEAK = {"bf16": "bf16", "fp16": "fp16", "bf16x3": "bf16", "fp32": "f32", "bf16x6": "bf16"}
+MODES = {"bf16": 1, "fp16": 2, "bf16x3": 3, "fp32": 4, "bf16x6": 5, "fp16x3": 6}
+MODE_PEAK = {"bf16": "bf16", "fp16": "fp16", "bf16x3": "bf16", "fp32": "f32", "bf16x6": "bf16", "fp16x3": "fp16"}
 
 
 def parse():

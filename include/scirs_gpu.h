@@ -54,6 +54,8 @@ extern "C" {
 #define SCIRS_MODE_FP32 4           /* FP32 arithmetic                          */
 #define SCIRS_MODE_BF16X6 5         /* split BF16 (hi+mid+lo), six products:
                                        FP32-accurate on BF16 matrix cores       */
+#define SCIRS_MODE_FP16X3 6         /* FP16 with power-of-two scaling, split
+                                       (hi+lo), three products on FP16 cores    */
 
 typedef struct scirs_solve_info {
     int32_t status;          /* same as the return value                          */

@@ -22,7 +22,7 @@ SCIRS_ERR_NOT_CONVERGED = -5
 SCIRS_ERR_UNSUPPORTED = -6
 
 PATH_FP64_LU, PATH_FP64_CHOL, PATH_MIXED_IR, PATH_MIXED_GMRES, PATH_FP64_FALLBACK, PATH_FP32_LU, PATH_FP32_CHOL = range(7)
-MODE_AUTO, MODE_BF16, MODE_FP16, MODE_BF16X3, MODE_FP32, MODE_BF16X6 = range(6)
+MODE_AUTO, MODE_BF16, MODE_FP16, MODE_BF16X3, MODE_FP32, MODE_BF16X6, MODE_FP16X3 = range(7)
 
 
 class SolveInfo(ctypes.Structure):

@@ -36,6 +36,7 @@ struct Context {
     void *scratch_ir = nullptr; size_t scratch_ir_bytes = 0;
     void *scratch_ir2 = nullptr; size_t scratch_ir2_bytes = 0;
     void *scratch_t = nullptr; size_t scratch_t_bytes = 0;   // L^T of a Cholesky solve
+    void *scratch_lu64 = nullptr; size_t scratch_lu64_bytes = 0;   // FP64 copy of GMRES-IR's low-precision factors
     void *xtri_mem = nullptr; size_t xtri_bytes = 0;
     void *colexp_mem = nullptr; size_t colexp_bytes = 0;
     double *noalloc_scratch = nullptr;   // device address of the fixed scratch array (set at init)

@@ -10,6 +10,9 @@
 //   SCIRS_LOWP_BF16X3  x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
 //                      C -= Ahi Bhi + Ahi Blo + Alo Bhi  (about 16 significant
 //                      bits per operand, three matrix-core products)
+//   SCIRS_LOWP_FP16X3  the FP16 scaling above, then x = hi + lo with hi = f16(x),
+//                      lo = f16(x - hi); C -= Ahi Bhi + Ahi Blo + Alo Bhi (about
+//                      22 significant bits per operand on FP16 matrix cores)
 //   SCIRS_LOWP_BF16X6  x = hi + mid + lo (three BF16 terms, ~24 bits: the FP32
 //                      significand); the six products whose order is above
 //                      2^-24 (hh, hm, mh, hl, lh, mm), so the update is as
@@ -68,7 +71,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
     const int row0 = tm * BM, col0 = tn * BN;
     if (g.lower && col0 > row0 + BM - 1) return;
 
-    constexpr int NSPLIT = (MODE == SCIRS_LOWP_BF16X6) ? 3 : (MODE == SCIRS_LOWP_BF16X3) ? 2 : 1;
+    constexpr bool F16 = MODE == SCIRS_LOWP_FP16 || MODE == SCIRS_LOWP_FP16X3;   // FP16 cores, scaled operands
+    constexpr int NSPLIT =
+        (MODE == SCIRS_LOWP_BF16X6) ? 3 : (MODE == SCIRS_LOWP_BF16X3 || MODE == SCIRS_LOWP_FP16X3) ? 2 : 1;
     constexpr int NBUF = NSPLIT == 3 ? 1 : 2;
     __shared__ __align__(16) u16 As[NBUF][NSPLIT][BM * PITCH];
     __shared__ __align__(16) u16 Bs[NBUF][NSPLIT][BN * PITCH];
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
     // B slab (row-major K x N): thread t loads k = t%32, cols (t/32)*16 .. +16
     const int b_k = t & 31, b_c = (t >> 5) * 16;
     float ra[16], rb[16];
-    const float ascale = (MODE == SCIRS_LOWP_FP16) ? __builtin_ldexpf(1.0f, g.a_shift) : 1.0f;
+    const float ascale = F16 ? __builtin_ldexpf(1.0f, g.a_shift) : 1.0f;
 
     // 16 consecutive floats of one row: four 16-byte loads when the row segment
     // is in range and aligned (VEC: every leading dimension a multiple of 4)
@@ -109,8 +114,12 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
         }
     };
     auto cvt = [&](float x, u16 &hi, u16 &lo, u16 &mid) {
-        if (MODE == SCIRS_LOWP_FP16) {
+        if (F16) {
             hi = f32_to_f16(x);
+            if (MODE == SCIRS_LOWP_FP16X3) {
+                _Float16 h = *reinterpret_cast<const _Float16 *>(&hi);
+                lo = f32_to_f16(x - (float)h);
+            }
         } else {
             hi = f32_to_bf16_rne(x);
             if (MODE == SCIRS_LOWP_BF16X3) lo = f32_to_bf16_rne(x - bf16_to_f32(hi));
@@ -137,7 +146,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 float x = rb[i];
-                if (MODE == SCIRS_LOWP_FP16 && g.colexp) {
+                if (F16 && g.colexp) {
                     const int gc = col0 + b_c + i;
                     x = __builtin_ldexpf(x, gc < g.N ? -g.colexp[gc] : 0);
                 }
@@ -147,7 +156,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
             for (int i = 0; i < 16; ++i) put(Bs[buf], (b_c + i) * PITCH + b_k, hi[i], lo[i], mid[i]);
         } else {
             float sc = 1.0f;
-            if (MODE == SCIRS_LOWP_FP16 && g.colexp) {
+            if (F16 && g.colexp) {
                 const int gc = col0 + a_r;
                 sc = __builtin_ldexpf(1.0f, gc < g.N ? -g.colexp[gc] : 0);
             }
@@ -222,9 +231,16 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 // operands swapped: op(B) fragment as the left factor, A as the right
-                if (MODE == SCIRS_LOWP_FP16) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                        __builtin_bit_cast(f16x8, bh[j]), __builtin_bit_cast(f16x8, ah[i]), acc[i][j], 0, 0, 0);
+                if (F16) {
+                    auto hma = [&](const u16x8 &x, const u16x8 &y, f4 &c) {
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, x),
+                                                                   __builtin_bit_cast(f16x8, y), c, 0, 0, 0);
+                    };
+                    if (MODE == SCIRS_LOWP_FP16X3) {
+                        hma(bl[j], ah[i], acc[i][j]);
+                        hma(bh[j], al[i], acc[i][j]);
+                    }
+                    hma(bh[j], ah[i], acc[i][j]);
                 } else if (MODE == SCIRS_LOWP_BF16X6) {
                     // smallest terms first
                     mma(bm[j], am[i], acc[i][j]);
@@ -259,7 +275,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void lowp_gemm_kernel(LowpArgs<MODE> g
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float sc = 1.0f;
-                if (MODE == SCIRS_LOWP_FP16)
+                if (F16)
                     sc = __builtin_ldexpf(1.0f, ((g.colexp && c + q < g.N) ? g.colexp[c + q] : 0) - g.a_shift);
                 out[q] = cpre[i][j][q] - acc[i][j][q] * sc;
             }
@@ -323,6 +339,9 @@ int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const 
     case SCIRS_LOWP_FP16:
         // |l_ij| <= 1 under partial pivoting: 2^14 keeps A in range and lifts small multipliers
         return launch_lowp<SCIRS_LOWP_FP16>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 14, stream);
+    case SCIRS_LOWP_FP16X3:
+        return launch_lowp<SCIRS_LOWP_FP16X3>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, nullptr, 14,
+                                              stream);
     default:
         scirs_set_error("unknown low-precision mode %d", mode);
         return SCIRS_ERR_ARG;
@@ -354,9 +373,11 @@ int scirs_launch_colexp(int K, int N, const float *B, long ldb, int *colexp, con
 }
 
 int scirs_launch_sgemm_fp16_scaled(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
-                                   float *C, long ldc, const int *colexp, int a_shift, const int *info,
+                                   float *C, long ldc, const int *colexp, int a_shift, int mode, const int *info,
                                    hipStream_t stream)
 {
     if (M <= 0 || N <= 0 || K <= 0) return SCIRS_OK;
+    if (mode == SCIRS_LOWP_FP16X3)
+        return launch_lowp<SCIRS_LOWP_FP16X3>(M, N, K, A, lda, B, ldb, 0, C, ldc, 0, info, colexp, a_shift, stream);
     return launch_lowp<SCIRS_LOWP_FP16>(M, N, K, A, lda, B, ldb, 0, C, ldc, 0, info, colexp, a_shift, stream);
 }

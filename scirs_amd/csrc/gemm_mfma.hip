@@ -426,13 +426,14 @@ template <>
 int scirs_trailing_update<float>(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
                                  float *C, long ldc, const ScirsWork &w, hipStream_t stream)
 {
-    if (w.lowp_mode == SCIRS_LOWP_FP16 && w.colexp && N <= w.colexp_half) {
+    const bool f16 = w.lowp_mode == SCIRS_LOWP_FP16 || w.lowp_mode == SCIRS_LOWP_FP16X3;
+    if (f16 && w.colexp && N <= w.colexp_half) {
         // per-column scaling of U12; the side stream's update has its own region
         int *ce = w.colexp + (stream == w.side_stream ? w.colexp_half : 0);
         int rc = scirs_launch_colexp(K, N, B, ldb, ce, w.info, stream);
         if (rc) return rc;
         // |l_ij| <= 1 under partial pivoting: 2^14 lifts small multipliers
-        return scirs_launch_sgemm_fp16_scaled(M, N, K, A, lda, B, ldb, C, ldc, ce, 14, w.info, stream);
+        return scirs_launch_sgemm_fp16_scaled(M, N, K, A, lda, B, ldb, C, ldc, ce, 14, w.lowp_mode, w.info, stream);
     }
     return scirs_launch_sgemm_sub(M, N, K, A, lda, B, ldb, 0, C, ldc, 0, w.lowp_mode, w.info, stream);
 }

@@ -129,6 +129,111 @@ __global__ void zero_ld_kernel(int rows, int cols, double *p, long ld)
     if (i < (long)rows * cols) p[(i / cols) * ld + i % cols] = 0.0;
 }
 
+// ---- block GMRES: nc right-hand sides side by side.  A block of vectors is
+// n x nc, row-major (entry (q, c) at q * nc + c), so the triangular solves and
+// the product with A read the matrix once for all nc columns.
+constexpr int GM_NC = 8;   // columns per block GMRES pass
+
+// W = A Z for nc <= GM_NC columns; one wave per 8 rows of A
+__global__ __launch_bounds__(NT) void gemv_block_kernel(int n, int nc, const double *A, long lda, const double *Z,
+                                                        double *W)
+{
+    const int lane = threadIdx.x & 63;
+    const int r0 = ((blockIdx.x * NT + threadIdx.x) >> 6) * 8;
+    if (r0 >= n) return;
+    const int nr = min(8, n - r0);
+    double acc[8][GM_NC];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < GM_NC; ++c) acc[a][c] = 0.0;
+    for (int k = lane; k < n; k += 64) {
+        double z[GM_NC];
+#pragma unroll
+        for (int c = 0; c < GM_NC; ++c) z[c] = c < nc ? Z[(long)k * nc + c] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const double av = a < nr ? A[(long)(r0 + a) * lda + k] : 0.0;
+#pragma unroll
+            for (int c = 0; c < GM_NC; ++c) acc[a][c] = fma(av, z[c], acc[a][c]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < GM_NC; ++c) acc[a][c] = wave_sum(acc[a][c]);
+    if (lane == 0)
+        for (int a = 0; a < nr; ++a)
+            for (int c = 0; c < nc; ++c) W[(long)(r0 + a) * nc + c] = acc[a][c];
+}
+
+// h[i * nc + c] = <V_i(:, c), W(:, c)> for i < k (V_i: block i of n x nc)
+__global__ __launch_bounds__(NT) void multidot_block_kernel(int n, int nc, const double *V, const double *W, double *h)
+{
+    const int i = blockIdx.x;
+    const double *v = V + (long)i * n * nc;
+    double s[GM_NC];
+#pragma unroll
+    for (int c = 0; c < GM_NC; ++c) s[c] = 0.0;
+    for (int q = threadIdx.x; q < n; q += NT)
+#pragma unroll
+        for (int c = 0; c < GM_NC; ++c)
+            if (c < nc) s[c] = fma(v[(long)q * nc + c], W[(long)q * nc + c], s[c]);
+    __shared__ double red[NT / 64][GM_NC];
+#pragma unroll
+    for (int c = 0; c < GM_NC; ++c) {
+        const double w = wave_sum(s[c]);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < nc) {
+        double r = 0;
+        for (int q = 0; q < NT / 64; ++q) r += red[q][threadIdx.x];
+        h[(long)i * nc + threadIdx.x] = r;
+    }
+}
+
+// W(:, c) -= sum_{i < k} h[i * nc + c] V_i(:, c)
+__global__ void mgs_block_kernel(int n, int nc, int k, const double *V, const double *h, double *W)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * nc) return;
+    const int c = (int)(e % nc);
+    double s = W[e];
+    for (int i = 0; i < k; ++i) s -= h[(long)i * nc + c] * V[(long)i * n * nc + e];
+    W[e] = s;
+}
+
+// dst(:, c) = src(:, c) / sqrt(ss[c])  (0 where ss[c] is 0: a breakdown column stays 0)
+__global__ void normalize_block_kernel(int n, int nc, const double *src, const double *ss, double *dst)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * nc) return;
+    const double v = ss[e % nc];
+    dst[e] = v > 0 ? src[e] / sqrt(v) : 0.0;
+}
+
+// V0(:, c) = R(:, cols[c])
+__global__ void gather_cols_kernel(int n, int nc, const double *R, long ldr, const int *cols, double *V0)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * nc) return;
+    const long q = e / nc;
+    V0[e] = R[q * ldr + cols[e % nc]];
+}
+
+// X(:, cols[c]) += sum_{j < kmax} Y[j * nc + c] Z_j(:, c)
+__global__ void combine_block_kernel(int n, int nc, int kmax, const double *Z, const double *Y, double *X, long ldx,
+                                     const int *cols)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * nc) return;
+    const int c = (int)(e % nc);
+    double s = 0.0;
+    for (int j = 0; j < kmax; ++j) s += Y[(long)j * nc + c] * Z[(long)j * n * nc + e];
+    X[(e / nc) * ldx + cols[c]] += s;
+}
+
 inline unsigned nb(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 // power of two closest above |v| (1 for v == 0), so scaling is exact
@@ -165,20 +270,6 @@ int lu_solve_f32(IrState &st, float *d32)
                                        st.dmt_u);
 }
 
-// apply z = (LU)^-1 P v in FP32 for one vector v (FP64), result FP64; scale keeps FP32 in range
-int precond_apply(IrState &st, const double *v, double vnorm, double *z)
-{
-    Context &c = ctx();
-    const double sc = pow2_above(vnorm);
-    hipLaunchKernelGGL(permute_to_f32_kernel, dim3(nb(st.n)), dim3(256), 0, st.s, st.n, st.perm, v, 1L, 0, 1.0 / sc,
-                       st.d32);
-    int rc = lu_solve_f32(st, st.d32);
-    if (rc) return rc;
-    hipLaunchKernelGGL(f32_to_f64_scaled_kernel, dim3(nb(st.n)), dim3(256), 0, st.s, st.n, st.d32, sc, z);
-    SCIRS_HIP_CHECK(hipGetLastError());
-    return SCIRS_OK;
-}
-
 // eta_j for every column; returns the worst, fills per-column values
 int backward_errors(IrState &st, std::vector<double> &eta, std::vector<double> &rnorm)
 {
@@ -213,100 +304,161 @@ int ir_step(IrState &st, int j, double rn)
     return SCIRS_OK;
 }
 
-// Restarted right-preconditioned GMRES on A d = R_j (FP64 Arnoldi), X_j += d.
-// Returns the number of inner iterations used in *iters.
-// Restarted GMRES on A d = r_j, right-preconditioned by the low-precision LU.
-// *stalled is set when a whole restart cycle fails to halve the true residual:
-// the preconditioner is then too weak for this system, and the caller stops
-// spending iterations on it.
-int gmres_correction(IrState &st, int j, int restart, int max_inner, double rtol, double *V, double *Z, double *w,
-                     double *hdev, int *iters, bool *stalled)
+// Restarted GMRES on A D = R(:, cols) for nc <= GM_NC columns side by side,
+// right-preconditioned by the low-precision factors applied in FP64:
+// z = U^-1 L^-1 P v with the factors upcast once to FP64 (LU64), so the
+// preconditioner adds no rounding of its own above 2^-53 (Amestoy, Buttari,
+// Higham, L'Excellent, Mary, Vieuble, SIMAX 2024: the precision of the
+// preconditioner's application bounds what GMRES-IR can resolve).  Each
+// column keeps its own Hessenberg matrix and Givens rotations (the small
+// least-squares update of scirs2-linalg/src/solvers/iterative.rs:604-664); the
+// device work of one Arnoldi step — preconditioner, product with A, classical
+// Gram-Schmidt twice, normalisation — is queued for all columns at once and
+// followed by one host read (the new Hessenberg columns).  A column stops
+// when its Arnoldi residual estimate reaches rtol times its starting residual,
+// when its budget is spent, or when the contraction of its true residual over
+// the last restart cycle, extrapolated, needs more cycles than its remaining
+// budget holds (stalled[c] is then set: the caller moves on at once instead of
+// spending the budget on a preconditioner too weak for this system).
+struct GmresWork {
+    double *V, *Z, *W, *h, *Y;   // (restart+1) blocks of n x nc each for V and Z; W n x nc
+    int *cols;                   // device copy of the column indices
+    double *LU64;                // FP64 copy of the factors
+};
+
+int precond_block(IrState &st, const GmresWork &gw, int nc, const double *Vk, double *Zk)
 {
-    *stalled = false;
-    const int n = st.n;
-    // r0 = R_j (copy of column j)
-    std::vector<double> H((size_t)(restart + 1) * restart, 0.0), cs(restart), sn(restart), g(restart + 1);
     int rc;
-    // V_0 <- R[:, j]
-    if ((rc = scirs_d2d_copy(n, 1, st.R + j, st.nrhs, V, 1, st.s))) return rc;
-    double beta;
-    hipLaunchKernelGGL(multidot_kernel, dim3(1), dim3(NT), 0, st.s, n, 1, V, V, hdev);
-    SCIRS_HIP_CHECK(hipMemcpyAsync(&beta, hdev, sizeof(double), hipMemcpyDeviceToHost, st.s));
-    SCIRS_HIP_CHECK(hipStreamSynchronize(st.s));
-    beta = std::sqrt(beta);
-    *iters = 0;
-    if (!(beta > 0)) return SCIRS_OK;
-    const double target = rtol * beta;
-    int total = 0;
-    while (total < max_inner) {
-        hipLaunchKernelGGL(scale_copy_kernel, dim3(nb(n)), dim3(256), 0, st.s, n, V, 1.0 / beta, V);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int k = 0;
-        double resid = beta;
-        for (; k < restart && total < max_inner; ++k, ++total) {
-            double *vk = V + (long)k * n, *vk1 = V + (long)(k + 1) * n, *zk = Z + (long)k * n;
-            // z_k = M^-1 v_k ;  w = A z_k
-            if ((rc = precond_apply(st, vk, 1.0, zk))) return rc;
-            hipLaunchKernelGGL(gemv_kernel, dim3(nb((long)((n + 3) / 4) * 64, NT)), dim3(NT), 0, st.s, n, st.A, st.lda,
-                               zk, w);
-            // classical Gram-Schmidt, twice
-            std::vector<double> hcol(k + 2, 0.0), tmp(k + 1);
-            for (int pass = 0; pass < 2; ++pass) {
-                hipLaunchKernelGGL(multidot_kernel, dim3(k + 1), dim3(NT), 0, st.s, n, k + 1, V, w, hdev);
-                hipLaunchKernelGGL(mgs_update_kernel, dim3(nb(n)), dim3(256), 0, st.s, n, k + 1, V, hdev, w);
-                SCIRS_HIP_CHECK(hipMemcpyAsync(tmp.data(), hdev, (k + 1) * sizeof(double), hipMemcpyDeviceToHost,
-                                               st.s));
-                SCIRS_HIP_CHECK(hipStreamSynchronize(st.s));
-                for (int i = 0; i <= k; ++i) hcol[i] += tmp[i];
-            }
-            double hn;
-            hipLaunchKernelGGL(multidot_kernel, dim3(1), dim3(NT), 0, st.s, n, 1, w, w, hdev);
-            SCIRS_HIP_CHECK(hipMemcpyAsync(&hn, hdev, sizeof(double), hipMemcpyDeviceToHost, st.s));
-            SCIRS_HIP_CHECK(hipStreamSynchronize(st.s));
-            hn = std::sqrt(hn);
-            hcol[k + 1] = hn;
-            if (hn > 0) hipLaunchKernelGGL(scale_copy_kernel, dim3(nb(n)), dim3(256), 0, st.s, n, w, 1.0 / hn, vk1);
-            // Givens rotations (as solvers/iterative.rs:604-664 solves the small LS problem)
-            for (int i = 0; i < k; ++i) {
-                const double a = hcol[i], b = hcol[i + 1];
-                hcol[i] = cs[i] * a + sn[i] * b;
-                hcol[i + 1] = -sn[i] * a + cs[i] * b;
-            }
-            const double a = hcol[k], b = hcol[k + 1];
-            const double r = std::hypot(a, b);
-            cs[k] = r > 0 ? a / r : 1.0;
-            sn[k] = r > 0 ? b / r : 0.0;
-            hcol[k] = r;
-            hcol[k + 1] = 0.0;
-            g[k + 1] = -sn[k] * g[k];
-            g[k] = cs[k] * g[k];
-            for (int i = 0; i <= k; ++i) H[(size_t)i * restart + k] = hcol[i];
-            resid = std::fabs(g[k + 1]);
-            if (resid <= target || !(hn > 0)) { ++k; ++total; break; }
-        }
-        // back substitution for y, then X_j += Z y
-        std::vector<double> y(k, 0.0);
-        for (int i = k - 1; i >= 0; --i) {
-            double sacc = g[i];
-            for (int q = i + 1; q < k; ++q) sacc -= H[(size_t)i * restart + q] * y[q];
-            y[i] = sacc / H[(size_t)i * restart + i];
-        }
-        SCIRS_HIP_CHECK(hipMemcpyAsync(hdev, y.data(), k * sizeof(double), hipMemcpyHostToDevice, st.s));
-        hipLaunchKernelGGL(combine_kernel, dim3(nb(n)), dim3(256), 0, st.s, n, k, Z, hdev, st.X, st.ldx, j);
-        SCIRS_HIP_CHECK(hipGetLastError());
-        if (resid <= target || total >= max_inner) break;
-        // restart from the new residual of this column
-        if ((rc = scirs_residual(n, 1, st.A, st.lda, st.X + j, st.ldx, st.B + j, st.ldb, V, 1, st.s))) return rc;
-        hipLaunchKernelGGL(multidot_kernel, dim3(1), dim3(NT), 0, st.s, n, 1, V, V, hdev);
-        const double beta_prev = beta;
-        SCIRS_HIP_CHECK(hipMemcpyAsync(&beta, hdev, sizeof(double), hipMemcpyDeviceToHost, st.s));
+    if ((rc = scirs_permute_rows<double>(st.n, nc, st.perm, Vk, nc, Zk, nc, st.s))) return rc;
+    return scirs::lu_substitute<double>(st.n, nc, gw.LU64, st.n, Zk, nc, st.s);
+}
+
+int gmres_block(IrState &st, const std::vector<int> &cols, int restart, const std::vector<int> &budget,
+                const std::vector<double> &rtol, const GmresWork &gw, std::vector<int> &used, std::vector<char> &stalled)
+{
+    const int n = st.n, nc = (int)cols.size();
+    const long blk = (long)n * nc;
+    const unsigned nbe = nb(blk);
+    int rc;
+    used.assign(nc, 0);
+    stalled.assign(nc, 0);
+    std::vector<char> active(nc, 1);
+    SCIRS_HIP_CHECK(hipMemcpyAsync(gw.cols, cols.data(), nc * sizeof(int), hipMemcpyHostToDevice, st.s));
+    std::vector<double> H((size_t)nc * (restart + 1) * restart), cs((size_t)nc * restart), sn((size_t)nc * restart),
+        g((size_t)nc * (restart + 1)), hbuf((size_t)(2 * (restart + 1) + 1) * nc), beta(nc), beta_prev(nc, INFINITY),
+        target(nc, 0.0), Y((size_t)restart * nc);
+    std::vector<int> kc(nc);
+    bool first = true;
+    while (true) {
+        // V_0 = R(:, cols) / ||R(:, cols)||, one host read for the norms
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nbe), dim3(256), 0, st.s, n, nc, st.R, (long)st.nrhs, gw.cols, gw.V);
+        hipLaunchKernelGGL(multidot_block_kernel, dim3(1), dim3(NT), 0, st.s, n, nc, gw.V, gw.V, gw.h);
+        SCIRS_HIP_CHECK(hipMemcpyAsync(hbuf.data(), gw.h, nc * sizeof(double), hipMemcpyDeviceToHost, st.s));
         SCIRS_HIP_CHECK(hipStreamSynchronize(st.s));
-        beta = std::sqrt(beta);
-        if (!(beta > 0)) break;
-        if (!(beta < 0.5 * beta_prev)) { *stalled = true; break; }
+        bool any = false;
+        for (int c = 0; c < nc; ++c) {
+            beta[c] = std::sqrt(hbuf[c]);
+            if (!active[c]) continue;
+            if (!(beta[c] > 0)) { active[c] = 0; continue; }
+            if (first) target[c] = rtol[c] * beta[c];
+            else if (beta[c] <= target[c]) { active[c] = 0; continue; }
+            else {
+                // the last cycle's contraction, extrapolated: give up at once when
+                // the cycles it predicts do not fit the remaining budget
+                const double ratio = beta[c] / beta_prev[c];
+                const double need = ratio < 1.0 ? std::log(target[c] / beta[c]) / std::log(ratio) : INFINITY;
+                if (!(need * restart <= budget[c] - used[c])) { stalled[c] = 1; active[c] = 0; continue; }
+            }
+            if (used[c] >= budget[c]) { active[c] = 0; continue; }
+            beta_prev[c] = beta[c];
+            any = true;
+        }
+        first = false;
+        if (!any) break;
+        hipLaunchKernelGGL(normalize_block_kernel, dim3(nbe), dim3(256), 0, st.s, n, nc, gw.V, gw.h, gw.V);
+        std::vector<char> run(active);   // columns still iterating in this cycle
+        for (int c = 0; c < nc; ++c) {
+            std::fill(g.begin() + (size_t)c * (restart + 1), g.begin() + (size_t)(c + 1) * (restart + 1), 0.0);
+            g[(size_t)c * (restart + 1)] = beta[c];
+            kc[c] = 0;
+        }
+        int k = 0;
+        for (; k < restart; ++k) {
+            bool anyrun = false;
+            for (int c = 0; c < nc; ++c) anyrun = anyrun || run[c];
+            if (!anyrun) break;
+            const double *vk = gw.V + (long)k * blk;
+            double *vk1 = gw.V + (long)(k + 1) * blk, *zk = gw.Z + (long)k * blk;
+            if ((rc = precond_block(st, gw, nc, vk, zk))) return rc;
+            hipLaunchKernelGGL(gemv_block_kernel, dim3(nb((long)((n + 7) / 8) * 64, NT)), dim3(NT), 0, st.s, n, nc,
+                               st.A, st.lda, zk, gw.W);
+            // classical Gram-Schmidt, twice; h rows 0..k of pass 1 and 2, then ||w||^2
+            double *h1 = gw.h, *h2 = gw.h + (long)(restart + 1) * nc, *hw = gw.h + 2L * (restart + 1) * nc;
+            hipLaunchKernelGGL(multidot_block_kernel, dim3(k + 1), dim3(NT), 0, st.s, n, nc, gw.V, gw.W, h1);
+            hipLaunchKernelGGL(mgs_block_kernel, dim3(nbe), dim3(256), 0, st.s, n, nc, k + 1, gw.V, h1, gw.W);
+            hipLaunchKernelGGL(multidot_block_kernel, dim3(k + 1), dim3(NT), 0, st.s, n, nc, gw.V, gw.W, h2);
+            hipLaunchKernelGGL(mgs_block_kernel, dim3(nbe), dim3(256), 0, st.s, n, nc, k + 1, gw.V, h2, gw.W);
+            hipLaunchKernelGGL(multidot_block_kernel, dim3(1), dim3(NT), 0, st.s, n, nc, gw.W, gw.W, hw);
+            hipLaunchKernelGGL(normalize_block_kernel, dim3(nbe), dim3(256), 0, st.s, n, nc, gw.W, hw, vk1);
+            SCIRS_HIP_CHECK(hipGetLastError());
+            SCIRS_HIP_CHECK(hipMemcpyAsync(hbuf.data(), gw.h, (size_t)(2 * (restart + 1) + 1) * nc * sizeof(double),
+                                           hipMemcpyDeviceToHost, st.s));
+            SCIRS_HIP_CHECK(hipStreamSynchronize(st.s));
+            for (int c = 0; c < nc; ++c) {
+                if (!run[c]) continue;
+                double *Hc = &H[(size_t)c * (restart + 1) * restart];
+                double *csc = &cs[(size_t)c * restart], *snc = &sn[(size_t)c * restart];
+                double *gc = &g[(size_t)c * (restart + 1)];
+                std::vector<double> hcol(k + 2);
+                for (int i = 0; i <= k; ++i)
+                    hcol[i] = hbuf[(size_t)i * nc + c] + hbuf[(size_t)(restart + 1) * nc + (size_t)i * nc + c];
+                const double hn = std::sqrt(std::max(0.0, hbuf[(size_t)2 * (restart + 1) * nc + c]));
+                hcol[k + 1] = hn;
+                for (int i = 0; i < k; ++i) {
+                    const double a = hcol[i], b = hcol[i + 1];
+                    hcol[i] = csc[i] * a + snc[i] * b;
+                    hcol[i + 1] = -snc[i] * a + csc[i] * b;
+                }
+                const double a = hcol[k], b = hcol[k + 1];
+                const double r = std::hypot(a, b);
+                csc[k] = r > 0 ? a / r : 1.0;
+                snc[k] = r > 0 ? b / r : 0.0;
+                hcol[k] = r;
+                gc[k + 1] = -snc[k] * gc[k];
+                gc[k] = csc[k] * gc[k];
+                for (int i = 0; i <= k; ++i) Hc[(size_t)i * restart + k] = hcol[i];
+                kc[c] = k + 1;
+                ++used[c];
+                if (std::fabs(gc[k + 1]) <= target[c] || !(hn > 0) || used[c] >= budget[c] || !(r > 0))
+                    run[c] = 0;
+            }
+        }
+        // y_c by back substitution on each column's Hessenberg matrix, then X(:, cols) += Z y
+        int kmax = 0;
+        std::fill(Y.begin(), Y.end(), 0.0);
+        for (int c = 0; c < nc; ++c) {
+            if (!active[c] || kc[c] == 0) continue;
+            const double *Hc = &H[(size_t)c * (restart + 1) * restart];
+            const double *gc = &g[(size_t)c * (restart + 1)];
+            int kk = kc[c];
+            while (kk > 0 && !(Hc[(size_t)(kk - 1) * restart + kk - 1] != 0.0)) --kk;   // drop a breakdown step
+            for (int i = kk - 1; i >= 0; --i) {
+                double sacc = gc[i];
+                for (int q = i + 1; q < kk; ++q) sacc -= Hc[(size_t)i * restart + q] * Y[(size_t)q * nc + c];
+                Y[(size_t)i * nc + c] = sacc / Hc[(size_t)i * restart + i];
+            }
+            kmax = std::max(kmax, kk);
+        }
+        if (kmax > 0) {
+            SCIRS_HIP_CHECK(hipMemcpyAsync(gw.Y, Y.data(), (size_t)kmax * nc * sizeof(double), hipMemcpyHostToDevice,
+                                           st.s));
+            hipLaunchKernelGGL(combine_block_kernel, dim3(nbe), dim3(256), 0, st.s, n, nc, kmax, gw.Z, gw.Y, st.X,
+                               st.ldx, gw.cols);
+            SCIRS_HIP_CHECK(hipGetLastError());
+        }
+        // true residuals of every column (one product with A), then the next cycle
+        if ((rc = scirs_residual(n, st.nrhs, st.A, st.lda, st.X, st.ldx, st.B, st.ldb, st.R, st.nrhs, st.s))) return rc;
     }
-    *iters = total;
     return SCIRS_OK;
 }
 
@@ -315,6 +467,7 @@ int lowp_mode_of(int mode)
     switch (mode) {
     case SCIRS_MODE_BF16: return SCIRS_LOWP_BF16;
     case SCIRS_MODE_FP16: return SCIRS_LOWP_FP16;
+    case SCIRS_MODE_FP16X3: return SCIRS_LOWP_FP16X3;
     case SCIRS_MODE_BF16X3: return SCIRS_LOWP_BF16X3;
     case SCIRS_MODE_FP32: return SCIRS_LOWP_FP32;
     case SCIRS_MODE_BF16X6: return SCIRS_LOWP_BF16X6;
@@ -350,9 +503,12 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     const long ld32 = ((long)n + 63) / 64 * 64;
     const size_t a32 = (size_t)n * ld32 * sizeof(float);
     const size_t dinv_elems = (size_t)((n + 63) / 64) * 64 * 64;
-    const size_t other = ((size_t)n * nrhs + (size_t)n + 256 + (size_t)n + 2 * (size_t)(restart + 1) * n + n + restart + 8) *
-                         sizeof(double) + (size_t)n * sizeof(float) + (size_t)n * sizeof(int64_t) +
-                         4 * dinv_elems * sizeof(float) + 4096;
+    const int gnc = std::min(nrhs, GM_NC);   // columns per block GMRES pass
+    const size_t other = ((size_t)n * nrhs + (size_t)n + 256 + (size_t)n + 2 * (size_t)(restart + 1) * n * gnc +
+                          (size_t)n * gnc + (size_t)(2 * (restart + 1) + 1) * gnc + (size_t)restart * gnc + 64) *
+                             sizeof(double) +
+                         (size_t)n * sizeof(float) + (size_t)n * sizeof(int64_t) + 4 * dinv_elems * sizeof(float) +
+                         (size_t)gnc * sizeof(int) + 8192;
     if ((rc = c.ensure(&c.scratch_ir, &c.scratch_ir_bytes, a32))) return rc;
     if ((rc = c.ensure(&c.scratch_ir2, &c.scratch_ir2_bytes, other))) return rc;
     IrState st;
@@ -366,10 +522,14 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     st.rowsum = (double *)take((size_t)n * sizeof(double));
     st.d32 = (float *)take((size_t)n * sizeof(float));
     st.perm = (int64_t *)take((size_t)n * sizeof(int64_t));
-    double *V = (double *)take((size_t)(restart + 1) * n * sizeof(double));
-    double *Z = (double *)take((size_t)(restart + 1) * n * sizeof(double));
-    double *w = (double *)take((size_t)n * sizeof(double));
-    double *hdev = (double *)take((size_t)(restart + 8) * sizeof(double));
+    GmresWork gw;
+    gw.V = (double *)take((size_t)(restart + 1) * n * gnc * sizeof(double));
+    gw.Z = (double *)take((size_t)(restart + 1) * n * gnc * sizeof(double));
+    gw.W = (double *)take((size_t)n * gnc * sizeof(double));
+    gw.h = (double *)take((size_t)(2 * (restart + 1) + 1) * gnc * sizeof(double));
+    gw.Y = (double *)take((size_t)restart * gnc * sizeof(double));
+    gw.cols = (int *)take((size_t)gnc * sizeof(int));
+    gw.LU64 = nullptr;
     float *dinv_l = (float *)take(dinv_elems * sizeof(float));
     float *dinv_u = (float *)take(dinv_elems * sizeof(float));
     float *dmt_l = (float *)take(dinv_elems * sizeof(float));
@@ -387,7 +547,7 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     if ((rc = scirs_colmax_d(n, nrhs, dB, ldb, st.norms + 128, s))) return rc;
     // 2. low-precision LU
     if ((rc = c.reset_flags(s))) return rc;
-    if (lowp == SCIRS_LOWP_FP16) {
+    if (lowp == SCIRS_LOWP_FP16 || lowp == SCIRS_LOWP_FP16X3) {
         if ((rc = c.ensure(&c.colexp_mem, &c.colexp_bytes, 2 * (size_t)n * sizeof(int)))) return rc;
         c.work.colexp = (int *)c.colexp_mem;
         c.work.colexp_half = n;
@@ -441,24 +601,43 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
             for (int j = 0; j < nrhs; ++j) budget_left = budget_left || (eta[j] > tol && gm_col[j] < max_gmres);
             if (it == max_iter || !budget_left) break;
             ++ir_iters;
+            std::vector<int> gcols, gbud;
+            std::vector<double> grtol;
             for (int j = 0; j < nrhs; ++j) {
                 if (eta[j] <= tol) continue;
                 // a step that does not halve eta marks the column for GMRES-IR
                 if (it > 0 && !(eta[j] < 0.5 * prev[j])) use_gmres[j] = 1;
                 if (use_gmres[j]) {
-                    int used = 0;
-                    path = SCIRS_PATH_MIXED_GMRES;
-                    const double rtol = std::max(1e-12, std::min(0.5, tol * 16 / std::max(eta[j], 1e-300)));
                     if (gm_col[j] >= max_gmres) continue;
-                    bool stalled = false;
-                    if ((rc = gmres_correction(st, j, restart, max_gmres - gm_col[j], std::max(rtol, 1e-10), V, Z, w,
-                                               hdev, &used, &stalled)))
-                        return rc;
-                    gm_iters += used;
-                    gm_col[j] += used;
-                    if (stalled) gm_col[j] = max_gmres;   // no further GMRES for this column
+                    gcols.push_back(j);
+                    gbud.push_back(max_gmres - gm_col[j]);
+                    grtol.push_back(std::max(1e-10, std::min(0.5, tol * 16 / std::max(eta[j], 1e-300))));
                 } else {
                     if ((rc = ir_step(st, j, rn[j]))) return rc;
+                }
+            }
+            if (!gcols.empty()) {
+                path = SCIRS_PATH_MIXED_GMRES;
+                if (!gw.LU64) {
+                    // the factors in FP64, once: the preconditioner is applied in FP64
+                    if ((rc = c.ensure(&c.scratch_lu64, &c.scratch_lu64_bytes, (size_t)n * n * sizeof(double))))
+                        return rc;
+                    gw.LU64 = (double *)c.scratch_lu64;
+                    if ((rc = scirs_s2d(n, n, st.LU, ld32, gw.LU64, n, s))) return rc;
+                }
+                for (size_t b0 = 0; b0 < gcols.size(); b0 += GM_NC) {
+                    const size_t b1 = std::min(gcols.size(), b0 + GM_NC);
+                    std::vector<int> cols(gcols.begin() + b0, gcols.begin() + b1), bud(gbud.begin() + b0,
+                                                                                      gbud.begin() + b1);
+                    std::vector<double> rt(grtol.begin() + b0, grtol.begin() + b1);
+                    std::vector<int> used;
+                    std::vector<char> stalled;
+                    if ((rc = gmres_block(st, cols, restart, bud, rt, gw, used, stalled))) return rc;
+                    for (size_t q = 0; q < cols.size(); ++q) {
+                        gm_iters += used[q];
+                        gm_col[cols[q]] += used[q];
+                        if (stalled[q]) gm_col[cols[q]] = max_gmres;   // no further GMRES for this column
+                    }
                 }
             }
             prev = eta;

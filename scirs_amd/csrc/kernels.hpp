@@ -71,6 +71,7 @@ enum {
     SCIRS_LOWP_FP16 = 2,   // FP16 operands with per-block power-of-two scaling, FP32 accumulation
     SCIRS_LOWP_BF16X3 = 3, // FP32 split into hi+lo BF16 pairs, three MFMA products
     SCIRS_LOWP_BF16X6 = 4, // FP32 split into hi+mid+lo BF16 terms, six MFMA products (FP32-accurate)
+    SCIRS_LOWP_FP16X3 = 5, // FP32 split into hi+lo FP16 pairs (power-of-two scaled), three MFMA products
 };
 
 enum { SCIRS_KB_MAX = 256, SCIRS_MAX_G = 512, SCIRS_PANEL_REP_MAX = 8 };
@@ -108,7 +109,7 @@ int scirs_launch_sgemm_sub(int M, int N, int K, const float *A, long lda, const 
 // FP16 update with power-of-two scalings: A times 2^a_shift, column c of B times
 // 2^-colexp[c] (both undone exactly in FP32); colexp from scirs_launch_colexp.
 int scirs_launch_sgemm_fp16_scaled(int M, int N, int K, const float *A, long lda, const float *B, long ldb,
-                                   float *C, long ldc, const int *colexp, int a_shift, const int *info,
+                                   float *C, long ldc, const int *colexp, int a_shift, int mode, const int *info,
                                    hipStream_t stream);
 int scirs_launch_colexp(int K, int N, const float *B, long ldb, int *colexp, const int *info, hipStream_t stream);
 

@@ -830,6 +830,7 @@ int lowp_of_mode(int mode)
     switch (mode) {
     case SCIRS_MODE_BF16: return SCIRS_LOWP_BF16;
     case SCIRS_MODE_FP16: return SCIRS_LOWP_FP16;
+    case SCIRS_MODE_FP16X3: return SCIRS_LOWP_FP16X3;
     case SCIRS_MODE_BF16X3: return SCIRS_LOWP_BF16X3;
     case SCIRS_MODE_FP32: return SCIRS_LOWP_FP32;
     case SCIRS_MODE_BF16X6: return SCIRS_LOWP_BF16X6;
@@ -842,7 +843,8 @@ extern "C" int scirs_gpu_sgetrf_tp_dev(int n, float *dA, int64_t lda, int64_t *d
 {
     using namespace scirs;
     const int lowp = lowp_of_mode(mode == SCIRS_MODE_AUTO ? SCIRS_MODE_BF16X6 : mode);
-    if (n <= 0 || !dA || !dperm || lda < n || (lda & 3) || lowp < 0 || lowp == SCIRS_LOWP_FP16) {
+    if (n <= 0 || !dA || !dperm || lda < n || (lda & 3) || lowp < 0 || lowp == SCIRS_LOWP_FP16 ||
+        lowp == SCIRS_LOWP_FP16X3) {
         scirs_set_error("sgetrf_tp_dev: invalid arguments (n=%d lda=%lld mode=%d; lda a multiple of 4, FP16 "
                         "needs the mixed solve's scaling)", n, (long long)lda, mode);
         return SCIRS_ERR_ARG;

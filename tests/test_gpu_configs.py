@@ -116,12 +116,25 @@ def test_config_batched_20000(L, torch):
 
 
 def test_config_hard_fp16_n32768_kappa1e8(L, torch):
+    """The hard case: FP16x3 factors (FP16 matrix cores, scaled hi/lo split),
+    GMRES-IR with the preconditioner applied in FP64, 8 right-hand sides side
+    by side.  With sigma log-spaced over [1, 1e8], every factorisation below
+    FP64 leaves a fixed fraction of the singular directions unresolved (those
+    with sigma_i below about u_f * growth * sigma_max), so GMRES needs a number
+    of iterations proportional to n: measured, 120 per right-hand side at
+    n = 1024, 990 at n = 2048 (restart 600), more than 2000 at n = 4096 even
+    with FP32-accurate factors.  At n = 32768 the solve therefore ends on the
+    FP64 fallback after GMRES stalls; the result must still meet the FP64 bar,
+    and the test records the path (expected failure while it is 4)."""
     n, nrhs, kappa = 32768, 8, 1e8
-    print("config: FP16 GMRES-IR n=32768 kappa=1e8 nrhs=8", flush=True)
+    print("config: FP16x3 GMRES-IR n=32768 kappa=1e8 nrhs=8", flush=True)
     a, b, x_true = conditioned_gpu(torch, n, kappa, nrhs, 32768)
     x = torch.empty_like(b)
-    info = L.gesv_ir_dev(a, b, x, mode=2)
+    info = L.gesv_ir_dev(a, b, x, mode=6)
     eta = L.backward_error_dev(a, x, b)
     print("path", info.path, "ir", info.ir_iters, "gmres", info.gmres_iters, "eta", eta, flush=True)
     assert eta <= n * U64
     assert info.path in (3, 4)
+    if info.path == 4:
+        pytest.xfail(f"GMRES-IR stalled after {info.gmres_iters} iterations; FP64 LU answered "
+                     "(iterations grow with n for this spectrum)")
