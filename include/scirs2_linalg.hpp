@@ -15,6 +15,8 @@
 //   compat_solve     scirs2-linalg/src/compat.rs:461-499
 //   qr, lstsq        scirs2-linalg/src/decomposition.rs:208-249, solve.rs:234-290 (rows >= cols)
 //   gmres            scirs2-linalg/src/solvers/iterative.rs:317-450
+//   preconditioned_gmres, PreconditionerOp, DiagonalPreconditioner
+//                    scirs2-linalg/src/preconditioners.rs:179-245, 934-1010
 //   batch_matmul, batch_matvec, batch_add, batch_sum
 //                    scirs2-linalg/src/batch/mod.rs:62-336
 //   validation       scirs2-linalg/src/validation.rs:22-329
@@ -828,6 +830,111 @@ Result<IterativeSolverResult<A>> gmres(const ArrayView2<A> &a, const ArrayView1<
     out.residual_norm = (A)res;
     out.converged = conv != 0;
     return out;
+}
+
+// preconditioners.rs:179-196 (PreconditionerOp).  The GPU solve below knows
+// two preconditioners by type; any other PreconditionerOp is refused (its
+// apply is host code the device iteration cannot call per step).
+template <class F>
+struct PreconditionerOp {
+    virtual ~PreconditionerOp() = default;
+    virtual Result<Array1<F>> apply(const ArrayView1<F> &x) const = 0;
+    virtual Result<Array1<F>> apply_transpose(const ArrayView1<F> &x) const { return apply(x); }
+    virtual std::size_t size() const = 0;
+    virtual bool is_symmetric() const { return true; }
+    // which device preconditioner stands in for this one (SCIRS_PRECOND_*), -1: none
+    virtual int gpu_kind() const { return -1; }
+};
+
+// preconditioners.rs:198-245: 1/a_ii, or 1 where |a_ii| < eps
+template <class F>
+class DiagonalPreconditioner : public PreconditionerOp<F> {
+  public:
+    static Result<DiagonalPreconditioner<F>> create(const ArrayView2<F> &matrix)
+    {
+        if (matrix.rows != matrix.cols)
+            return LinalgError(LinalgError::Kind::ShapeError, "Diagonal preconditioner requires square matrix");
+        DiagonalPreconditioner<F> p;
+        p.inverse_diagonal_ = Array1<F>(matrix.rows);
+        for (std::size_t i = 0; i < matrix.rows; ++i) {
+            const F d = matrix(i, i);
+            p.inverse_diagonal_[i] = std::fabs(d) < std::numeric_limits<F>::epsilon() ? F(1) : F(1) / d;
+        }
+        return p;
+    }
+    Result<Array1<F>> apply(const ArrayView1<F> &x) const override
+    {
+        Array1<F> y(x.len);
+        for (std::size_t i = 0; i < x.len; ++i) y[i] = inverse_diagonal_[i] * x[i];
+        return y;
+    }
+    std::size_t size() const override { return inverse_diagonal_.size(); }
+    int gpu_kind() const override { return SCIRS_PRECOND_DIAGONAL; }
+
+  private:
+    Array1<F> inverse_diagonal_;
+};
+
+// No counterpart in the reference: the mixed-precision LU of the matrix (FP32
+// tournament-pivoted panel, FP32-accurate update) as a preconditioner, built
+// on the GPU inside preconditioned_gmres; apply() on the host is not offered.
+template <class F>
+class LUPreconditioner : public PreconditionerOp<F> {
+  public:
+    explicit LUPreconditioner(std::size_t n) : n_(n) {}
+    Result<Array1<F>> apply(const ArrayView1<F> &) const override
+    {
+        return LinalgError(LinalgError::Kind::NotImplementedError,
+                           "LUPreconditioner is applied on the GPU inside preconditioned_gmres");
+    }
+    std::size_t size() const override { return n_; }
+    bool is_symmetric() const override { return false; }
+    int gpu_kind() const override { return SCIRS_PRECOND_LU; }
+
+  private:
+    std::size_t n_;
+};
+
+// preconditioners.rs:934-1010 (preconditioned_gmres), f64 on the GPU.  The
+// reference's body never solves its least-squares problem (x += 0.1 tol v_0);
+// this is the real right-preconditioned restarted GMRES with the gmres
+// iteration of solvers/iterative.rs:317-450, so the result solves A x = rhs.
+// Shape check and text as preconditioners.rs:947-952; restart 0 (a division
+// by zero in the reference) is an InvalidInputError.
+template <class F>
+Result<Array1<F>> preconditioned_gmres(const ArrayView2<F> &matrix, const ArrayView1<F> &rhs,
+                                       const PreconditionerOp<F> &preconditioner, std::size_t max_iterations,
+                                       F tolerance, std::size_t restart, const ArrayView1<F> *initial_guess)
+{
+    static_assert(std::is_same<F, double>::value, "the GPU GMRES handles f64");
+    const std::size_t n = matrix.rows;
+    if (matrix.cols != n || rhs.len != n)
+        return LinalgError(LinalgError::Kind::ShapeError, "Matrix and RHS dimensions must be compatible");
+    if (restart == 0) return LinalgError(LinalgError::Kind::InvalidInputError, "GMRES restart must be positive");
+    const int kind = preconditioner.gpu_kind();
+    if (kind < 0)
+        return LinalgError(LinalgError::Kind::NotImplementedError,
+                           "preconditioned_gmres on the GPU takes DiagonalPreconditioner or LUPreconditioner");
+    Array1<F> x(n);
+    if (n == 0) return x;
+    std::vector<F> x0d;
+    if (initial_guess) {
+        if (initial_guess->len != n)
+            return LinalgError(LinalgError::Kind::ShapeError, "Matrix and RHS dimensions must be compatible");
+        x0d.resize(n);
+        for (std::size_t i = 0; i < n; ++i) x0d[i] = (*initial_guess)[i];
+    }
+    std::vector<F> ad = detail::standard(matrix), bd(n);
+    for (std::size_t i = 0; i < n; ++i) bd[i] = rhs[i];
+    int it = 0, conv = 0;
+    double res = 0.0;
+    int rc = scirs_gpu_dgmres_precond((int)n, ad.data(), (int64_t)n, bd.data(), initial_guess ? x0d.data() : nullptr,
+                                      (int)max_iterations, (double)tolerance, (int)restart, kind, nullptr, nullptr,
+                                      x.data(), &it, &res, &conv, nullptr);
+    if (rc > 0)
+        return LinalgError(LinalgError::Kind::SingularMatrixError, "Matrix is singular in GMRES least squares solve");
+    if (rc < 0) return detail::gpu_failure(rc);
+    return x;
 }
 
 // ------------------------------------------------------------------ batch (batch/mod.rs)

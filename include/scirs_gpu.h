@@ -154,6 +154,32 @@ int scirs_gpu_dgmres_dev(int n, const double *dA, int64_t lda, const double *dB,
                          int max_iter, double tol, int restart, int *iterations, double *residual_norm,
                          int *converged, void *stream, scirs_solve_info *info);
 
+/* Replaces scirs2-linalg/src/preconditioners.rs:934-1010 `preconditioned_gmres`
+ * for f64 with a real solver (the reference's body never solves its
+ * least-squares problem and ends with x += 0.1 tol v_0): the gmres iteration
+ * above, right-preconditioned — the Arnoldi process runs on A M^-1 and each
+ * cycle ends with x += M^-1 (V y) — so residual norm and stopping test are
+ * those of A x = b.  precond: SCIRS_PRECOND_NONE, _DIAGONAL (the Jacobi
+ * preconditioner of preconditioners.rs:210-229: 1/a_ii, or 1 where
+ * |a_ii| < eps), _LU (the library factors A: FP32 tournament-pivoted panel,
+ * FP32-accurate split-BF16 update, applied in FP64), _LU_GIVEN (lu: the
+ * caller's n x n packed FP64 factors, unit lower L below the diagonal and U on
+ * and above, with perm[i] the row of A at position i: P A = L U).  Returns as
+ * scirs_gpu_dgmres; with _LU, k > 0 also when the FP32 factorisation meets a
+ * zero pivot at column k. */
+#define SCIRS_PRECOND_NONE 0
+#define SCIRS_PRECOND_DIAGONAL 1
+#define SCIRS_PRECOND_LU 2
+#define SCIRS_PRECOND_LU_GIVEN 3
+int scirs_gpu_dgmres_precond(int n, const double *a, int64_t lda, const double *b, const double *x0, int max_iter,
+                             double tol, int restart, int precond, const double *lu, const int64_t *perm, double *x,
+                             int *iterations, double *residual_norm, int *converged, scirs_solve_info *info);
+/* The same on device-resident data; dLU / dPerm (device) for _LU_GIVEN. */
+int scirs_gpu_dgmres_precond_dev(int n, const double *dA, int64_t lda, const double *dB, double *dX, int use_x0,
+                                 int max_iter, double tol, int restart, int precond, const double *dLU,
+                                 const int64_t *dPerm, int *iterations, double *residual_norm, int *converged,
+                                 void *stream, scirs_solve_info *info);
+
 /* ---- batched array operations ------------------------------------------------ */
 /* Replace scirs2-linalg/src/batch/mod.rs for f64 / f32, bit-identical to its
  * loops (each element summed from +0 in the reference's order, no fused

@@ -70,6 +70,7 @@ def _load():
         getattr(lib, f"oracle_batch_add_{s}").argtypes = [I, I, I, P, P, I, P]
         getattr(lib, f"oracle_batch_sum_{s}").argtypes = [I, I, I, P, P]
     lib.oracle_gmres_d.argtypes = [I, P, L, P, P, I, ctypes.c_double, I, P, P, P, P]
+    lib.oracle_pgmres_d.argtypes = [I, P, L, P, P, I, ctypes.c_double, I, I, P, P, P, P, P, P]
     lib.oracle_qr_factor_d.argtypes = [I, I, P, L, P, P]
     lib.oracle_lstsq_d.argtypes = [I, I, P, L, P, P, P]
     _lib = lib
@@ -308,4 +309,30 @@ def gmres(a, b, x0=None, max_iter=1000, tol=1e-10, restart=0):
         raise OracleError("singular", rc)
     if rc < 0:
         raise OracleError("memory", rc)
+    return x, int(it[0]), float(res[0]), bool(conv[0])
+
+
+def pgmres(a, b, kind, lu=None, perm=None, x0=None, max_iter=1000, tol=1e-10, restart=0):
+    """Right-preconditioned restatement of solvers/iterative.rs:317-450 (the
+    real counterpart of preconditioners.rs:934-1010): kind 0 none, 1 Jacobi
+    (preconditioners.rs:210-229), 3 given packed LU factors with perm (P A = L U,
+    perm[i] the original row at position i).  (x, iterations, residual_norm,
+    converged)."""
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    n = a.shape[0]
+    x = np.empty(n)
+    it = np.zeros(1, dtype=np.int32)
+    conv = np.zeros(1, dtype=np.int32)
+    res = np.zeros(1)
+    x0p = None if x0 is None else _c(x0, np.float64)
+    lup = None if lu is None else _c(lu, np.float64)
+    pp = None if perm is None else np.ascontiguousarray(perm, dtype=np.int64)
+    rc = _load().oracle_pgmres_d(n, _ptr(a), n, _ptr(b), None if x0p is None else _ptr(x0p), int(max_iter),
+                                 float(tol), int(restart), int(kind), None if lup is None else _ptr(lup),
+                                 None if pp is None else _ptr(pp), _ptr(x), _ptr(it), _ptr(res), _ptr(conv))
+    if rc > 0:
+        raise OracleError("singular", rc)
+    if rc < 0:
+        raise OracleError("argument" if rc == -2 else "memory", rc)
     return x, int(it[0]), float(res[0]), bool(conv[0])

@@ -420,13 +420,44 @@ static double oracle_resid_norm(int n, const double *a, long lda, const double *
     return sqrt(s);
 }
 
+/* M^-1 v of the right-preconditioned variant below: kind 0 identity, 1 the
+ * Jacobi preconditioner of preconditioners.rs:210-229 (1/a_ii, or 1 where
+ * |a_ii| < eps), 3 given LU factors (packed: unit lower L below the diagonal,
+ * U on and above; P A = L U with perm[i] the original row at position i). */
+static void oracle_prec_apply(int n, int kind, const double *dinv, const double *lu, const int64_t *perm,
+                              const double *v, double *z)
+{
+    if (kind == 1) {
+        for (int i = 0; i < n; ++i) z[i] = dinv[i] * v[i];
+    } else if (kind == 3) {
+        for (int i = 0; i < n; ++i) z[i] = v[perm[i]];
+        for (int i = 0; i < n; ++i) {
+            double s = z[i];
+            for (int j = 0; j < i; ++j) s -= lu[(size_t)i * n + j] * z[j];
+            z[i] = s;
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double s = z[i];
+            for (int j = i + 1; j < n; ++j) s -= lu[(size_t)i * n + j] * z[j];
+            z[i] = s / lu[(size_t)i * n + i];
+        }
+    } else {
+        memcpy(z, v, sizeof(double) * (size_t)n);
+    }
+}
+
 /* solvers/iterative.rs:317-450 (gmres), f64: restarted GMRES with classical
  * Gram-Schmidt against the unmodified w, a break (counting no iteration) when
  * h[j+1][j] < eps, (max_iterations / restart).max(1) cycles.  x0 may be NULL
  * (zeros); restart <= 0 means min(n, 30).  Returns 0, or k+1 > 0 when the
- * least-squares solve is singular at row k+1. */
-int oracle_gmres_d(int n, const double *a, long lda, const double *b, const double *x0, int max_iter, double tol,
-                   int restart, double *x, int *iterations, double *residual_norm, int *converged)
+ * least-squares solve is singular at row k+1.
+ * With kind != 0 the same iteration is right-preconditioned: the Arnoldi
+ * process runs on A M^-1 (w = A M^-1 v_j) and the cycle's update is
+ * x += M^-1 (sum_i y_i v_i), so the residual, its norm and the stopping test
+ * stay those of A x = b. */
+static int oracle_gmres_core(int n, const double *a, long lda, const double *b, const double *x0, int max_iter,
+                             double tol, int restart, int kind, const double *lu, const int64_t *perm, double *x,
+                             int *iterations, double *residual_norm, int *converged)
 {
     if (restart <= 0) restart = n < 30 ? n : 30;
     double *v = (double *)calloc((size_t)(restart + 1) * n, sizeof(double));
@@ -435,7 +466,18 @@ int oracle_gmres_d(int n, const double *a, long lda, const double *b, const doub
     double *wo = (double *)malloc(sizeof(double) * (size_t)n);
     double *r = (double *)malloc(sizeof(double) * (size_t)n);
     double *y = (double *)malloc(sizeof(double) * (size_t)(restart + 1));
-    if (!v || !h || !w || !wo || !r || !y) { free(v); free(h); free(w); free(wo); free(r); free(y); return -3; }
+    double *z = (double *)malloc(sizeof(double) * (size_t)n);
+    double *t = (double *)malloc(sizeof(double) * (size_t)n);
+    double *dinv = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!v || !h || !w || !wo || !r || !y || !z || !t || !dinv) {
+        free(v); free(h); free(w); free(wo); free(r); free(y); free(z); free(t); free(dinv);
+        return -3;
+    }
+    if (kind == 1)
+        for (int i = 0; i < n; ++i) {
+            const double d = a[(size_t)i * lda + i];
+            dinv[i] = fabs(d) < DBL_EPSILON ? 1.0 : 1.0 / d;
+        }
     for (int i = 0; i < n; ++i) x[i] = x0 ? x0[i] : 0.0;
     int total = 0, rc = 0;
     int cycles = max_iter / restart;
@@ -452,9 +494,10 @@ int oracle_gmres_d(int n, const double *a, long lda, const double *b, const doub
         int j = 0;
         while (j < restart && total < max_iter) {
             const double *vj = v + (size_t)j * n;
+            oracle_prec_apply(n, kind, dinv, lu, perm, vj, z);
             for (int i = 0; i < n; ++i) {
                 double s = 0.0;
-                for (int q = 0; q < n; ++q) s += a[(size_t)i * lda + q] * vj[q];
+                for (int q = 0; q < n; ++q) s += a[(size_t)i * lda + q] * z[q];
                 w[i] = s;
             }
             memcpy(wo, w, sizeof(double) * (size_t)n);
@@ -483,8 +526,16 @@ int oracle_gmres_d(int n, const double *a, long lda, const double *b, const doub
             free(hp);
             if (rc) goto done;
         }
-        for (int i = 0; i < j; ++i)
-            for (int q = 0; q < n; ++q) x[q] += y[i] * v[(size_t)i * n + q];
+        if (kind == 0) {
+            for (int i = 0; i < j; ++i)
+                for (int q = 0; q < n; ++q) x[q] += y[i] * v[(size_t)i * n + q];
+        } else {
+            for (int q = 0; q < n; ++q) t[q] = 0.0;
+            for (int i = 0; i < j; ++i)
+                for (int q = 0; q < n; ++q) t[q] += y[i] * v[(size_t)i * n + q];
+            oracle_prec_apply(n, kind, dinv, lu, perm, t, z);
+            for (int q = 0; q < n; ++q) x[q] += z[q];
+        }
         const double rn = oracle_resid_norm(n, a, lda, b, x, r);
         if (rn < tol) {
             *iterations = total; *residual_norm = rn; *converged = 1;
@@ -494,9 +545,29 @@ int oracle_gmres_d(int n, const double *a, long lda, const double *b, const doub
     *iterations = total;
     *residual_norm = oracle_resid_norm(n, a, lda, b, x, r);
 done:
-    free(v); free(h); free(w); free(wo); free(r); free(y);
+    free(v); free(h); free(w); free(wo); free(r); free(y); free(z); free(t); free(dinv);
     return rc;
 }
 
-/* Version tag so a test can tell that the built library matches this source. */
+int oracle_gmres_d(int n, const double *a, long lda, const double *b, const double *x0, int max_iter, double tol,
+                   int restart, double *x, int *iterations, double *residual_norm, int *converged)
+{
+    return oracle_gmres_core(n, a, lda, b, x0, max_iter, tol, restart, 0, NULL, NULL, x, iterations, residual_norm,
+                             converged);
+}
+
+/* The right-preconditioned restatement of the same iteration (the real
+ * counterpart of preconditioners.rs:934-1010 `preconditioned_gmres`, whose
+ * body never solves the least-squares problem): kind 0 none, 1 Jacobi,
+ * 3 the given LU factors (lu n x n packed, perm). */
+int oracle_pgmres_d(int n, const double *a, long lda, const double *b, const double *x0, int max_iter, double tol,
+                    int restart, int kind, const double *lu, const int64_t *perm, double *x, int *iterations,
+                    double *residual_norm, int *converged)
+{
+    if (kind != 0 && kind != 1 && kind != 3) return -2;
+    if (kind == 3 && (!lu || !perm)) return -2;
+    return oracle_gmres_core(n, a, lda, b, x0, max_iter, tol, restart, kind, lu, perm, x, iterations, residual_norm,
+                             converged);
+}
+
 int oracle_version(void) { return 1; }

@@ -23,6 +23,7 @@ SCIRS_ERR_UNSUPPORTED = -6
 
 PATH_FP64_LU, PATH_FP64_CHOL, PATH_MIXED_IR, PATH_MIXED_GMRES, PATH_FP64_FALLBACK, PATH_FP32_LU, PATH_FP32_CHOL = range(7)
 MODE_AUTO, MODE_BF16, MODE_FP16, MODE_BF16X3, MODE_FP32, MODE_BF16X6, MODE_FP16X3 = range(7)
+PRECOND_NONE, PRECOND_DIAGONAL, PRECOND_LU, PRECOND_LU_GIVEN = range(4)
 
 
 class SolveInfo(ctypes.Structure):
@@ -88,6 +89,8 @@ SIGNATURES = {
     "scirs_gpu_sposv": ([_I, _I, _P, _L, _P, _L, _P, _L, _INFO], _I),
     "scirs_gpu_dgmres": ([_I, _P, _L, _P, _P, _I, _D, _I, _P, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dgmres_dev": ([_I, _P, _L, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _INFO], _I),
+    "scirs_gpu_dgmres_precond": ([_I, _P, _L, _P, _P, _I, _D, _I, _I, _P, _P, _P, _P, _P, _P, _INFO], _I),
+    "scirs_gpu_dgmres_precond_dev": ([_I, _P, _L, _P, _P, _I, _I, _D, _I, _I, _P, _P, _P, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dbatch_matmul": ([_I, _I, _I, _I, _P, _P, _P, _INFO], _I),
     "scirs_gpu_sbatch_matmul": ([_I, _I, _I, _I, _P, _P, _P, _INFO], _I),
     "scirs_gpu_dbatch_matvec": ([_I, _I, _I, _P, _P, _P, _INFO], _I),

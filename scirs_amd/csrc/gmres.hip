@@ -1,6 +1,9 @@
 // Restarted GMRES of scirs2-linalg/src/solvers/iterative.rs:317-450 on the GPU:
 //
 //   scirs_gpu_dgmres  <- gmres  solvers/iterative.rs:317-450 (f64)
+//   scirs_gpu_dgmres_precond <- preconditioned_gmres  preconditioners.rs:934-1010
+//                     (the same iteration, right-preconditioned: the reference's
+//                     body never solves its least-squares problem)
 //
 // The reference's iteration, step for step: (max_iterations / restart).max(1)
 // cycles; per cycle r = b - A x, beta = ||r||, V_0 = r / beta; per inner step
@@ -114,6 +117,42 @@ __global__ __launch_bounds__(NT) void gm_combine_kernel(int n, int k, const doub
     x[q] = s;
 }
 
+// z = dinv * v (the Jacobi preconditioner)
+__global__ __launch_bounds__(NT) void gm_scale_vec_kernel(int n, const double *__restrict__ dinv,
+                                                          const double *__restrict__ v, double *__restrict__ z)
+{
+    const int q = blockIdx.x * NT + threadIdx.x;
+    if (q < n) z[q] = dinv[q] * v[q];
+}
+
+// dinv[i] = 1 / a_ii, or 1 where |a_ii| < eps (preconditioners.rs:210-229)
+__global__ __launch_bounds__(NT) void gm_jacobi_kernel(int n, const double *__restrict__ A, long lda,
+                                                       double *__restrict__ dinv)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const double d = A[(long)i * lda + i];
+    dinv[i] = fabs(d) < DBL_EPSILON ? 1.0 : 1.0 / d;
+}
+
+// dst = sum_{i<k} y_i V_i, added in order from 0
+__global__ __launch_bounds__(NT) void gm_basis_combine_kernel(int n, int k, const double *__restrict__ V,
+                                                              const double *__restrict__ y, double *__restrict__ dst)
+{
+    const int q = blockIdx.x * NT + threadIdx.x;
+    if (q >= n) return;
+    double s = 0.0;
+    for (int i = 0; i < k; ++i) s += y[i] * V[(long)i * n + q];
+    dst[q] = s;
+}
+
+// x += z
+__global__ __launch_bounds__(NT) void gm_add_kernel(int n, const double *__restrict__ z, double *__restrict__ x)
+{
+    const int q = blockIdx.x * NT + threadIdx.x;
+    if (q < n) x[q] += z[q];
+}
+
 // iterative.rs:604-664: Givens sweep over the m x nc Hessenberg block, then
 // back substitution; returns i+1 when |r_ii| < eps.
 int gmres_lsq(int m, int nc, std::vector<double> hc, double beta, std::vector<double> &y)
@@ -148,21 +187,45 @@ int gmres_lsq(int m, int nc, std::vector<double> hc, double beta, std::vector<do
 
 unsigned blocks(int n, int per) { return (unsigned)((n + per - 1) / per); }
 
+// Right preconditioner M^-1 on the device: none, Jacobi (dinv) or LU factors
+// in FP64 (lu64, n x n packed, P A = L U with perm) applied by permutation and
+// the two chained triangular solves.
+struct Precond {
+    int kind = SCIRS_PRECOND_NONE;
+    const double *dinv = nullptr;
+    const double *lu64 = nullptr;
+    const int64_t *perm = nullptr;
+};
+
+int precond_apply(const Precond &m, int n, const double *v, double *z, hipStream_t s)
+{
+    if (m.kind == SCIRS_PRECOND_DIAGONAL) {
+        gm_scale_vec_kernel<<<blocks(n, NT), NT, 0, s>>>(n, m.dinv, v, z);
+        SCIRS_HIP_CHECK(hipGetLastError());
+        return SCIRS_OK;
+    }
+    if (int rc = scirs_permute_rows<double>(n, 1, m.perm, v, 1, z, 1, s)) return rc;
+    return scirs::lu_substitute<double>(n, 1, m.lu64, n, z, 1, s);
+}
+
 // The iteration on device-resident A (lda), b and x (x holds x0 on entry, the
 // solution on return), on stream s; the Krylov basis and small vectors live in
-// the library's scratch.  Returns 0, k > 0 (least squares singular at row k)
-// or an SCIRS_ERR_* code.
+// the library's scratch.  With a preconditioner the Arnoldi process runs on
+// A M^-1 and each cycle ends with x += M^-1 (V y): right preconditioning, so
+// the residual and the stopping test stay those of A x = b.  Returns 0,
+// k > 0 (least squares singular at row k) or an SCIRS_ERR_* code.
 int gmres_device(int n, const double *dA, long lda, const double *db, double *dx, int max_iter, double tol,
                  int restart, int *iterations, double *residual_norm, int *converged, hipStream_t s,
-                 scirs_solve_info *info)
+                 scirs_solve_info *info, const Precond &pm = Precond())
 {
     Context &c = ctx();
     int rc;
     const size_t nn = (size_t)n;
-    const size_t wbytes = ((size_t)(restart + 1) * nn + 2 * nn + 2 * (size_t)(restart + 2) + 64) * sizeof(double);
+    const size_t wbytes = ((size_t)(restart + 1) * nn + 3 * nn + 2 * (size_t)(restart + 2) + 64) * sizeof(double);
     if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, wbytes))) return rc;
-    double *dV = (double *)c.scratch_b, *dr = dV + (size_t)(restart + 1) * nn, *dw = dr + nn, *dh = dw + nn,
-           *dy = dh + (restart + 2), *dsc = dy + (restart + 2);
+    double *dV = (double *)c.scratch_b, *dr = dV + (size_t)(restart + 1) * nn, *dw = dr + nn, *dz = dw + nn,
+           *dh = dz + nn, *dy = dh + (restart + 2), *dsc = dy + (restart + 2);
+    const bool prec = pm.kind != SCIRS_PRECOND_NONE;
 
     // ||b - A x|| into the host; r stays in dr
     auto resid_norm = [&](double *out) -> int {
@@ -195,7 +258,13 @@ int gmres_device(int n, const double *dA, long lda, const double *db, double *dx
         int j = 0;
         while (j < restart && total < max_iter) {
             double *vj = dV + (size_t)j * nn, *vn = vj + nn;
-            gm_gemv_kernel<<<blocks(n, NT / 64), NT, 0, s>>>(n, dA, lda, vj, dw);
+            if (prec) {
+                // w = A M^-1 v_j
+                if ((rc = precond_apply(pm, n, vj, dz, s))) return rc;
+                gm_gemv_kernel<<<blocks(n, NT / 64), NT, 0, s>>>(n, dA, lda, dz, dw);
+            } else {
+                gm_gemv_kernel<<<blocks(n, NT / 64), NT, 0, s>>>(n, dA, lda, vj, dw);
+            }
             gm_dots_kernel<<<j + 1, 1024, 0, s>>>(n, j + 1, dV, dw, dh);
             gm_orth_kernel<<<blocks(n, NT), NT, 0, s>>>(n, j + 1, dV, dh, dw, vn);
             gm_dots_kernel<<<1, 1024, 0, s>>>(n, 1, vn, vn, dh + j + 1);
@@ -223,7 +292,14 @@ int gmres_device(int n, const double *dA, long lda, const double *db, double *dx
                 return st;
             }
             SCIRS_HIP_CHECK(hipMemcpyAsync(dy, y.data(), j * sizeof(double), hipMemcpyHostToDevice, s));
-            gm_combine_kernel<<<blocks(n, NT), NT, 0, s>>>(n, j, dV, dy, dx);
+            if (prec) {
+                // x += M^-1 (V y)
+                gm_basis_combine_kernel<<<blocks(n, NT), NT, 0, s>>>(n, j, dV, dy, dw);
+                if ((rc = precond_apply(pm, n, dw, dz, s))) return rc;
+                gm_add_kernel<<<blocks(n, NT), NT, 0, s>>>(n, dz, dx);
+            } else {
+                gm_combine_kernel<<<blocks(n, NT), NT, 0, s>>>(n, j, dV, dy, dx);
+            }
             SCIRS_HIP_CHECK(hipGetLastError());
         }
         double rn;
@@ -286,5 +362,151 @@ extern "C" int scirs_gpu_dgmres_dev(int n, const double *dA, int64_t lda, const 
     hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
     if (!use_x0) SCIRS_HIP_CHECK(hipMemsetAsync(dX, 0, (size_t)n * sizeof(double), s));
     rc = gmres_device(n, dA, lda, dB, dX, max_iter, tol, restart, iterations, residual_norm, converged, s, info);
+    return finish(info, rc, t0);
+}
+
+namespace {
+// The mixed-precision LU of A as the preconditioner: FP32 tournament-pivoted
+// panel and FP32-accurate split-BF16 update (the factors of the mixed solve),
+// upcast to FP64 so that M^-1 is applied without FP32 rounding.  Returns 0,
+// k > 0 when the FP32 factorisation breaks down at column k, or an error.
+int lu_preconditioner(int n, const double *dA, long lda, double *lu64, int64_t *perm, hipStream_t s)
+{
+    Context &c = ctx();
+    int rc;
+    const long ld32 = ((long)n + 63) / 64 * 64;
+    if ((rc = c.ensure(&c.scratch_ir, &c.scratch_ir_bytes, (size_t)n * ld32 * sizeof(float)))) return rc;
+    float *a32 = (float *)c.scratch_ir;
+    if ((rc = scirs_d2s(n, n, dA, lda, a32, ld32, s))) return rc;
+    if ((rc = c.reset_flags(s))) return rc;
+    const bool tslu = c.work.tslu != 0;
+    const int kb = tslu ? c.kb_f32 : c.lu_kb<float>(n);
+    const int saved = c.work.lowp_mode;
+    c.work.lowp_mode = SCIRS_LOWP_BF16X6;
+    c.work.tslu_active = tslu;
+    rc = scirs_getrf_device<float>(n, n, a32, ld32, kb, c.work, s);
+    c.work.tslu_active = 0;
+    c.work.lowp_mode = saved;
+    if (rc) return rc;
+    if ((rc = scirs_perm_from_pairs(n, n, kb, c.work, perm, s))) return rc;
+    int st = 0;
+    if ((rc = c.read_flags(s, &st, nullptr))) return rc;
+    if (st) return st;
+    return scirs_s2d(n, n, a32, ld32, lu64, n, s);
+}
+
+// Builds the device preconditioner of kind `precond` for device-resident A;
+// for SCIRS_PRECOND_LU_GIVEN, dLU / dPerm are the caller's device factors.
+int make_precond(int n, const double *dA, long lda, int precond, const double *dLU, const int64_t *dPerm,
+                 hipStream_t s, Precond &pm)
+{
+    Context &c = ctx();
+    int rc;
+    pm.kind = precond;
+    if (precond == SCIRS_PRECOND_NONE) return SCIRS_OK;
+    if (precond == SCIRS_PRECOND_DIAGONAL) {
+        if ((rc = c.ensure(&c.scratch_perm, &c.scratch_perm_bytes, (size_t)n * sizeof(double)))) return rc;
+        double *dinv = (double *)c.scratch_perm;
+        gm_jacobi_kernel<<<blocks(n, NT), NT, 0, s>>>(n, dA, lda, dinv);
+        SCIRS_HIP_CHECK(hipGetLastError());
+        pm.dinv = dinv;
+        return SCIRS_OK;
+    }
+    if (precond == SCIRS_PRECOND_LU_GIVEN) {
+        pm.lu64 = dLU;
+        pm.perm = dPerm;
+        return SCIRS_OK;
+    }
+    // SCIRS_PRECOND_LU
+    if ((rc = c.ensure(&c.scratch_lu64, &c.scratch_lu64_bytes, (size_t)n * n * sizeof(double)))) return rc;
+    if ((rc = c.ensure(&c.scratch_perm, &c.scratch_perm_bytes, (size_t)n * sizeof(int64_t)))) return rc;
+    double *lu64 = (double *)c.scratch_lu64;
+    int64_t *perm = (int64_t *)c.scratch_perm;
+    if ((rc = lu_preconditioner(n, dA, lda, lu64, perm, s))) return rc;
+    pm.lu64 = lu64;
+    pm.perm = perm;
+    return SCIRS_OK;
+}
+
+bool precond_kind_ok(int k)
+{
+    return k == SCIRS_PRECOND_NONE || k == SCIRS_PRECOND_DIAGONAL || k == SCIRS_PRECOND_LU ||
+           k == SCIRS_PRECOND_LU_GIVEN;
+}
+} // namespace
+
+extern "C" int scirs_gpu_dgmres_precond(int n, const double *a, int64_t lda, const double *b, const double *x0,
+                                        int max_iter, double tol, int restart, int precond, const double *lu,
+                                        const int64_t *perm, double *x, int *iterations, double *residual_norm,
+                                        int *converged, scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (n <= 0 || max_iter < 0 || !a || !b || !x || lda < n || !iterations || !residual_norm || !converged ||
+        !precond_kind_ok(precond) || (precond == SCIRS_PRECOND_LU_GIVEN && (!lu || !perm))) {
+        scirs_set_error("dgmres_precond: invalid arguments (n=%d max_iter=%d precond=%d; the given-LU "
+                        "preconditioner needs lu and perm)", n, max_iter, precond);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    if (precond == SCIRS_PRECOND_LU_GIVEN)
+        for (int i = 0; i < n; ++i)
+            if (perm[i] < 0 || perm[i] >= n) {
+                scirs_set_error("dgmres_precond: perm[%d] = %lld is not a row index", i, (long long)perm[i]);
+                return finish(info, SCIRS_ERR_ARG, t0);
+            }
+    if (restart <= 0) restart = std::min(n, 30);
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    Context &c = ctx();
+    int rc = c.init(0);
+    if (rc) return finish(info, rc, t0);
+    hipStream_t s = c.stream;
+    const size_t nn = (size_t)n;
+    const size_t extra = precond == SCIRS_PRECOND_LU_GIVEN ? nn * nn + nn : 0;   // given factors and perm
+    if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (nn * nn + 2 * nn + extra) * sizeof(double))))
+        return finish(info, rc, t0);
+    double *dA = (double *)c.scratch_a, *db = dA + nn * nn, *dx = db + nn;
+    double *dLU = extra ? dx + nn : nullptr;
+    int64_t *dP = extra ? (int64_t *)(dLU + nn * nn) : nullptr;
+    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, nn * sizeof(double), a, lda * sizeof(double), nn * sizeof(double), n,
+                                     hipMemcpyHostToDevice, s));
+    SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, nn * sizeof(double), hipMemcpyHostToDevice, s));
+    if (extra) {
+        SCIRS_HIP_CHECK(hipMemcpyAsync(dLU, lu, nn * nn * sizeof(double), hipMemcpyHostToDevice, s));
+        SCIRS_HIP_CHECK(hipMemcpyAsync(dP, perm, nn * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    }
+    if (x0) SCIRS_HIP_CHECK(hipMemcpyAsync(dx, x0, nn * sizeof(double), hipMemcpyHostToDevice, s));
+    else SCIRS_HIP_CHECK(hipMemsetAsync(dx, 0, nn * sizeof(double), s));
+    Precond pm;
+    if ((rc = make_precond(n, dA, n, precond, dLU, dP, s, pm))) return finish(info, rc, t0);
+    rc = gmres_device(n, dA, n, db, dx, max_iter, tol, restart, iterations, residual_norm, converged, s, info, pm);
+    if (rc) return finish(info, rc, t0);
+    SCIRS_HIP_CHECK(hipMemcpyAsync(x, dx, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    SCIRS_HIP_CHECK(hipStreamSynchronize(s));
+    return finish(info, SCIRS_OK, t0);
+}
+
+extern "C" int scirs_gpu_dgmres_precond_dev(int n, const double *dA, int64_t lda, const double *dB, double *dX,
+                                            int use_x0, int max_iter, double tol, int restart, int precond,
+                                            const double *dLU, const int64_t *dPerm, int *iterations,
+                                            double *residual_norm, int *converged, void *stream,
+                                            scirs_solve_info *info)
+{
+    const double t0 = now_ms();
+    info_init(info, SCIRS_PATH_FP64_LU);
+    if (n <= 0 || max_iter < 0 || !dA || !dB || !dX || lda < n || !iterations || !residual_norm || !converged ||
+        !precond_kind_ok(precond) || (precond == SCIRS_PRECOND_LU_GIVEN && (!dLU || !dPerm))) {
+        scirs_set_error("dgmres_precond_dev: invalid arguments (n=%d max_iter=%d precond=%d; the given-LU "
+                        "preconditioner needs dLU and dPerm)", n, max_iter, precond);
+        return finish(info, SCIRS_ERR_ARG, t0);
+    }
+    if (restart <= 0) restart = std::min(n, 30);
+    std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+    int rc = ctx().init(0);
+    if (rc) return finish(info, rc, t0);
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream, ordered after the caller's work on it
+    if (!use_x0) SCIRS_HIP_CHECK(hipMemsetAsync(dX, 0, (size_t)n * sizeof(double), s));
+    Precond pm;
+    if ((rc = make_precond(n, dA, lda, precond, dLU, dPerm, s, pm))) return finish(info, rc, t0);
+    rc = gmres_device(n, dA, lda, dB, dX, max_iter, tol, restart, iterations, residual_norm, converged, s, info, pm);
     return finish(info, rc, t0);
 }

@@ -170,6 +170,31 @@ def gmres(a, b, x0=None, max_iter=1000, tol=1e-10, restart=0):
     return x, it.value, res.value, bool(conv.value)
 
 
+def gmres_precond(a, b, precond, lu=None, perm=None, x0=None, max_iter=1000, tol=1e-10, restart=0):
+    """Right-preconditioned restarted GMRES (the real counterpart of
+    preconditioners.rs:934-1010): precond is N.PRECOND_NONE / _DIAGONAL / _LU /
+    _LU_GIVEN (lu: packed FP64 factors, perm: P A = L U).  Returns
+    (x, iterations, residual_norm, converged)."""
+    lib = N.load()
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    n = a.shape[0]
+    x = np.empty(n)
+    it = ctypes.c_int(0)
+    conv = ctypes.c_int(0)
+    res = ctypes.c_double(0.0)
+    x0p = None if x0 is None else _c(x0, np.float64)
+    lup = None if lu is None else _c(lu, np.float64)
+    pp = None if perm is None else np.ascontiguousarray(perm, dtype=np.int64)
+    info = N.SolveInfo()
+    rc = lib.scirs_gpu_dgmres_precond(n, _ptr(a), n, _ptr(b), None if x0p is None else _ptr(x0p), int(max_iter),
+                                      float(tol), int(restart), int(precond), None if lup is None else _ptr(lup),
+                                      None if pp is None else _ptr(pp), _ptr(x), ctypes.byref(it),
+                                      ctypes.byref(res), ctypes.byref(conv), ctypes.byref(info))
+    _check("gmres_precond", rc, info)
+    return x, it.value, res.value, bool(conv.value)
+
+
 def gmres_dev(a, b, x, use_x0=False, max_iter=1000, tol=1e-10, restart=0):
     """gmres on device tensors (float64, cuda): x holds x0 when use_x0 and the
     solution on return.  Returns (iterations, residual_norm, converged)."""

@@ -155,6 +155,26 @@ static void run(const std::string &op, int f1, int f2, std::size_t ar, std::size
             std::printf("ERR Other gmres is f64 only\n");
             return;
         }
+    } else if (op == "preconditioned_gmres") {
+        // f1 = preconditioner: 0 Diagonal, 1 LU; f2 = restart (< 0: 0); max_iterations 1000, tol 1e-10
+        if constexpr (std::is_same<F, double>::value) {
+            const std::size_t restart = f2 < 0 ? 0 : (f2 > 0 ? (std::size_t)f2 : 30);
+            Result<Array1<F>> r = [&]() -> Result<Array1<F>> {
+                if (f1 == 1) {
+                    LUPreconditioner<F> lu(a.rows);
+                    return preconditioned_gmres<F>(a.view(), view1(bvec), lu, 1000, 1e-10, restart, nullptr);
+                }
+                auto dp = DiagonalPreconditioner<F>::create(a.view());
+                if (dp.is_err()) return dp.error();
+                return preconditioned_gmres<F>(a.view(), view1(bvec), dp.value(), 1000, 1e-10, restart, nullptr);
+            }();
+            if (r.is_err()) return print_err(r.error());
+            std::printf("OK 1");
+            put(r.value().size(), 1, r.value().data());
+        } else {
+            std::printf("ERR Other preconditioned_gmres is f64 only\n");
+            return;
+        }
     } else if (op == "batch_matmul" || op == "batch_matvec" || op == "batch_add" || op == "batch_sum") {
         // f1 = batch count: A is given as (batch*m) x k; f2 = axis for batch_add
         const std::size_t nb = (std::size_t)f1, m = nb ? ar / nb : 0;

@@ -586,3 +586,41 @@ def test_gmres_results():
         assert int(it[0, 0]) == ito and bool(conv[0, 0]) == convo
         assert np.abs(x[:, 0] - xo).max() <= 1e-9 * max(1.0, np.abs(xo).max())
     assert np.allclose(got[2][1][0][:, 0], [1.0, 2.0], atol=1e-10)
+
+
+def test_preconditioned_gmres_validation_texts():
+    """preconditioned_gmres (preconditioners.rs:947-952): the reference's shape
+    text; DiagonalPreconditioner's own shape check (preconditioners.rs:211-216);
+    restart 0 (a division by zero in the reference) is an error.  No device
+    work is reached."""
+    got = run([
+        ("preconditioned_gmres", "f64", 1, 0, np.ones((2, 3)), np.ones(2)),
+        ("preconditioned_gmres", "f64", 1, 0, np.eye(3), np.ones(2)),
+        ("preconditioned_gmres", "f64", 0, 0, np.ones((2, 3)), np.ones(2)),
+        ("preconditioned_gmres", "f64", 1, -1, np.eye(3), np.ones(3)),
+    ])
+    assert got[0] == ("ERR", "ShapeError", "Shape error: Matrix and RHS dimensions must be compatible")
+    assert got[1] == ("ERR", "ShapeError", "Shape error: Matrix and RHS dimensions must be compatible")
+    assert got[2] == ("ERR", "ShapeError", "Shape error: Diagonal preconditioner requires square matrix")
+    assert got[3] == ("ERR", "InvalidInputError", "Invalid input error: GMRES restart must be positive")
+
+
+@pytest.mark.gpu
+def test_preconditioned_gmres_results():
+    """Through the mirror, both preconditioners solve A x = b (the reference's
+    stub would not): Jacobi gives the oracle's right-preconditioned iterate,
+    the LU preconditioner LAPACK's solution."""
+    from oracle import oracle
+    rng = np.random.default_rng(41)
+    n = 90
+    d = np.logspace(-2, 2, n)
+    a = d[:, None] * (np.eye(n) + 0.3 * rng.standard_normal((n, n)) / np.sqrt(n))
+    b = rng.standard_normal(n)
+    got = run([("preconditioned_gmres", "f64", 0, 10, a, b), ("preconditioned_gmres", "f64", 1, 10, a, b)])
+    xo, ito, _, convo = oracle.pgmres(a, b, 1, max_iter=1000, tol=1e-10, restart=10)
+    assert convo and got[0][0] == "OK", got[0]
+    kappa = np.linalg.cond(a)
+    assert np.abs(got[0][1][0][:, 0] - xo).max() <= 1e-10 * kappa * max(1.0, np.abs(xo).max())
+    xs = np.linalg.solve(a, b)
+    assert got[1][0] == "OK", got[1]
+    assert np.abs(got[1][1][0][:, 0] - xs).max() <= 1e-10 * kappa * max(1.0, np.abs(xs).max())

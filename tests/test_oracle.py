@@ -475,3 +475,112 @@ def test_gmres_cycle_arithmetic_and_x0():
     assert conv and it == 0 and res < 1e-10
     x, it, res, conv = oracle.gmres(a, b, restart=n)
     assert conv and it <= n
+
+
+# ------------------------------------- right-preconditioned GMRES (pgmres) ---
+# The oracle's restatement of gmres (iterative.rs:317-450) with right
+# preconditioning, the real counterpart of preconditioners.rs:934-1010 (whose
+# body never solves the least-squares problem, so it cannot serve as a check).
+def _row_scaled(n, seed):
+    """D (I + E) with D log-spaced over 1e-3 .. 1e3: hard for plain GMRES,
+    nearly the identity after Jacobi scaling."""
+    rng = np.random.default_rng(seed)
+    d = np.logspace(-3, 3, n)
+    return d[:, None] * (np.eye(n) + 0.3 * rng.standard_normal((n, n)) / np.sqrt(n)), rng.standard_normal(n)
+
+
+def test_pgmres_without_preconditioner_is_gmres():
+    rng = np.random.default_rng(4)
+    n = 80
+    a = rng.standard_normal((n, n)) / np.sqrt(n) + 1.5 * np.eye(n)
+    b = rng.standard_normal(n)
+    x0, it0, r0, c0 = oracle.gmres(a, b, max_iter=300, restart=20)
+    x1, it1, r1, c1 = oracle.pgmres(a, b, 0, max_iter=300, restart=20)
+    assert (it0, c0) == (it1, c1) and np.array_equal(x0, x1) and r0 == r1
+
+
+@pytest.mark.parametrize("n", [30, 120])
+def test_pgmres_jacobi_matches_lapack_and_beats_plain(n):
+    a, b = _row_scaled(n, n)
+    x, it, res, conv = oracle.pgmres(a, b, 1, max_iter=2000, tol=1e-10, restart=30)
+    xs = np.linalg.solve(a, b)
+    assert conv
+    # the reported residual is that of A x = b (right preconditioning), not of M^-1 A
+    # (to the rounding of a residual: 8 n u (||A|| ||x|| + ||b||))
+    rb = 8 * n * 2.0 ** -53 * (np.linalg.norm(a, 2) * np.linalg.norm(x) + np.linalg.norm(b))
+    assert abs(res - np.linalg.norm(b - a @ x)) <= rb
+    kappa = np.linalg.cond(a)
+    assert np.abs(x - xs).max() <= 1e-10 * kappa * max(1.0, np.abs(xs).max())
+    # the reference's loop tests convergence only at the end of a cycle, so
+    # iteration counts move in whole cycles: short cycles tell them apart
+    _, itp, _, cp = oracle.pgmres(a, b, 1, max_iter=5000, tol=1e-10, restart=10)
+    _, it0, _, c0 = oracle.gmres(a, b, max_iter=5000, tol=1e-10, restart=10)
+    assert cp and (it0 > 2 * itp or not c0), (itp, it0, c0)
+
+
+def test_pgmres_exact_lu_takes_one_cycle():
+    rng = np.random.default_rng(9)
+    n = 150
+    a = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    lu, piv, _ = oracle.lu_factor(a)
+    x, it, res, conv = oracle.pgmres(a, b, 3, lu=lu, perm=piv, max_iter=300, tol=1e-9, restart=30)
+    assert conv and it <= 30
+    assert np.abs(x - np.linalg.solve(a, b)).max() <= 1e-9 * np.linalg.cond(a)
+    # an FP32 LU preconditioner still solves to FP64 accuracy, in far fewer
+    # iterations than no preconditioner
+    lu32, piv32, _ = oracle.lu_factor(a.astype(np.float32))
+    x, it, res, conv = oracle.pgmres(a, b, 3, lu=lu32.astype(np.float64), perm=piv32, max_iter=300, tol=1e-11,
+                                     restart=30)
+    _, it0, _, c0 = oracle.gmres(a, b, max_iter=300, tol=1e-11, restart=10)
+    _, itp, _, cp = oracle.pgmres(a, b, 3, lu=lu32.astype(np.float64), perm=piv32, max_iter=300, tol=1e-11,
+                                  restart=10)
+    assert conv and cp and (not c0 or itp < it0), (itp, it0, c0)
+
+
+def _py_pgmres(a, b, minv, restart, max_iter, tol, mutation=None):
+    """Pure-Python right-preconditioned GMRES used only to plant mistakes in."""
+    n = a.shape[0]
+    x = np.zeros(n)
+    total = 0
+    for _ in range(max(max_iter // restart, 1)):
+        r = b - a @ x
+        beta = np.linalg.norm(r)
+        if beta < tol:
+            break
+        V = [r / beta]
+        H = np.zeros((restart + 1, restart))
+        j = 0
+        while j < restart and total < max_iter:
+            z = V[j] if mutation == "no_prec_in_arnoldi" else minv(V[j])
+            w = a @ z
+            wo = w.copy()
+            for i in range(j + 1):
+                H[i, j] = w @ V[i]
+                wo -= H[i, j] * V[i]
+            H[j + 1, j] = np.linalg.norm(wo)
+            if H[j + 1, j] < np.finfo(float).eps:
+                j += 1
+                break
+            V.append(wo / H[j + 1, j])
+            j += 1
+            total += 1
+        g = np.zeros(j + 1)
+        g[0] = beta
+        y = np.linalg.lstsq(H[:j + 1, :j], g, rcond=None)[0]
+        t = sum(y[i] * V[i] for i in range(j))
+        x = x + (t if mutation == "no_prec_in_update" else minv(t))
+    return x
+
+
+@pytest.mark.parametrize("mutation", ["no_prec_in_arnoldi", "no_prec_in_update"])
+def test_pgmres_checks_reject_planted_mistakes(mutation):
+    """The LAPACK check above catches a preconditioner left out of the Arnoldi
+    step or out of the update (the unmutated Python copy passes it)."""
+    a, b = _row_scaled(60, 3)
+    dinv = 1.0 / np.diag(a)
+    xs = np.linalg.solve(a, b)
+    ok = _py_pgmres(a, b, lambda v: dinv * v, 30, 600, 1e-10)
+    assert np.abs(ok - xs).max() <= 1e-8 * max(1.0, np.abs(xs).max())
+    bad = _py_pgmres(a, b, lambda v: dinv * v, 30, 600, 1e-10, mutation)
+    assert not np.abs(bad - xs).max() <= 1e-8 * max(1.0, np.abs(xs).max())
