@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 LU comparison line")
     ap.add_argument("--cpu-n", type=int, default=4096, help="size of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the host-memory (PCIe-inclusive) solve")
     ap.add_argument("--workload", default="mixed", choices=["mixed", "batched", "fp64", "next"],
                     help="mixed: the headline n=16384 mixed-precision solve; batched: 20,000 uneven "
                          "systems in one launch against a per-matrix launch loop")
@@ -128,27 +129,61 @@ def time_steps(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps, infos
 
 
-def cpu_baseline(n_sample, seed):
-    """The oracle (plain-C restatement of the reference solve, one thread)."""
+def cpu_baseline(n_sample, seed, sweep=(512, 1024, 2048), lapack_n=16384):
+    """The oracle (plain-C restatement of the reference solve, one thread) timed
+    on a sweep of n up to n_sample, the c n^3 fit extrapolated to the headline
+    n = 16384 (labelled as an extrapolation), and the host's LAPACK (numpy's
+    dgesv on every core it is given) at lapack_n, separately labelled."""
+    import os
     import numpy as np
     from oracle import oracle
     rng = np.random.default_rng(seed)
-    a = rng.standard_normal((n_sample, n_sample))
-    b = rng.standard_normal(n_sample)
-    oracle.solve(a[:64, :64], b[:64])   # load / build
-    t0 = time.perf_counter()
-    oracle.solve(a, b)
-    t = time.perf_counter() - t0
-    flops = 2.0 * n_sample ** 3 / 3.0
-    return {
-        "value": flops / t / 1e12,
+    oracle.solve(np.eye(64) + 0.1, np.ones(64))   # load / build
+    sizes = sorted(set([m for m in sweep if m < n_sample] + [n_sample]))
+    points = []
+    for m in sizes:
+        a = rng.standard_normal((m, m))
+        b = rng.standard_normal(m)
+        t0 = time.perf_counter()
+        oracle.solve(a, b)
+        points.append((m, time.perf_counter() - t0))
+    m_big, t_big = points[-1]
+    flops = 2.0 * m_big ** 3 / 3.0
+    # least-squares c in t = c n^3 over the two largest sizes (the smaller ones
+    # are cache-resident and faster per flop)
+    tail = points[-2:]
+    c = sum(t * m ** 3 for m, t in tail) / sum(float(m) ** 6 for m, _ in tail)
+    t_extra = c * 16384.0 ** 3
+    out = {
+        "value": flops / t_big / 1e12,
         "unit": "TFLOP/s",
         "cores": 1,
         "kind": "port",
         "sample": f"oracle solve (lapack.rs:72-147 + solve.rs:54-111 restated in C, 1 thread) of one "
-                  f"{n_sample}x{n_sample} N(0,1) system, 1 rhs: {t:.2f} s",
-        "seconds": t,
+                  f"{m_big}x{m_big} N(0,1) system, 1 rhs: {t_big:.2f} s",
+        "seconds": t_big,
+        "sweep": [{"n": m, "seconds": round(t, 4), "tflops": 2.0 * m ** 3 / 3.0 / t / 1e12} for m, t in points],
+        "extrapolated_n16384": {"seconds": t_extra, "tflops": 2.0 * 16384.0 ** 3 / 3.0 / t_extra / 1e12,
+                                "model": f"t = c n^3 fitted on n = {tail[0][0]}, {tail[-1][0]} (extrapolation, "
+                                         "not a measurement)"},
     }
+    # host LAPACK, timed where it finishes in seconds (else at half the size)
+    threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
+    a = rng.standard_normal((2048, 2048))
+    t0 = time.perf_counter()
+    np.linalg.solve(a, a[:, 0])
+    t_probe = time.perf_counter() - t0
+    nl = lapack_n if t_probe * (lapack_n / 2048.0) ** 3 < 20.0 else lapack_n // 2
+    a = rng.standard_normal((nl, nl))
+    b = rng.standard_normal(nl)
+    t0 = time.perf_counter()
+    np.linalg.solve(a, b)
+    t_l = time.perf_counter() - t0
+    del a
+    out["host_lapack"] = {"n": nl, "seconds": t_l, "tflops": 2.0 * nl ** 3 / 3.0 / t_l / 1e12,
+                          "threads": threads, "kind": "numpy.linalg.solve (host LAPACK dgesv), not the reference",
+                          "cpu_count": os.cpu_count()}
+    return out
 
 
 def make_batch(count, seed, torch):
@@ -450,9 +485,27 @@ def main():
             "forward_error": float((x64 - x_true).abs().max() / x_true.abs().max()),
             "factor_ms": infos64[-1].time_factor_ms,
             "update_tflops": (upd["flops"] / (upd["total_ms"] * 1e-3) / 1e12) if upd["total_ms"] else None,
+            "roofline_update": ({"bound": "mfma", "kernel": "dgemm_fold_kernel (FP64 MFMA)", "unit": "TFLOP/s",
+                                 "achieved": upd["flops"] / (upd["total_ms"] * 1e-3) / 1e12,
+                                 "peak": PEAK_MFMA_TFLOPS["f64"],
+                                 "frac": upd["flops"] / (upd["total_ms"] * 1e-3) / 1e12 / PEAK_MFMA_TFLOPS["f64"]}
+                                if upd["total_ms"] else None),
             "kernel_ms": {k: round(v["total_ms"] / max(1, len(infos64p)), 3) for k, v in prof64.items()},
         }
         del lu, perm, x64
+
+    # ---- the same solve from host memory: A and b cross PCIe, x comes back
+    pcie = None
+    if not args.no_pcie:
+        a_h = a.cpu().numpy()
+        b_h = b.cpu().numpy()
+        L.gesv_ir(a_h, b_h, mode=mode, tol=tol)   # warm the host path
+        t0 = time.perf_counter()
+        x_h, info_h = L.gesv_ir(a_h, b_h, mode=mode, tol=tol)
+        t_h = time.perf_counter() - t0
+        pcie = {"ms": t_h * 1e3, "tflops": flops / t_h / 1e12, "path": info_h.path,
+                "note": "scirs_gpu_dgesv_ir on host arrays: H2D of A and b, solve, D2H of x (not the value)"}
+        del a_h, b_h, x_h
 
     # ---- roofline of the dominant kernel class of the mixed solve
     per_step = {k: v["total_ms"] / prof_steps for k, v in prof.items()}
@@ -472,6 +525,19 @@ def main():
     roof["avg_launch_ms"] = avg_ms
     roof["launches_per_step"] = d["launches"] / prof_steps
 
+    # the trailing update on the matrix cores: algorithmic flops (2 m n k per
+    # launch) and the work of the split products the MFMA pipe executes
+    u = prof.get("update")
+    roof_upd = None
+    if u and u["total_ms"] > 0:
+        alg = u["flops"] / (u["total_ms"] * 1e-3) / 1e12
+        mult = {"bf16x6": 6, "bf16x3": 3, "fp16x3": 3}.get(args.mode, 1)
+        peak = PEAK_MFMA_TFLOPS[MODE_PEAK[args.mode]]
+        roof_upd = {"bound": "mfma", "kernel": f"lowp_gemm ({args.mode})", "achieved_algorithmic": alg,
+                    "mfma_products_per_flop": mult, "achieved": alg * mult, "peak": peak, "unit": "TFLOP/s",
+                    "frac": alg * mult / peak,
+                    "note": "achieved = algorithmic rate x split products (the MFMA pipe's work); update kernels "
+                            "overlap the panel on the side stream, so the class total is their summed durations"}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n, args.seed)
@@ -509,6 +575,8 @@ def main():
         "fp64_lu": fp64,
         "speedup_vs_fp64_lu": (value / fp64["tflops"]) if fp64 else None,
         "roofline": roof,
+        "roofline_update": roof_upd,
+        "pcie_inclusive": pcie,
         "cpu_baseline": cpu,
     }
     if bwd > tol:
