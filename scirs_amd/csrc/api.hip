@@ -146,12 +146,13 @@ int Context::init(int dev)
                  xrj_b = rep * 2 * (size_t)kbmax * 2 * 8, flags_b = 64 * sizeof(int),
                  pairs_b = (size_t)work.max_panels * 4 * kbmax * sizeof(int),
                  npairs_b = (size_t)work.max_panels * sizeof(int),
-                 linv_b = 2 * 128 * 128 * sizeof(float);   // two FP32 L11^-1 (double-buffered)
+                 linv_b = 2 * 128 * 128 * sizeof(float),   // two FP32 L11^-1 (double-buffered)
+                 linv64_b = 128 * 128 * sizeof(double);
     const size_t tpc_b = (size_t)TSLU_MAXLEV * TSLU_MAXNODE * 65 * 8, tpn_b = (size_t)TSLU_MAXLEV * TSLU_MAXNODE * 4,
                  tpu_b = 64 * 64 * sizeof(float), tpr_b = 64 * 64 * sizeof(float),
                  tpt_b = 64 * (size_t)kbmax * sizeof(float), tpv_b = 65 * sizeof(int),
                  tpm_b = (2 * (size_t)TSLU_MAP + 1) * sizeof(int);
-    for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b, linv_b, tpc_b, tpn_b, tpu_b, tpr_b, tpt_b, tpt_b,
+    for (size_t b : {xw_b, xrow_b, xrj_b, flags_b, pairs_b, npairs_b, linv_b, linv64_b, tpc_b, tpn_b, tpu_b, tpr_b, tpt_b, tpt_b,
                      tpv_b, tpm_b})
         bytes += (b + 255) & ~(size_t)255;
     char *base = nullptr;
@@ -173,6 +174,7 @@ int Context::init(int dev)
     work.pairs = (int *)take(pairs_b);
     work.npairs = (int *)take(npairs_b);
     work.linv = take(linv_b);
+    work.linv64 = (double *)take(linv64_b);
     work.tp_cand = (unsigned long long *)take(tpc_b);
     work.tp_cnt = (int *)take(tpn_b);
     work.tp_uinv = (float *)take(tpu_b);

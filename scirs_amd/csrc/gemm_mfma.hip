@@ -194,7 +194,11 @@ __global__ __launch_bounds__(NTHREADS, CPRE ? 1 : 2) void gemm_sub_kernel(GemmAr
 // work instead of preceding it (the memory counter retires loads in order: a
 // wait for the slab operands never waits for C).  Operands come in as 16-byte
 // loads; B goes to LDS as 16-byte stores.
-template <int KS, bool VEC, int OCC, bool TRANS_B>
+// ASSIGN: C = A op(B) instead (C is not read; A enters LDS unnegated): the
+// Cholesky TRSM L21 = A21 L11^-T, in place — a workgroup reads all K columns of
+// its rows before it writes them, and with N <= 128 no other workgroup reads
+// those rows.
+template <int KS, bool VEC, int OCC, bool TRANS_B, bool ASSIGN = false>
 __global__ __launch_bounds__(NTHREADS, OCC) void dgemm_fold_kernel(GemmArgs<double> g)
 {
     static_assert(KS >= 8 && KS % 8 == 0, "eight half block rows of C folded over the slabs");
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(NTHREADS, OCC) void dgemm_fold_kernel(GemmArgs<doub
     };
     auto store_slab = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) As[buf][(a_k + i) * LDA_S + a_r] = -ra[i];
+        for (int i = 0; i < 8; ++i) As[buf][(a_k + i) * LDA_S + a_r] = ASSIGN ? ra[i] : -ra[i];
         if (TRANS_B) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) Bs[buf][(a_k + i) * LDB_S + a_r] = rb[i];
@@ -306,8 +310,10 @@ __global__ __launch_bounds__(NTHREADS, OCC) void dgemm_fold_kernel(GemmArgs<doub
         if (s + 1 < KS) load_slab((s + 1) * BK);
         // C half row h: loaded during slab h * KS / 8, folded in the next slab
         constexpr int STEP = KS / 8;
-        if (s >= STEP && s % STEP == 0) fold_c(s / STEP - 1);
-        if (s % STEP == 0) load_c(s / STEP);
+        if (!ASSIGN) {
+            if (s >= STEP && s % STEP == 0) fold_c(s / STEP - 1);
+            if (s % STEP == 0) load_c(s / STEP);
+        }
         // keep the loads ahead of the matrix-core work and the LDS stores behind it
         __builtin_amdgcn_sched_barrier(0);
         const double *as = As[buf];
@@ -331,8 +337,8 @@ __global__ __launch_bounds__(NTHREADS, OCC) void dgemm_fold_kernel(GemmArgs<doub
         if (s + 1 < KS) store_slab(buf ^ 1);
         __syncthreads();
     }
-    fold_c(7);
-    // epilogue: C = C - A B, already in the accumulators
+    if (!ASSIGN) fold_c(7);
+    // epilogue: C = C - A B (or A B), already in the accumulators
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -406,6 +412,31 @@ int scirs_launch_dgemm_sub(int M, int N, int K, const double *A, long lda, const
                            hipStream_t stream)
 {
     return launch_gemm_sub<double>(M, N, K, A, lda, B, ldb, trans_b, C, ldc, lower, info, stream);
+}
+
+int scirs_launch_dgemm_assign_t(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                                double *C, long ldc, const int *info, hipStream_t stream)
+{
+    if (M <= 0 || N <= 0) return SCIRS_OK;
+    if (K != 8 * BK || N > BN) {
+        scirs_set_error("dgemm_assign_t: K must be %d and N at most %d (got K=%d N=%d)", 8 * BK, BN, K, N);
+        return SCIRS_ERR_ARG;
+    }
+    GemmArgs<double> g;
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+    g.tiles_m = (M + BM - 1) / BM;
+    g.tiles_n = 1;
+    g.lower = 0;
+    g.info = info;
+    const int nwg = (g.tiles_m + 7) / 8 * 8;
+    const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0;
+    if (vec)
+        hipLaunchKernelGGL((dgemm_fold_kernel<8, true, 2, true, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    else
+        hipLaunchKernelGGL((dgemm_fold_kernel<8, false, 1, true, true>), dim3(nwg), dim3(NTHREADS), 0, stream, g);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
 }
 
 int scirs_launch_sgemm_f32_sub(int M, int N, int K, const float *A, long lda, const float *B, long ldb,

@@ -8,6 +8,10 @@
 int scirs_launch_dgemm_sub(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
                            int trans_b, double *C, long ldc, int lower, const int *info,
                            hipStream_t stream);
+// C = A B^T for the Cholesky TRSM: C[M x N] = A[M x 128] B^T with B N x 128,
+// N <= 128; C may alias A (each workgroup owns whole rows).
+int scirs_launch_dgemm_assign_t(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                                double *C, long ldc, const int *info, hipStream_t stream);
 
 // Device scratch used by the factorisations (allocated once per context).
 struct ScirsWork {
@@ -31,6 +35,7 @@ struct ScirsWork {
     size_t xtri_words;
     int *ticket;         // chained triangular solve: block-row ticket counter
     void *linv;          // 128 x 128 inverse of a panel's unit-lower block (FP32 TRSM)
+    double *linv64;      // 128 x 128 inverse of the Cholesky diagonal block (FP64 TRSM as a product)
     int panel_w0;        // panel critical path on wave 0 alone (1) or on all waves (0)
     int *colexp;         // FP16 update: per-column exponents of U12, two regions (main / side stream)
     int colexp_half;     // columns per region

@@ -10,6 +10,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include <algorithm>
+#include <cstdlib>
+#include <type_traits>
 
 namespace {
 constexpr int NT = 256;
@@ -243,6 +245,84 @@ __global__ __launch_bounds__(NT) void trsm_rltn_kernel(T *A, long lda, int k0, i
     }
 }
 
+// Linv = L11^-1 of the 128 x 128 FP64 diagonal block (lower, non-unit), for
+// the TRSM as a product L21 = A21 Linv^T.  Blocks of 32:
+//   1. wave w inverts diagonal block w, lane c computing column c by forward
+//      substitution in registers (true division by the diagonal);
+//   2. block rows i = 1..3 in order: T_j = sum_{k=j}^{i-1} L_ik Linv_kj for
+//      every j < i (all 256 threads, four columns per thread), then
+//      Linv_ij = -Linv_ii T_j;
+//   3. Linv (lower, zeros above) to global, row stride 128.
+// The whole inverse lives in LDS; L is read from global memory (cached).
+constexpr int TI64 = 128, TB64 = 32, XP = TI64 + 1, TP = TB64 + 1;
+__global__ __launch_bounds__(NT) void trtri_lower64_kernel(const double *L, long ldl, double *Linv, const int *info)
+{
+    if (*info != 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *X = reinterpret_cast<double *>(smem_raw);   // 128 x XP
+    double *Ts = X + TI64 * XP;                          // 3 x 32 x TP
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    {
+        const double *D = L + (long)(TB64 * wave) * ldl + TB64 * wave;
+        const int c = lane;
+        double x[TB64];
+#pragma unroll
+        for (int i = 0; i < TB64; ++i) {
+            double sacc = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+            for (int j = 0; j < i; ++j) sacc -= D[(long)i * ldl + j] * x[j];
+            x[i] = sacc / D[(long)i * ldl + i];
+        }
+        if (c < TB64)
+#pragma unroll
+            for (int i = 0; i < TB64; ++i) X[(TB64 * wave + i) * XP + TB64 * wave + c] = x[i];
+    }
+    __syncthreads();
+    for (int i = 1; i < TI64 / TB64; ++i) {
+        // (a) T_j for j < i: thread strip = (j, row r, 4 columns 4 c4)
+        for (int sidx = t; sidx < i * TB64 * (TB64 / 4); sidx += NT) {
+            const int j = sidx / (TB64 * TB64 / 4), rem = sidx % (TB64 * TB64 / 4);
+            const int r = rem / (TB64 / 4), c4 = rem % (TB64 / 4);
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            const double *Lr = L + (long)(TB64 * i + r) * ldl;
+            for (int k = j; k < i; ++k) {
+#pragma unroll 8
+                for (int q = 0; q < TB64; ++q) {
+                    const double a = Lr[TB64 * k + q];
+                    const double *xr = X + (TB64 * k + q) * XP + TB64 * j + 4 * c4;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[e] = fma(a, xr[e], acc[e]);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Ts[(j * TB64 + r) * TP + 4 * c4 + e] = acc[e];
+        }
+        __syncthreads();
+        // (b) Linv_ij = -Linv_ii T_j
+        for (int sidx = t; sidx < i * TB64 * (TB64 / 4); sidx += NT) {
+            const int j = sidx / (TB64 * TB64 / 4), rem = sidx % (TB64 * TB64 / 4);
+            const int r = rem / (TB64 / 4), c4 = rem % (TB64 / 4);
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            const double *dr = X + (TB64 * i + r) * XP + TB64 * i;
+#pragma unroll 8
+            for (int q = 0; q <= r; ++q) {
+                const double a = dr[q];
+                const double *tr = Ts + (j * TB64 + q) * TP + 4 * c4;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = fma(a, tr[e], acc[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) X[(TB64 * i + r) * XP + TB64 * j + 4 * c4 + e] = -acc[e];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < TI64 * TI64; e += NT) {
+        const int r = e / TI64, c = e % TI64;
+        Linv[e] = c <= r ? X[r * XP + c] : 0.0;
+    }
+}
+
 template <typename T>
 __global__ void zero_upper_kernel(T *A, long lda, int n)
 {
@@ -274,10 +354,28 @@ int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t 
                            A, lda, k0, kbw, w.info, w.info_val);
         const int rest = n - k0 - kbw;
         if (rest > 0) {
-            const size_t lds = ((size_t)kbw * (kbw + 1) / 2 + (size_t)kbw * 65) * sizeof(T);
-            if (int rc = scirs_allow_lds((const void *)trsm_rltn_kernel<T>, lds)) return rc;
-            hipLaunchKernelGGL(trsm_rltn_kernel<T>, dim3((rest + 63) / 64), dim3(NT), lds, stream,
-                               A, lda, k0, kbw, k0 + kbw, rest, (const int *)w.info);
+            static const bool inv_trsm = [] {
+                const char *v = getenv("SCIRS_GPU_CHOL_INV_TRSM");
+                return !(v && *v && atoi(v) == 0);
+            }();
+            if (std::is_same<T, double>::value && kbw == TI64 && w.linv64 && inv_trsm) {
+                // L21 = A21 L11^-T as a product on the matrix cores: L11^-1 by one
+                // workgroup, then one 128-row tile of A21 per workgroup, in place
+                const size_t lds_i = ((size_t)TI64 * XP + 3 * TB64 * TP) * sizeof(double);
+                if (int rc = scirs_allow_lds((const void *)trtri_lower64_kernel, lds_i)) return rc;
+                hipLaunchKernelGGL(trtri_lower64_kernel, dim3(1), dim3(NT), lds_i, stream,
+                                   (const double *)(A + (long)k0 * lda + k0), lda, w.linv64, (const int *)w.info);
+                SCIRS_HIP_CHECK(hipGetLastError());
+                double *A21 = (double *)(A + (long)(k0 + kbw) * lda + k0);
+                if (int rc = scirs_launch_dgemm_assign_t(rest, TI64, TI64, A21, lda, w.linv64, TI64, A21, lda,
+                                                         (const int *)w.info, stream))
+                    return rc;
+            } else {
+                const size_t lds = ((size_t)kbw * (kbw + 1) / 2 + (size_t)kbw * 65) * sizeof(T);
+                if (int rc = scirs_allow_lds((const void *)trsm_rltn_kernel<T>, lds)) return rc;
+                hipLaunchKernelGGL(trsm_rltn_kernel<T>, dim3((rest + 63) / 64), dim3(NT), lds, stream,
+                                   A, lda, k0, kbw, k0 + kbw, rest, (const int *)w.info);
+            }
             const T *L21 = A + (long)(k0 + kbw) * lda + k0;
             T *A22 = A + (long)(k0 + kbw) * lda + k0 + kbw;
             const int nn = lookahead ? std::min(kb, rest) : rest;   // columns updated on the main stream
