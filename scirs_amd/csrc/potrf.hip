@@ -48,11 +48,15 @@ __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int k
     if (*info != 0) return;
     constexpr int PB = 16;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *S = reinterpret_cast<T *>(smem_raw);   // kb x (kb+1)
-    const int p = kb + 1;
+    // the lower triangle, packed by rows (row r at r (r + 1) / 2): half the LDS
+    // of a square tile, so the kernel fits on a CU beside one workgroup of the
+    // trailing update running on the other stream
+    T *S = reinterpret_cast<T *>(smem_raw);
+    auto pk = [](int r, int c) { return r * (r + 1) / 2 + c; };
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     T *D = A + (long)k0 * lda + k0;
-    g2s_block<T, 16>(S, p, D, lda, kb, kb, kb, kb);   // (the strict upper part is never used)
+    for (int i = wave; i < kb; i += NT / 64)
+        for (int j = lane; j <= i; j += 64) S[pk(i, j)] = D[(long)i * lda + j];
     __shared__ int s_fail;
     if (t == 0) s_fail = 0;
     __syncthreads();
@@ -65,7 +69,7 @@ __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int k
 #pragma unroll
                 for (int i = 0; i < PB; ++i) {
                     const int row = lane + 64 * sl;
-                    a[sl][i] = (row < kb && i < pw) ? S[row * p + jb + i] : (T)0;
+                    a[sl][i] = (row < kb && i < pw && jb + i <= row) ? S[pk(row, jb + i)] : (T)0;
                 }
             int fail_row = -1;
             T fail_val = (T)0;
@@ -103,7 +107,7 @@ __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int k
 #pragma unroll
                 for (int i = 0; i < PB; ++i) {
                     const int row = lane + 64 * sl;
-                    if (row < kb && i < pw && row >= jb) S[row * p + jb + i] = a[sl][i];
+                    if (row < kb && i < pw && row >= jb + i) S[pk(row, jb + i)] = a[sl][i];
                 }
             if (fail_row >= 0 && lane == 0) {
                 s_fail = 1;
@@ -135,8 +139,8 @@ __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int k
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const bool kk = k + q < pw;
-                        pr[x][q] = (kk && r0 + x < kb) ? S[(r0 + x) * p + jb + k + q] : (T)0;
-                        pc[x][q] = (kk && c0 + x < kb) ? S[(c0 + x) * p + jb + k + q] : (T)0;
+                        pr[x][q] = (kk && r0 + x < kb) ? S[pk(r0 + x, jb + k + q)] : (T)0;
+                        pc[x][q] = (kk && c0 + x < kb) ? S[pk(c0 + x, jb + k + q)] : (T)0;
                     }
 #pragma unroll
                 for (int x = 0; x < 4; ++x)
@@ -150,13 +154,13 @@ __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int k
 #pragma unroll
                 for (int y = 0; y < 4; ++y) {
                     const int r = r0 + x, c = c0 + y;
-                    if (r < kb && c <= r) S[r * p + c] -= acc[x][y];
+                    if (r < kb && c <= r) S[pk(r, c)] -= acc[x][y];
                 }
         }
         __syncthreads();
     }
     for (int i = wave; i < kb; i += NT / 64)
-        for (int j = lane; j <= i; j += 64) D[(long)i * lda + j] = S[i * p + j];
+        for (int j = lane; j <= i; j += 64) D[(long)i * lda + j] = S[pk(i, j)];
 }
 
 // L21 = A21 L11^-T for rows [r0, r0 + nrows): each row x solves x L11^T = a,
@@ -249,18 +253,21 @@ __global__ __launch_bounds__(NT) void trsm_rltn_kernel(T *A, long lda, int k0, i
 // the TRSM as a product L21 = A21 Linv^T.  Blocks of 32:
 //   1. wave w inverts diagonal block w, lane c computing column c by forward
 //      substitution in registers (true division by the diagonal);
-//   2. block rows i = 1..3 in order: T_j = sum_{k=j}^{i-1} L_ik Linv_kj for
-//      every j < i (all 256 threads, four columns per thread), then
-//      Linv_ij = -Linv_ii T_j;
+//   2. block rows i = 1..3 in order, for each j < i:
+//      T = sum_{k=j}^{i-1} L_ik Linv_kj, then Linv_ij = -Linv_ii T
+//      (all 256 threads, one row x 4 columns of the 32 x 32 block each);
 //   3. Linv (lower, zeros above) to global, row stride 128.
-// The whole inverse lives in LDS; L is read from global memory (cached).
-constexpr int TI64 = 128, TB64 = 32, XP = TI64 + 1, TP = TB64 + 1;
+// The inverse's ten lower blocks and one T block live in LDS (88 KiB, so the
+// kernel fits on a CU beside one workgroup of the trailing update running on
+// the other stream); L is read from global memory (cached).
+constexpr int TI64 = 128, TB64 = 32;
+__device__ __forceinline__ int blk64(int i, int j) { return (i * (i + 1) / 2 + j) * TB64 * TB64; }
 __global__ __launch_bounds__(NT) void trtri_lower64_kernel(const double *L, long ldl, double *Linv, const int *info)
 {
     if (*info != 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double *X = reinterpret_cast<double *>(smem_raw);   // 128 x XP
-    double *Ts = X + TI64 * XP;                          // 3 x 32 x TP
+    double *X = reinterpret_cast<double *>(smem_raw);   // 10 blocks of 32 x 32
+    double *Ts = X + 10 * TB64 * TB64;                   // 32 x 32
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     {
@@ -276,50 +283,42 @@ __global__ __launch_bounds__(NT) void trtri_lower64_kernel(const double *L, long
         }
         if (c < TB64)
 #pragma unroll
-            for (int i = 0; i < TB64; ++i) X[(TB64 * wave + i) * XP + TB64 * wave + c] = x[i];
+            for (int i = 0; i < TB64; ++i) X[blk64(wave, wave) + i * TB64 + c] = x[i];
     }
     __syncthreads();
+    const int r = t >> 3, c4 = t & 7;   // this thread's row and four columns of a block
     for (int i = 1; i < TI64 / TB64; ++i) {
-        // (a) T_j for j < i: thread strip = (j, row r, 4 columns 4 c4)
-        for (int sidx = t; sidx < i * TB64 * (TB64 / 4); sidx += NT) {
-            const int j = sidx / (TB64 * TB64 / 4), rem = sidx % (TB64 * TB64 / 4);
-            const int r = rem / (TB64 / 4), c4 = rem % (TB64 / 4);
+        const double *Lr = L + (long)(TB64 * i + r) * ldl;
+        for (int j = 0; j < i; ++j) {
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
-            const double *Lr = L + (long)(TB64 * i + r) * ldl;
             for (int k = j; k < i; ++k) {
+                const double *xk = X + blk64(k, j) + 4 * c4;
 #pragma unroll 8
                 for (int q = 0; q < TB64; ++q) {
                     const double a = Lr[TB64 * k + q];
-                    const double *xr = X + (TB64 * k + q) * XP + TB64 * j + 4 * c4;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) acc[e] = fma(a, xr[e], acc[e]);
+                    for (int e = 0; e < 4; ++e) acc[e] = fma(a, xk[q * TB64 + e], acc[e]);
                 }
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Ts[(j * TB64 + r) * TP + 4 * c4 + e] = acc[e];
-        }
-        __syncthreads();
-        // (b) Linv_ij = -Linv_ii T_j
-        for (int sidx = t; sidx < i * TB64 * (TB64 / 4); sidx += NT) {
-            const int j = sidx / (TB64 * TB64 / 4), rem = sidx % (TB64 * TB64 / 4);
-            const int r = rem / (TB64 / 4), c4 = rem % (TB64 / 4);
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-            const double *dr = X + (TB64 * i + r) * XP + TB64 * i;
-#pragma unroll 8
+            for (int e = 0; e < 4; ++e) Ts[r * TB64 + 4 * c4 + e] = acc[e];
+            __syncthreads();
+            const double *dr = X + blk64(i, i) + r * TB64;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[e] = 0.0;
             for (int q = 0; q <= r; ++q) {
                 const double a = dr[q];
-                const double *tr = Ts + (j * TB64 + q) * TP + 4 * c4;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) acc[e] = fma(a, tr[e], acc[e]);
+                for (int e = 0; e < 4; ++e) acc[e] = fma(a, Ts[q * TB64 + 4 * c4 + e], acc[e]);
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) X[(TB64 * i + r) * XP + TB64 * j + 4 * c4 + e] = -acc[e];
+            for (int e = 0; e < 4; ++e) X[blk64(i, j) + r * TB64 + 4 * c4 + e] = -acc[e];
+            __syncthreads();
         }
-        __syncthreads();
     }
     for (int e = t; e < TI64 * TI64; e += NT) {
-        const int r = e / TI64, c = e % TI64;
-        Linv[e] = c <= r ? X[r * XP + c] : 0.0;
+        const int rr = e / TI64, cc = e % TI64;
+        Linv[e] = cc <= rr ? X[blk64(rr / TB64, cc / TB64) + (rr % TB64) * TB64 + cc % TB64] : 0.0;
     }
 }
 
@@ -349,8 +348,9 @@ int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t 
     }
     for (int k0 = 0, p = 0; k0 < n; k0 += kb, ++p) {
         const int kbw = std::min(kb, n - k0);
-        if (int rc = scirs_allow_lds((const void *)potf2_kernel<T>, (size_t)kbw * (kbw + 1) * sizeof(T))) return rc;
-        hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(NT), (size_t)kbw * (kbw + 1) * sizeof(T), stream,
+        const size_t lds_p = (size_t)kbw * (kbw + 1) / 2 * sizeof(T);
+        if (int rc = scirs_allow_lds((const void *)potf2_kernel<T>, lds_p)) return rc;
+        hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(NT), lds_p, stream,
                            A, lda, k0, kbw, w.info, w.info_val);
         const int rest = n - k0 - kbw;
         if (rest > 0) {
@@ -361,7 +361,7 @@ int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t 
             if (std::is_same<T, double>::value && kbw == TI64 && w.linv64 && inv_trsm) {
                 // L21 = A21 L11^-T as a product on the matrix cores: L11^-1 by one
                 // workgroup, then one 128-row tile of A21 per workgroup, in place
-                const size_t lds_i = ((size_t)TI64 * XP + 3 * TB64 * TP) * sizeof(double);
+                const size_t lds_i = (size_t)11 * TB64 * TB64 * sizeof(double);
                 if (int rc = scirs_allow_lds((const void *)trtri_lower64_kernel, lds_i)) return rc;
                 hipLaunchKernelGGL(trtri_lower64_kernel, dim3(1), dim3(NT), lds_i, stream,
                                    (const double *)(A + (long)k0 * lda + k0), lda, w.linv64, (const int *)w.info);
