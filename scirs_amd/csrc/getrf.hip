@@ -27,6 +27,8 @@
 #include "prof.hpp"
 #include "api_internal.hpp"
 #include <algorithm>
+#include <cfloat>
+#include <cstdlib>
 #include <type_traits>
 
 namespace {
@@ -425,6 +427,155 @@ __global__ __launch_bounds__(NT_LT) void laswp_trsm_kernel(T *__restrict__ A, lo
         for (int i = wave; i < kb; i += NW) A[(long)(k0 + i) * lda + c] = Xs[i * 64 + lane];
 }
 
+// ---- register-resident FP64 panel for short panels (m <= 512 rows, kb <= 32)
+// One workgroup of 512 threads (two waves per SIMD, which hide each other's
+// latencies); thread t holds the row that starts at position t in registers.
+// Rows never move: a position register per row follows the interchanges
+// (lapack.rs:124-131 swaps positions j and p), and the search of
+// scirs2-linalg/src/lapack.rs:95-104 (largest |a_ij| over positions >= j, the
+// first position on ties) is a wave arg-max plus an eight-way pick.  The
+// column loop is a real loop, not unrolled: every row rotates left by one
+// column per step (the update writes column c + 1 into register c, and the
+// step's multiplier goes to the last register), so the pivot column is always
+// register 0 and every step runs the same code from the instruction cache.
+// Per column one barrier: each wave's candidate publishes its row
+// (double-buffered by parity); the rows below divide by the pivot (true
+// division, lapack.rs:138) and subtract their multiple (lapack.rs:140-142) —
+// a row that does not take part, or a column that already holds a
+// multiplier, gets a zero factor, which leaves it exactly unchanged.  The
+// (destination, source) list of the moved rows comes from the final
+// positions.
+constexpr int GR_W = 32, GR_NT = 512, GR_MMAX = GR_NT, GR_NW = GR_NT / 64;
+struct GrXch {
+    double slot[2][GR_NW][GR_W];
+    __attribute__((aligned(16))) double cv[2][GR_NW];
+    __attribute__((aligned(16))) int ci[2][GR_NW];
+    int label[GR_MMAX];
+};
+
+__global__ __launch_bounds__(GR_NT) void getf2_regs_kernel(double *A, long lda, int k0, int m, int kb, int *pairs,
+                                                           int *npairs, int *info, double *info_val)
+{
+    if (*info != 0) return;
+    __shared__ GrXch X;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    double *P = A + (long)k0 * lda + k0;
+    double a[GR_W];
+    int pos = t;
+    {
+        const double *src = P + (long)min(pos, m - 1) * lda;
+#pragma unroll
+        for (int c = 0; c < GR_W; ++c) a[c] = (pos < m && c < kb) ? src[c] : 0.0;
+        if (pos >= m) pos = GR_MMAX;   // no row: never a candidate, never updated
+    }
+    bool stop = false;
+#pragma unroll 1
+    for (int j = 0; j < GR_W; ++j) {
+        const int par = j & 1;
+        const bool live = j < kb && !stop;   // uniform
+        double l = 0.0;
+        double pr[GR_W];
+        bool below = false;
+        if (live) {
+            double bv = 0.0;
+            int bi = -1;
+            if (pos >= j && pos < m) { bv = fabs(a[0]); bi = pos; }
+            // Fast wave arg-max on a 32-bit key: the top 22 bits of |v| (exponent
+            // and 10 mantissa bits), a valid bit and the inverted position.  It is
+            // exact unless another lane's |v| shares the winner's top 22 bits (then
+            // the exact double arg-max runs instead).
+            const unsigned hi = (unsigned)((unsigned long long)__double_as_longlong(bv) >> 32) & 0x7FFFFFFFu;
+            const unsigned key = bi < 0 ? 0u : ((hi & ~0x3FFu) | 0x200u | (unsigned)(GR_MMAX - 1 - bi));
+            unsigned km = key;
+            km = max(km, (unsigned)__builtin_amdgcn_update_dpp(0, (int)km, 0x111, 0xf, 0xf, false));
+            km = max(km, (unsigned)__builtin_amdgcn_update_dpp(0, (int)km, 0x112, 0xf, 0xf, false));
+            km = max(km, (unsigned)__builtin_amdgcn_update_dpp(0, (int)km, 0x114, 0xf, 0xf, false));
+            km = max(km, (unsigned)__builtin_amdgcn_update_dpp(0, (int)km, 0x118, 0xf, 0xf, false));
+            km = max(km, (unsigned)__builtin_amdgcn_update_dpp(0, (int)km, 0x142, 0xa, 0xf, false));
+            km = max(km, (unsigned)__builtin_amdgcn_update_dpp(0, (int)km, 0x143, 0xc, 0xf, false));
+            km = (unsigned)__builtin_amdgcn_readlane((int)km, 63);
+            const unsigned long long same = __ballot(bi >= 0 && (key & ~0x3FFu) == (km & ~0x3FFu));
+            if (km != 0u && __popcll(same) == 1) {
+                bi = GR_MMAX - 1 - (int)(km & 0x1FFu);
+                const int src = __builtin_amdgcn_readfirstlane(__ffsll((long long)same) - 1);
+                bv = readlane_val(bv, src);
+            } else {
+                wave_argmax(bv, bi);
+            }
+            if (bi >= 0 && pos == bi)
+#pragma unroll
+                for (int c = 0; c < GR_W; c += 2)
+                    *reinterpret_cast<double2 *>(&X.slot[par][wave][c]) = double2{a[c], a[c + 1]};
+            if (lane == 0) { X.cv[par][wave] = bv; X.ci[par][wave] = bi; }
+            __syncthreads();
+            int cis[GR_NW];
+            double cvs[GR_NW];
+#pragma unroll
+            for (int w = 0; w < GR_NW; w += 4) {
+                const int4 q = *reinterpret_cast<const int4 *>(&X.ci[par][w]);
+                cis[w] = q.x; cis[w + 1] = q.y; cis[w + 2] = q.z; cis[w + 3] = q.w;
+            }
+#pragma unroll
+            for (int w = 0; w < GR_NW; w += 2) {
+                const double2 q = *reinterpret_cast<const double2 *>(&X.cv[par][w]);
+                cvs[w] = q.x; cvs[w + 1] = q.y;
+            }
+            double v = cvs[0];
+            int p = cis[0], pw = 0;
+#pragma unroll
+            for (int w = 1; w < GR_NW; ++w)
+                if (cis[w] >= 0 && (p < 0 || cvs[w] > v || (cvs[w] == v && cis[w] < p))) {
+                    v = cvs[w]; p = cis[w]; pw = w;
+                }
+            if (!(p >= 0 && v >= DBL_EPSILON)) {   // |pivot| < eps: singular (lapack.rs:106-120)
+                if (t == 0) { *info = k0 + j + 1; *info_val = p >= 0 ? v : 0.0; }
+                stop = true;
+            } else {
+                pos = pos == p ? j : (pos == j ? p : pos);   // the interchange of positions j and p
+#pragma unroll
+                for (int c = 0; c < GR_W; c += 2) {
+                    const double2 q = *reinterpret_cast<const double2 *>(&X.slot[par][pw][c]);
+                    // registers past the trailing columns hold multipliers: factor 0
+                    pr[c] = c <= GR_W - 1 - j ? q.x : 0.0;
+                    pr[c + 1] = c + 1 <= GR_W - 1 - j ? q.y : 0.0;
+                }
+                below = pos > j && pos < m;
+                if (below) l = a[0] / pr[0];
+            }
+        }
+        if (!live || stop) {
+#pragma unroll
+            for (int c = 0; c < GR_W; ++c) pr[c] = 0.0;
+        }
+        // rotate left by one column, updating the trailing columns of the rows below
+        const double a0 = a[0];
+#pragma unroll
+        for (int c = 1; c < GR_W; ++c) a[c - 1] = fma(-l, pr[c], a[c]);
+        a[GR_W - 1] = below ? l : a0;
+    }
+    // rows in natural column order again after GR_W rotations; write each to its position
+    if (pos < m) {
+        double *dst = P + (long)pos * lda;
+#pragma unroll
+        for (int c = 0; c < GR_W; ++c)
+            if (c < kb) dst[c] = a[c];
+        X.label[pos] = t;
+    }
+    __syncthreads();
+    if (t < 64) {
+        int np = 0;
+        for (int base = 0; base < m; base += 64) {
+            const int q = base + t;
+            const bool moved = q < m && X.label[q] != q;
+            const unsigned long long bl = __ballot(moved);
+            const int at = np + __popcll(bl & ((1ull << t) - 1ull));
+            if (moved) { pairs[2 * at] = k0 + q; pairs[2 * at + 1] = k0 + X.label[q]; }
+            np += __popcll(bl);
+        }
+        if (t == 0) *npairs = np;
+    }
+}
+
 } // namespace
 
 int scirs_lu_panel_kb(int m, int kb, size_t elem_bytes)
@@ -498,7 +649,16 @@ int scirs_getrf_device(int m, int n, T *A, long lda, int kb, ScirsWork &w, hipSt
             if (int rc = scirs_tslu_panel((float *)A, lda, m, k0, kbw, p, w, stream)) return rc;
         } else {
             scirs::ProfScope ps(SCIRS_PROF_PANEL, (double)mrows * kbw * kbw, 2.0 * mrows * kbw * sizeof(T), stream);
-            if (G == 1) {
+            static const bool regs_panel = [] {
+                const char *v = getenv("SCIRS_GPU_REGS_PANEL");
+                return !(v && *v && atoi(v) == 0);
+            }();
+            if (std::is_same<T, double>::value && regs_panel && mrows <= GR_MMAX && kbw <= GR_W) {
+                // a short FP64 panel: rows in registers, one barrier per column
+                hipLaunchKernelGGL(getf2_regs_kernel, dim3(1), dim3(GR_NT), 0, stream, (double *)A, lda, k0, mrows,
+                                   kbw, pa.pairs, pa.npairs, pa.info, pa.info_val);
+                SCIRS_HIP_CHECK(hipGetLastError());
+            } else if (G == 1) {
                 // one workgroup: nothing to exchange, 512 threads share the rows
                 constexpr int NTS = 2 * panel::NT;
                 if (int rc = scirs_allow_lds((const void *)panel::getf2_panel_kernel<T, false, NTS>, lds)) return rc;
