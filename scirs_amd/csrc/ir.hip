@@ -4,20 +4,22 @@
 // (`iterative_refinement_solve`), which re-factors A in the working precision
 // on every step and stops on an absolute residual.  Here:
 //   1. A32 = fp32(A) fused with the row sums of |A| (||A||_inf);
-//   2. LU of A32: FP32 panel, trailing update in the requested arithmetic
-//      (BF16 / FP16-scaled / split-BF16 / FP32 matrix cores, FP32 accumulation);
+//   2. LU of A32: FP32 panel with tournament pivoting, trailing update in the
+//      requested arithmetic (BF16 / FP16-scaled / split-BF16 / split-FP16 /
+//      FP32 matrix cores, FP32 accumulation);
 //   3. refinement  R = B - A X (FP64, GPU), D = U^-1 L^-1 P R (FP32),
 //      X += D, stopping when the normwise backward error
 //      max_j ||R_j||_inf / (||A||_inf ||X_j||_inf + ||B_j||_inf) <= tol;
 //   4. when a step fails to halve the backward error, restarted GMRES (FP64
 //      Arnoldi with classical Gram-Schmidt applied twice, right-preconditioned
-//      by the low-precision factors) solves the correction equation A D = R
+//      by the low-precision factors upcast to FP64, up to 8 right-hand sides in
+//      lockstep) solves the correction equation A D = R
 //      (Carson & Higham, SISC 40(2), 2018, "GMRES-IR"); the Givens least-squares
 //      update is the one of scirs2-linalg/src/solvers/iterative.rs:604-664;
 //   5. when GMRES-IR also stalls, or the low-precision LU breaks down, FP64 LU
 //      solves the system (the breakdown is then decided the FP64 way).
 // FP64 is used on the GPU only for the residual, the update of X and the
-// stopping test (plus the Arnoldi vectors of GMRES).
+// stopping test (plus the Arnoldi vectors and the preconditioner of GMRES).
 #include "api_internal.hpp"
 
 #include <cmath>

@@ -8,18 +8,19 @@
 // result to FP64 and judges it by the backward error, so its FP32 factors may
 // use other pivots; the reference's pivot sequence is owed only by the FP64
 // `lu` / `solve` path.  Here a panel of kb <= 128 columns is factored as
-// sub-panels of TW = 64 columns.  Per sub-panel:
+// sub-panels of TW = 32 columns.  Per sub-panel:
 //
 //   tour_kernel  — the rows below the sub-panel's diagonal are split into
-//                  blocks of 512 rows (one workgroup each).  Every block runs
-//                  Gaussian elimination with partial pivoting on its rows
-//                  (registers, two rows per thread) and keeps the TW rows it
-//                  picked.  The picks of up to 8 blocks (8 x 64 = 512 rows,
-//                  original values gathered from A) are eliminated again by
-//                  the last of those workgroups to finish — an arrival counter,
-//                  so no workgroup ever waits for one that may not be resident
-//                  — and so on up the tree until one workgroup holds the TW
-//                  pivot rows and, from its own elimination, their L11 \ U11.
+//                  blocks of NODE = 1024 rows (one 256-thread workgroup each).
+//                  Every block runs Gaussian elimination with partial pivoting
+//                  on its rows (registers, four rows per thread) and keeps the
+//                  TW rows it picked.  The picks of up to FAN = 32 blocks
+//                  (32 x 32 = 1024 rows, original values gathered from A) are
+//                  eliminated again by the last of those workgroups to finish
+//                  — an arrival counter, so no workgroup ever waits for one
+//                  that may not be resident — and so on up the tree until one
+//                  workgroup holds the TW pivot rows and, from its own
+//                  elimination, their L11 \ U11.
 //                  That workgroup stages the rows the pivots displace, forms
 //                  U11^-1 and L11^-1, the pivot rows' U on the panel's later
 //                  columns (U12 = L11^-1 A12), and composes the panel's row
@@ -30,11 +31,11 @@
 //                  panel's later columns; displaced rows move to the positions
 //                  the pivots left.
 //
-// Two launches per sub-panel replace 64 device-wide exchanges.  The pivot
-// search keys are |a| with its 10 lowest bits replaced by the row index (the
-// lowest row wins among values equal in their top 21 bits); the pivot is
-// within 2^-13 relative of the column maximum, which is all the tournament
-// needs.
+// Two launches per sub-panel replace 32 device-wide exchanges.  The pivot
+// search keys are |a| with its 11 lowest bits replaced by a valid bit and the
+// row index (the lowest row wins among values equal in their top 20 bits); the
+// pivot is within 2^-12 relative of the column maximum, which is all the
+// tournament needs.
 #include "common.hpp"
 #include "kernels.hpp"
 #include "api_internal.hpp"
@@ -96,7 +97,7 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
     return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Rows of one elimination: thread t holds local rows t and t + NT in registers.
+// Rows of one elimination: thread t holds local rows t + x NT (x < RPT) in registers.
 struct Rows {
     float v[RPT][TW];
     int step[RPT];   // -1 not yet picked, >= 0 the step that picked it, -2 no row
@@ -109,7 +110,7 @@ __device__ __forceinline__ void load_row_masked(float (&v)[TW], const float *src
     for (int c = 0; c < TW; ++c) v[c] = (c < w) ? src[c] : 0.0f;
 }
 
-// Both rows of the thread: a full-width sub-panel issues all 32 16-byte loads
+// All rows of the thread: a full-width sub-panel issues all 32 16-byte loads
 // before using any (one memory latency, not 32)
 __device__ __forceinline__ void rows_load(Rows &R, const float *A, long lda, int c0, int w, const int *rid, int nr)
 {
