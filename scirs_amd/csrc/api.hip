@@ -419,7 +419,7 @@ static int host_getrf(int m, int n, T *a, int64_t lda, int64_t *perm, scirs_solv
     if ((rc = c.ensure(&c.scratch_perm, &c.scratch_perm_bytes, (size_t)m * sizeof(int64_t)))) return finish(info, rc, t0);
     T *dA = (T *)c.scratch_a;
     int64_t *dP = (int64_t *)c.scratch_perm;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), m, hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), m, s))) return finish(info, rc, t0);
     if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
     const int kb = c.lu_kb<T>(m);
     if ((rc = scirs_getrf_device<T>(m, n, dA, n, kb, c.work, s))) return finish(info, rc, t0);
@@ -429,7 +429,8 @@ static int host_getrf(int m, int n, T *a, int64_t lda, int64_t *perm, scirs_solv
     if ((rc = c.read_flags(s, &st, &pv))) return finish(info, rc, t0);
     if (info) info->pivot_value = st ? pv : 0.0;
     if (st == 0) {
-        SCIRS_HIP_CHECK(hipMemcpy2DAsync(a, lda * sizeof(T), dA, n * sizeof(T), n * sizeof(T), m, hipMemcpyDeviceToHost, s));
+        if ((rc = scirs::copy_d2h(a, lda * sizeof(T), dA, n * sizeof(T), n * sizeof(T), m, s)))
+            return finish(info, rc, t0);
         SCIRS_HIP_CHECK(hipMemcpyAsync(perm, dP, (size_t)m * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -465,12 +466,14 @@ static int host_gesv(int n, int nrhs, const T *a, int64_t lda, const T *b, int64
     if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, 2 * bbytes))) return finish(info, rc, t0);
     if ((rc = c.ensure(&c.scratch_perm, &c.scratch_perm_bytes, (size_t)n * sizeof(int64_t)))) return finish(info, rc, t0);
     T *dA = (T *)c.scratch_a, *dB = (T *)c.scratch_b, *dX = dB + (size_t)n * nrhs;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dB, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n, hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, s))) return finish(info, rc, t0);
+    if ((rc = scirs::copy_h2d(dB, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n, s)))
+        return finish(info, rc, t0);
     int st = getrf_solve_device<T>(n, nrhs, dA, n, (int64_t *)c.scratch_perm, dB, nrhs, dX, nrhs, s, info);
     if (st < 0) return finish(info, st, t0);
     if (st == 0) {
-        SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n, hipMemcpyDeviceToHost, s));
+        if ((rc = scirs::copy_d2h(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n, s)))
+            return finish(info, rc, t0);
         SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     }
     return finish(info, st, t0);
@@ -533,7 +536,7 @@ static int host_potrf(int n, T *a, int64_t lda, scirs_solve_info *info)
     int rc;
     if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(T)))) return finish(info, rc, t0);
     T *dA = (T *)c.scratch_a;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, s))) return finish(info, rc, t0);
     if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
     if ((rc = scirs_potrf_device<T>(n, dA, n, c.kb_chol, c.work, s))) return finish(info, rc, t0);
     int st = 0;
@@ -541,7 +544,8 @@ static int host_potrf(int n, T *a, int64_t lda, scirs_solve_info *info)
     if ((rc = c.read_flags(s, &st, &pv))) return finish(info, rc, t0);
     if (info) info->pivot_value = st ? pv : 0.0;
     if (st == 0) {
-        SCIRS_HIP_CHECK(hipMemcpy2DAsync(a, lda * sizeof(T), dA, n * sizeof(T), n * sizeof(T), n, hipMemcpyDeviceToHost, s));
+        if ((rc = scirs::copy_d2h(a, lda * sizeof(T), dA, n * sizeof(T), n * sizeof(T), n, s)))
+            return finish(info, rc, t0);
         SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     }
     return finish(info, st, t0);
@@ -618,15 +622,14 @@ static int host_posv(int n, int nrhs, const T *a, int64_t lda, const T *b, int64
     if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, 2 * bbytes))) return finish(info, rc, t0);
     T *dA = (T *)c.scratch_a, *dL = dA + (size_t)n * n;
     T *dB = (T *)c.scratch_b, *dX = dB + (size_t)n * nrhs;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n,
-                                     hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dB, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n,
-                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, s))) return finish(info, rc, t0);
+    if ((rc = scirs::copy_h2d(dB, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n, s)))
+        return finish(info, rc, t0);
     int st = posv_device<T>(n, nrhs, dA, n, dB, nrhs, dX, nrhs, dL, n, s, info);
     if (st < 0) return finish(info, st, t0);
     if (st == 0) {
-        SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n,
-                                         hipMemcpyDeviceToHost, s));
+        if ((rc = scirs::copy_d2h(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n, s)))
+            return finish(info, rc, t0);
         SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     }
     return finish(info, st, t0);
@@ -697,18 +700,17 @@ static int host_trsv(int n, int nrhs, const T *a, int64_t lda, const T *b, int64
     if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (size_t)n * n * sizeof(T)))) return finish(info, rc, t0);
     if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, (size_t)n * nrhs * sizeof(T)))) return finish(info, rc, t0);
     T *dA = (T *)c.scratch_a, *dX = (T *)c.scratch_b;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n,
-                                     hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dX, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n,
-                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, n * sizeof(T), a, lda * sizeof(T), n * sizeof(T), n, s))) return finish(info, rc, t0);
+    if ((rc = scirs::copy_h2d(dX, nrhs * sizeof(T), b, ldb * sizeof(T), nrhs * sizeof(T), n, s)))
+        return finish(info, rc, t0);
     if ((rc = c.ensure_xtri(n))) return finish(info, rc, t0);
     if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
     if ((rc = scirs_trsv_chain<T>(n, nrhs, dA, n, dX, nrhs, lower, unit_diagonal, c.work, s)))
         return finish(info, rc, t0);
     int st = 0;
     if ((rc = c.read_flags(s, &st, nullptr))) return finish(info, rc, t0);
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n,
-                                     hipMemcpyDeviceToHost, s));
+    if ((rc = scirs::copy_d2h(x, ldx * sizeof(T), dX, nrhs * sizeof(T), nrhs * sizeof(T), n, s)))
+        return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
 }

@@ -83,6 +83,16 @@ int finish(scirs_solve_info *info, int rc, double t0);
 int backward_error(int n, int nrhs, const double *dA, long lda, const double *dX, long ldx, const double *dB,
                    long ldb, hipStream_t s, double *out);
 
+// hostcopy.hip: rows x width bytes between a strided host array (pitch hpitch)
+// and device memory (pitch dpitch) on stream s, through pinned staging.
+// copy_h2d returns once the host array has been read (the last transfer may
+// still run on s); copy_d2h returns once the host array has been written.
+int copy_h2d(void *d, size_t dpitch, const void *h, size_t hpitch, size_t width, size_t rows, hipStream_t s);
+int copy_d2h(void *h, size_t hpitch, const void *d, size_t dpitch, size_t width, size_t rows, hipStream_t s);
+// the same for one contiguous block of bytes
+int copy_h2d_flat(void *d, const void *h, size_t bytes, hipStream_t s);
+int copy_d2h_flat(void *h, const void *d, size_t bytes, hipStream_t s);
+
 } // namespace scirs
 
 // vec.hip

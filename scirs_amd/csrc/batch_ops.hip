@@ -200,11 +200,11 @@ int host_batch_matmul(int batch, int m, int k, int n, const T *a, const T *b, T 
     int rc;
     const size_t ab = (size_t)batch * m * k * sizeof(T), bb = (size_t)k * n * sizeof(T), cb = mn * sizeof(T);
     if ((rc = scratch3(ab, bb, cb, &w))) return finish(info, rc, t0);
-    if (ab) SCIRS_HIP_CHECK(hipMemcpyAsync(w.a, a, ab, hipMemcpyHostToDevice, s));
-    if (bb) SCIRS_HIP_CHECK(hipMemcpyAsync(w.b, b, bb, hipMemcpyHostToDevice, s));
+    if (ab && (rc = scirs::copy_h2d_flat(w.a, a, ab, s))) return finish(info, rc, t0);
+    if (bb && (rc = scirs::copy_h2d_flat(w.b, b, bb, s))) return finish(info, rc, t0);
     if ((rc = launch_matmul<T>(batch, m, k, n, (const T *)w.a, (const T *)w.b, (T *)w.c, s)))
         return finish(info, rc, t0);
-    SCIRS_HIP_CHECK(hipMemcpyAsync(out, w.c, cb, hipMemcpyDeviceToHost, s));
+    if ((rc = scirs::copy_d2h_flat(out, w.c, cb, s))) return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
 }
@@ -229,12 +229,12 @@ int host_batch_matvec(int batch, int m, int n, const T *a, const T *x, T *y, sci
     int rc;
     const size_t ab = (size_t)M * n * sizeof(T), xb = (size_t)n * sizeof(T), yb = (size_t)M * sizeof(T);
     if ((rc = scratch3(ab, xb, yb, &w))) return finish(info, rc, t0);
-    if (ab) SCIRS_HIP_CHECK(hipMemcpyAsync(w.a, a, ab, hipMemcpyHostToDevice, s));
-    if (xb) SCIRS_HIP_CHECK(hipMemcpyAsync(w.b, x, xb, hipMemcpyHostToDevice, s));
+    if (ab && (rc = scirs::copy_h2d_flat(w.a, a, ab, s))) return finish(info, rc, t0);
+    if (xb && (rc = scirs::copy_h2d_flat(w.b, x, xb, s))) return finish(info, rc, t0);
     batch_matvec_kernel<T><<<(unsigned)((M + 255) / 256), 256, 0, s>>>((const T *)w.a, (const T *)w.b, (T *)w.c, M,
                                                                        n);
     SCIRS_HIP_CHECK(hipGetLastError());
-    SCIRS_HIP_CHECK(hipMemcpyAsync(y, w.c, yb, hipMemcpyDeviceToHost, s));
+    if ((rc = scirs::copy_d2h_flat(y, w.c, yb, s))) return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
 }
@@ -255,11 +255,11 @@ int host_batch_add(int batch, int m, int n, const T *a, const T *v, int axis, T 
     int rc;
     const size_t ab = (size_t)total * sizeof(T), vb = (size_t)(axis == 0 ? m : n) * sizeof(T);
     if ((rc = scratch3(ab, vb, ab, &w))) return finish(info, rc, t0);
-    SCIRS_HIP_CHECK(hipMemcpyAsync(w.a, a, ab, hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpyAsync(w.b, v, vb, hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d_flat(w.a, a, ab, s))) return finish(info, rc, t0);
+    if ((rc = scirs::copy_h2d_flat(w.b, v, vb, s))) return finish(info, rc, t0);
     batch_add_kernel<T><<<grid_for(total), 256, 0, s>>>((const T *)w.a, (const T *)w.b, (T *)w.c, total, m, n, axis);
     SCIRS_HIP_CHECK(hipGetLastError());
-    SCIRS_HIP_CHECK(hipMemcpyAsync(out, w.c, ab, hipMemcpyDeviceToHost, s));
+    if ((rc = scirs::copy_d2h_flat(out, w.c, ab, s))) return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
 }
@@ -280,10 +280,10 @@ int host_batch_sum(int batch, int m, int n, const T *a, T *out, scirs_solve_info
     int rc;
     const size_t ab = (size_t)batch * mn * sizeof(T), ob = (size_t)mn * sizeof(T);
     if ((rc = scratch3(ab, 0, ob, &w))) return finish(info, rc, t0);
-    if (ab) SCIRS_HIP_CHECK(hipMemcpyAsync(w.a, a, ab, hipMemcpyHostToDevice, s));
+    if (ab && (rc = scirs::copy_h2d_flat(w.a, a, ab, s))) return finish(info, rc, t0);
     batch_sum_kernel<T><<<grid_for(mn), 256, 0, s>>>((const T *)w.a, (T *)w.c, batch, mn);
     SCIRS_HIP_CHECK(hipGetLastError());
-    SCIRS_HIP_CHECK(hipMemcpyAsync(out, w.c, ob, hipMemcpyDeviceToHost, s));
+    if ((rc = scirs::copy_d2h_flat(out, w.c, ob, s))) return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
 }

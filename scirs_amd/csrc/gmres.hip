@@ -333,8 +333,8 @@ extern "C" int scirs_gpu_dgmres(int n, const double *a, int64_t lda, const doubl
     if ((rc = c.ensure(&c.scratch_a, &c.scratch_a_bytes, (nn * nn + 2 * nn) * sizeof(double))))
         return finish(info, rc, t0);
     double *dA = (double *)c.scratch_a, *db = dA + nn * nn, *dx = db + nn;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, nn * sizeof(double), a, lda * sizeof(double), nn * sizeof(double), n,
-                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, nn * sizeof(double), a, lda * sizeof(double), nn * sizeof(double), n, s)))
+        return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, nn * sizeof(double), hipMemcpyHostToDevice, s));
     if (x0) SCIRS_HIP_CHECK(hipMemcpyAsync(dx, x0, nn * sizeof(double), hipMemcpyHostToDevice, s));
     else SCIRS_HIP_CHECK(hipMemsetAsync(dx, 0, nn * sizeof(double), s));
@@ -413,6 +413,9 @@ int make_precond(int n, const double *dA, long lda, int precond, const double *d
         return SCIRS_OK;
     }
     if (precond == SCIRS_PRECOND_LU_GIVEN) {
+        // the chained triangular solves that apply it skip their work while the
+        // breakdown flag holds a previous call's value
+        if ((rc = c.reset_flags(s))) return rc;
         pm.lu64 = dLU;
         pm.perm = dPerm;
         return SCIRS_OK;
@@ -426,6 +429,15 @@ int make_precond(int n, const double *dA, long lda, int precond, const double *d
     pm.lu64 = lu64;
     pm.perm = perm;
     return SCIRS_OK;
+}
+
+// After GMRES with an LU preconditioner: a chained triangular solve that timed
+// out (its wait is bounded) is reported, not returned as a solution.
+int check_precond_flags(const Precond &pm, hipStream_t s)
+{
+    if (pm.kind != SCIRS_PRECOND_LU && pm.kind != SCIRS_PRECOND_LU_GIVEN) return SCIRS_OK;
+    int st = 0;
+    return ctx().read_flags(s, &st, nullptr);
 }
 
 bool precond_kind_ok(int k)
@@ -467,11 +479,11 @@ extern "C" int scirs_gpu_dgmres_precond(int n, const double *a, int64_t lda, con
     double *dA = (double *)c.scratch_a, *db = dA + nn * nn, *dx = db + nn;
     double *dLU = extra ? dx + nn : nullptr;
     int64_t *dP = extra ? (int64_t *)(dLU + nn * nn) : nullptr;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, nn * sizeof(double), a, lda * sizeof(double), nn * sizeof(double), n,
-                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, nn * sizeof(double), a, lda * sizeof(double), nn * sizeof(double), n, s)))
+        return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, nn * sizeof(double), hipMemcpyHostToDevice, s));
     if (extra) {
-        SCIRS_HIP_CHECK(hipMemcpyAsync(dLU, lu, nn * nn * sizeof(double), hipMemcpyHostToDevice, s));
+        if ((rc = scirs::copy_h2d_flat(dLU, lu, nn * nn * sizeof(double), s))) return finish(info, rc, t0);
         SCIRS_HIP_CHECK(hipMemcpyAsync(dP, perm, nn * sizeof(int64_t), hipMemcpyHostToDevice, s));
     }
     if (x0) SCIRS_HIP_CHECK(hipMemcpyAsync(dx, x0, nn * sizeof(double), hipMemcpyHostToDevice, s));
@@ -479,7 +491,7 @@ extern "C" int scirs_gpu_dgmres_precond(int n, const double *a, int64_t lda, con
     Precond pm;
     if ((rc = make_precond(n, dA, n, precond, dLU, dP, s, pm))) return finish(info, rc, t0);
     rc = gmres_device(n, dA, n, db, dx, max_iter, tol, restart, iterations, residual_norm, converged, s, info, pm);
-    if (rc) return finish(info, rc, t0);
+    if (rc || (rc = check_precond_flags(pm, s))) return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipMemcpyAsync(x, dx, nn * sizeof(double), hipMemcpyDeviceToHost, s));
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
@@ -508,5 +520,6 @@ extern "C" int scirs_gpu_dgmres_precond_dev(int n, const double *dA, int64_t lda
     Precond pm;
     if ((rc = make_precond(n, dA, lda, precond, dLU, dPerm, s, pm))) return finish(info, rc, t0);
     rc = gmres_device(n, dA, lda, dB, dX, max_iter, tol, restart, iterations, residual_norm, converged, s, info, pm);
+    if (!rc) rc = check_precond_flags(pm, s);
     return finish(info, rc, t0);
 }

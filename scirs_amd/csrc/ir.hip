@@ -726,14 +726,14 @@ extern "C" int scirs_gpu_dgesv_ir(int n, int nrhs, const double *a, int64_t lda,
     const size_t abytes = (size_t)n * n * sizeof(double), bbytes = (size_t)n * nrhs * sizeof(double);
     if ((rc = c.ensure(&c.scratch_b, &c.scratch_b_bytes, abytes + 2 * bbytes))) return finish(info, rc, t0);
     double *dA = (double *)c.scratch_b, *dB = dA + (size_t)n * n, *dX = dB + (size_t)n * nrhs;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n,
-                                     hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dB, nrhs * sizeof(double), b, ldb * sizeof(double), nrhs * sizeof(double), n,
-                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dA, n * sizeof(double), a, lda * sizeof(double), n * sizeof(double), n, s)))
+        return finish(info, rc, t0);
+    if ((rc = scirs::copy_h2d(dB, nrhs * sizeof(double), b, ldb * sizeof(double), nrhs * sizeof(double), n, s)))
+        return finish(info, rc, t0);
     rc = gesv_ir_device(n, nrhs, dA, n, dB, nrhs, dX, nrhs, mode, max_iter, max_gmres, tol, s, info);
     if (rc == 0) {
-        SCIRS_HIP_CHECK(hipMemcpy2DAsync(x, ldx * sizeof(double), dX, nrhs * sizeof(double), nrhs * sizeof(double), n,
-                                         hipMemcpyDeviceToHost, s));
+        if ((rc = scirs::copy_d2h(x, ldx * sizeof(double), dX, nrhs * sizeof(double), nrhs * sizeof(double), n, s)))
+            return finish(info, rc, t0);
         SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     }
     return finish(info, rc, t0);

@@ -287,17 +287,17 @@ extern "C" int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, d
     double *dv = dQ + (size_t)rows * pq;
     int *dflag = (int *)(dv + rows);
     double *dpart = dv + rows + 64;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
-                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows, s)))
+        return finish(info, rc, t0);
     eye_kernel<<<(unsigned)(((long)rows * rows + 255) / 256), 256, 0, s>>>(dQt, pq, rows);
     if ((rc = householder_sweep(rows, cols, cols, dR, pr, dQt, pq, dv, dflag, dpart, s))) return finish(info, rc, t0);
     dim3 tg((rows + 31) / 32, (rows + 31) / 32);
     transpose_kernel<<<tg, 256, 0, s>>>(dQt, pq, rows, rows, dQ, pq);
     SCIRS_HIP_CHECK(hipGetLastError());
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(q, ldq * sizeof(double), dQ, pq * sizeof(double), rows * sizeof(double), rows,
-                                     hipMemcpyDeviceToHost, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(r, ldr * sizeof(double), dR, pr * sizeof(double), cols * sizeof(double), rows,
-                                     hipMemcpyDeviceToHost, s));
+    if ((rc = scirs::copy_d2h(q, ldq * sizeof(double), dQ, pq * sizeof(double), rows * sizeof(double), rows, s)))
+        return finish(info, rc, t0);
+    if ((rc = scirs::copy_d2h(r, ldr * sizeof(double), dR, pr * sizeof(double), cols * sizeof(double), rows, s)))
+        return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipStreamSynchronize(s));
     return finish(info, SCIRS_OK, t0);
 }
@@ -330,10 +330,10 @@ extern "C" int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda
     double *dsq = dv + rows, *dres = dsq + rows;
     int *dflag = (int *)(dres + 1);
     double *dpart = dres + 64;
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
-                                     hipMemcpyHostToDevice, s));
-    SCIRS_HIP_CHECK(hipMemcpy2DAsync(dA, pa * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows,
-                                     hipMemcpyHostToDevice, s));
+    if ((rc = scirs::copy_h2d(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows, s)))
+        return finish(info, rc, t0);
+    if ((rc = scirs::copy_h2d(dA, pa * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows, s)))
+        return finish(info, rc, t0);
     SCIRS_HIP_CHECK(hipMemcpyAsync(db, b, rows * sizeof(double), hipMemcpyHostToDevice, s));
     // b into the extra column; strided columns move by kernels, only whole
     // contiguous vectors cross PCIe

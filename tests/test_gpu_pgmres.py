@@ -64,6 +64,25 @@ def test_given_lu_matches_oracle(L, N, n):
     assert np.abs(x - xo).max() <= 1e-9 * np.linalg.cond(a) * max(1.0, np.abs(xo).max())
 
 
+def test_given_lu_after_a_breakdown(L, N):
+    """A singular solve leaves its breakdown column in the library's flag; the
+    given-LU preconditioner's triangular solves must not read it as their own."""
+    s = np.ones((6, 6))
+    with pytest.raises(L.SingularError):
+        L.gesv(s, np.ones(6))
+    n = 64
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    lu32, piv32, _ = oracle.lu_factor(a.astype(np.float32))
+    lu = lu32.astype(np.float64)
+    print("pgmres given LU after a singular gesv", flush=True)
+    x, it, res, conv = L.gmres_precond(a, b, N.PRECOND_LU_GIVEN, lu=lu, perm=piv32, max_iter=300, tol=1e-11,
+                                       restart=10)
+    _, ito, _, convo = oracle.pgmres(a, b, 3, lu=lu, perm=piv32, max_iter=300, tol=1e-11, restart=10)
+    assert (it, conv) == (ito, convo) and conv
+
+
 @pytest.mark.parametrize("n", [500, 2000])
 def test_library_lu_preconditioner(L, N, n):
     rng = np.random.default_rng(n)
