@@ -32,84 +32,124 @@ __device__ __forceinline__ double readlane_t<double>(double v, int k)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// The kb x kb diagonal block (kb <= 128) in LDS, factored 16 columns at a time:
-//   (a) wave 0 holds the 16 columns of every row in registers (lane = row mod 64,
-//       two rows per lane) and factors them column by column with no barrier:
-//       the diagonal by readlane, sqrt, true division of the rows below
-//       (lapack.rs:661-676), the later columns of the 16 updated with the
-//       column's entries broadcast by readlane;
-//   (b) all waves apply the rank-16 update to the lower triangle right of and
-//       below the 16 columns, in 4 x 4 register tiles.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+template <typename T> struct MV;
+template <> struct MV<double> {
+    typedef dv4 V;
+    static __device__ __forceinline__ V mfma(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row of accumulator register q in lane group lk (column = lane % 16)
+    static __device__ __forceinline__ int row(int lk, int q) { return lk + 4 * q; }
+};
+template <> struct MV<float> {
+    typedef fv4 V;
+    static __device__ __forceinline__ V mfma(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
+};
+
+// The kb x kb diagonal block (kb <= 128) in LDS (lower triangle packed by
+// rows), factored in column blocks of PB = 32:
+//   A. wave 0 factors the PB x PB diagonal block, lane i holding row i in
+//      registers: per column the diagonal by readlane, sqrt, true division of
+//      the rows below (lapack.rs:661-676), the column handed to every lane
+//      through LDS, and the later columns updated — one wave, no barrier;
+//   B. the rows below solve x L_jj^T = a (one thread per row, left-looking,
+//      true division, the diagonal block read from LDS as broadcasts);
+//   C. the trailing lower triangle takes the rank-PB update on the matrix
+//      cores (16 x 16 tiles, one wave per tile).
 // The test of the reference, a_jj - sum <= 0 (lapack.rs:661-670), sets
 // info = row + 1 and stops.
 template <typename T>
 __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int kb, int *info, double *info_val)
 {
     if (*info != 0) return;
-    constexpr int PB = 16;
+    constexpr int PB = 32;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // the lower triangle, packed by rows (row r at r (r + 1) / 2): half the LDS
     // of a square tile, so the kernel fits on a CU beside one workgroup of the
     // trailing update running on the other stream
     T *S = reinterpret_cast<T *>(smem_raw);
     auto pk = [](int r, int c) { return r * (r + 1) / 2 + c; };
+    __shared__ T colb[2][PB];
+    // the factored diagonal block, unpacked, so that phase B addresses it with
+    // one base register and constant offsets
+    __shared__ T Dsq[PB][PB + 1];
+    __shared__ int s_fail;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     T *D = A + (long)k0 * lda + k0;
-    for (int i = wave; i < kb; i += NT / 64)
-        for (int j = lane; j <= i; j += 64) S[pk(i, j)] = D[(long)i * lda + j];
-    __shared__ int s_fail;
+    // rows wave, wave + 4, ...: 64 loads in flight per lane before any
+    // LDS store (a load-then-store loop would wait for each row in turn)
+    for (int i0 = wave; i0 < kb; i0 += 32 * (NT / 64)) {
+        T v[32][2];
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int i = min(i0 + q * (NT / 64), kb - 1);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) v[q][h] = D[(long)i * lda + min(lane + 64 * h, i)];
+        }
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int i = i0 + q * (NT / 64);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (i < kb && lane + 64 * h <= i) S[pk(i, lane + 64 * h)] = v[q][h];
+        }
+    }
     if (t == 0) s_fail = 0;
     __syncthreads();
     for (int jb = 0; jb < kb; jb += PB) {
-        const int pw = min(PB, kb - jb);
+        const int pw = min(PB, kb - jb), je = jb + pw;
+        // A. the diagonal block
         if (wave == 0) {
-            T a[2][PB];
+            const int i = lane;
+            T a[PB];
 #pragma unroll
-            for (int sl = 0; sl < 2; ++sl)
-#pragma unroll
-                for (int i = 0; i < PB; ++i) {
-                    const int row = lane + 64 * sl;
-                    a[sl][i] = (row < kb && i < pw && jb + i <= row) ? S[pk(row, jb + i)] : (T)0;
-                }
+            for (int c = 0; c < PB; ++c) a[c] = (i < pw && c <= i) ? S[pk(jb + i, jb + c)] : (T)0;
+            bool ok = true;
             int fail_row = -1;
             T fail_val = (T)0;
 #pragma unroll
-            for (int i = 0; i < PB; ++i) {
-                if (i < pw && fail_row < 0) {
-                    const int cj = jb + i;
-                    const T dv = readlane_t<T>((cj >> 6) ? a[1][i] : a[0][i], cj & 63);
+            for (int j = 0; j < PB; ++j) {
+                if (ok && j < pw) {
+                    const T dv = readlane_t<T>(a[j], j);
                     if (!(dv > (T)0)) {
-                        fail_row = cj;
+                        ok = false;
+                        fail_row = jb + j;
                         fail_val = dv;
                     } else {
                         const T d = sqrt(dv);
+                        a[j] = (i == j) ? d : (i > j ? a[j] / d : a[j]);
+                        T *cb = colb[j & 1];
+                        if (i < PB) cb[i] = a[j];
+                        wave_sync();
+                        // rows at or above j take a zero multiplier instead of a
+                        // select per column (a - 0 * l is a: l is finite)
+                        const T lj = i > j ? a[j] : (T)0;
 #pragma unroll
-                        for (int sl = 0; sl < 2; ++sl) {
-                            const int row = lane + 64 * sl;
-                            if (row == cj) a[sl][i] = d;
-                            else if (row > cj) a[sl][i] = a[sl][i] / d;
-                        }
-#pragma unroll
-                        for (int i2 = i + 1; i2 < PB; ++i2) {
-                            if (i2 < pw) {
-                                const int c2 = jb + i2;
-                                const T lc = readlane_t<T>((c2 >> 6) ? a[1][i] : a[0][i], c2 & 63);
-#pragma unroll
-                                for (int sl = 0; sl < 2; ++sl)
-                                    if (lane + 64 * sl > cj) a[sl][i2] -= a[sl][i] * lc;
-                            }
-                        }
+                        for (int c = j + 1; c < PB; ++c) a[c] -= lj * cb[c];
                     }
                 }
             }
 #pragma unroll
-            for (int sl = 0; sl < 2; ++sl)
+            for (int c = 0; c < PB; ++c)
+                if (i < pw && c <= i) S[pk(jb + i, jb + c)] = a[c];
+            if (i < PB)
 #pragma unroll
-                for (int i = 0; i < PB; ++i) {
-                    const int row = lane + 64 * sl;
-                    if (row < kb && i < pw && row >= jb + i) S[pk(row, jb + i)] = a[sl][i];
-                }
-            if (fail_row >= 0 && lane == 0) {
+                for (int c = 0; c < PB; ++c) Dsq[i][c] = a[c];
+            if (!ok && lane == 0) {
                 s_fail = 1;
                 *info = k0 + fail_row + 1;
                 *info_val = (double)fail_val;
@@ -117,45 +157,71 @@ __global__ __launch_bounds__(NT) void potf2_kernel(T *A, long lda, int k0, int k
         }
         __syncthreads();
         if (s_fail) return;
-        // (b) S[r][c] -= sum_k S[r][jb+k] S[c][jb+k] for je <= c <= r < kb, 4 x 4 tiles
-        const int je = jb + pw;
-        const int nt = (kb - je + 3) / 4;
-        const int ntiles = nt * (nt + 1) / 2;
-        for (int tile = t; tile < ntiles; tile += NT) {
-            int ti = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
-            while (ti * (ti + 1) / 2 > tile) --ti;
-            while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
-            const int tj = tile - ti * (ti + 1) / 2;
-            const int r0 = je + 4 * ti, c0 = je + 4 * tj;
-            T acc[4][4];
+        if (je >= kb) break;
+        // B. rows below: x L_jj^T = a, x_c = (a_c - sum_{q<c} x_q L[c][q]) / L[c][c]
+        //    (left-looking, four partial sums to shorten the chain).  Every
+        //    address is one base plus a constant: past pw the loads read the
+        //    padding after the triangle or stale entries, and the values they
+        //    give are masked before anything is stored.
+        // (kb <= 128, so at most 96 rows: one per thread — a loop over rows
+        // would let the compiler hoist the diagonal block's 528 loads out of it)
+        if (je + t < kb) {
+            const int r = je + t;
+            T *xr = S + pk(r, jb);
+            T x[PB];
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
-#pragma unroll
-                for (int y = 0; y < 4; ++y) acc[x][y] = (T)0;
-            for (int k = 0; k < pw; k += 4) {
-                T pr[4][4], pc[4][4];
-#pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const bool kk = k + q < pw;
-                        pr[x][q] = (kk && r0 + x < kb) ? S[pk(r0 + x, jb + k + q)] : (T)0;
-                        pc[x][q] = (kk && c0 + x < kb) ? S[pk(c0 + x, jb + k + q)] : (T)0;
-                    }
-#pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) acc[x][y] += pr[x][q] * pc[y][q];
+            for (int c = 0; c < PB; ++c) {
+                const T v = xr[c];
+                x[c] = c < pw ? v : (T)0;
             }
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
+            for (int c = 0; c < PB; ++c) {
+                // row c of the block, addressed through an offset the compiler
+                // sees only once x[c - 2] exists: its loads then overlap the
+                // previous column's division, and 528 loads are never in flight
+                // (or in registers) at once
+                int off = c * (PB + 1);
+                if (c >= 2) asm volatile("" : "+v"(off) : "v"(x[c - 2]));
+                const T *drow = &Dsq[0][0] + off;
+                T sp[4] = {x[c], (T)0, (T)0, (T)0};
 #pragma unroll
-                for (int y = 0; y < 4; ++y) {
-                    const int r = r0 + x, c = c0 + y;
-                    if (r < kb && c <= r) S[pk(r, c)] -= acc[x][y];
+                for (int q = 0; q < c; ++q) sp[q & 3] -= x[q] * drow[q];
+                const T v = ((sp[0] + sp[1]) + (sp[2] + sp[3])) / drow[c];
+                x[c] = c < pw ? v : (T)0;
+            }
+#pragma unroll
+            for (int c = 0; c < PB; ++c)
+                if (c < pw) xr[c] = x[c];
+        }
+        __syncthreads();
+        // C. S[r][c] -= sum_k S[r][jb+k] S[c][jb+k], je <= c <= r < kb, 16 x 16 tiles
+        {
+            const int lr = lane & 15, lk = lane >> 4;
+            const int nt = (kb - je + 15) / 16, ntiles = nt * (nt + 1) / 2;
+            for (int tile = wave; tile < ntiles; tile += NT / 64) {
+                int ti = 0;
+                while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+                const int tj = tile - ti * (ti + 1) / 2;
+                const int r0 = je + 16 * ti, c0 = je + 16 * tj;
+                const int ra = min(r0 + lr, kb - 1), ca = min(c0 + lr, kb - 1);
+                typename MV<T>::V acc = {0, 0, 0, 0};
+                const T *ar = S + pk(ra, jb) + lk, *br = S + pk(ca, jb) + lk;
+                T av[PB / 4], bv[PB / 4];
+#pragma unroll
+                for (int k = 0; k < PB / 4; ++k) {
+                    const bool kk = 4 * k + lk < pw;
+                    const T x = ar[4 * k], y = br[4 * k];
+                    av[k] = kk ? x : (T)0;
+                    bv[k] = kk ? y : (T)0;
                 }
+#pragma unroll
+                for (int k = 0; k < PB / 4; ++k) acc = MV<T>::mfma(av[k], bv[k], acc);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r0 + MV<T>::row(lk, q), c = c0 + lr;
+                    if (r < kb && c <= r) S[pk(r, c)] -= acc[q];
+                }
+            }
         }
         __syncthreads();
     }
@@ -293,12 +359,14 @@ __global__ __launch_bounds__(NT) void trtri_lower64_kernel(const double *L, long
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
             for (int k = j; k < i; ++k) {
                 const double *xk = X + blk64(k, j) + 4 * c4;
-#pragma unroll 8
-                for (int q = 0; q < TB64; ++q) {
-                    const double a = Lr[TB64 * k + q];
+                // the row's 32 entries of L_ik in one round trip to memory
+                double lv[TB64];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) acc[e] = fma(a, xk[q * TB64 + e], acc[e]);
-                }
+                for (int q = 0; q < TB64; ++q) lv[q] = Lr[TB64 * k + q];
+#pragma unroll
+                for (int q = 0; q < TB64; ++q)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[e] = fma(lv[q], xk[q * TB64 + e], acc[e]);
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) Ts[r * TB64 + 4 * c4 + e] = acc[e];
@@ -348,7 +416,9 @@ int scirs_potrf_device(int n, T *A, long lda, int kb, ScirsWork &w, hipStream_t 
     }
     for (int k0 = 0, p = 0; k0 < n; k0 += kb, ++p) {
         const int kbw = std::min(kb, n - k0);
-        const size_t lds_p = (size_t)kbw * (kbw + 1) / 2 * sizeof(T);
+        // the packed triangle and PB elements of padding that reads past a
+        // partial column block may touch
+        const size_t lds_p = ((size_t)kbw * (kbw + 1) / 2 + 32) * sizeof(T);
         if (int rc = scirs_allow_lds((const void *)potf2_kernel<T>, lds_p)) return rc;
         hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(NT), lds_p, stream,
                            A, lda, k0, kbw, w.info, w.info_val);
