@@ -273,9 +273,21 @@ int lu_solve_f32(IrState &st, float *d32)
 }
 
 // eta_j for every column; returns the worst, fills per-column values
-int backward_errors(IrState &st, std::vector<double> &eta, std::vector<double> &rnorm)
+int backward_errors(IrState &st, std::vector<double> &eta, std::vector<double> &rnorm, bool x_zero = false)
 {
     int rc;
+    if (x_zero) {
+        // X = 0: R = B, ||R_j|| = ||B_j|| and eta_j = 1 (0 for a zero column),
+        // with no pass over A
+        if ((rc = scirs_d2d_copy(st.n, st.nrhs, st.B, st.ldb, st.R, st.nrhs, st.s))) return rc;
+        eta.assign(st.nrhs, 0.0);
+        rnorm.assign(st.nrhs, 0.0);
+        for (int j = 0; j < st.nrhs; ++j) {
+            rnorm[j] = st.normB[j];
+            eta[j] = st.normB[j] > 0 ? 1.0 : 0.0;
+        }
+        return SCIRS_OK;
+    }
     if ((rc = scirs_residual(st.n, st.nrhs, st.A, st.lda, st.X, st.ldx, st.B, st.ldb, st.R, st.nrhs, st.s))) return rc;
     if ((rc = scirs_colmax_d(st.n, st.nrhs, st.R, st.nrhs, st.norms, st.s))) return rc;
     if ((rc = scirs_colmax_d(st.n, st.nrhs, st.X, st.ldx, st.norms + 64, st.s))) return rc;
@@ -595,7 +607,7 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
         std::vector<double> eta, rn, prev(nrhs, INFINITY);
         std::vector<int> use_gmres(nrhs, 0), gm_col(nrhs, 0);   // GMRES budget per right-hand side
         for (int it = 0; it <= max_iter; ++it) {
-            if ((rc = backward_errors(st, eta, rn))) return rc;
+            if ((rc = backward_errors(st, eta, rn, it == 0))) return rc;
             worst = 0;
             for (double e : eta) worst = std::max(worst, e);
             if (worst <= tol) { converged = true; break; }
