@@ -223,6 +223,9 @@ int Context::read_flags(hipStream_t s, int *info, double *info_val)
         if (abort_flag > 0)
             scirs_set_error("the LU panel's exchange timed out at column %d (another process may hold the device's "
                             "compute units; SCIRS_GPU_SPIN_LIMIT bounds the wait)", abort_flag);
+        else if (abort_flag == SCIRS_ABORT_QR_PANEL)
+            scirs_set_error("the QR panel's exchange timed out (another process may hold the device's compute "
+                            "units; SCIRS_GPU_SPIN_LIMIT bounds the wait)");
         else
             scirs_set_error("a chained triangular solve timed out waiting for its hand-over (SCIRS_GPU_SPIN_LIMIT "
                             "bounds the wait)");

@@ -82,6 +82,9 @@ enum {
 enum { SCIRS_KB_MAX = 256, SCIRS_MAX_G = 512, SCIRS_PANEL_REP_MAX = 8 };
 // chains of a chained triangular solve launched together (one ticket each, in the flags block)
 enum { SCIRS_CHAIN_GROUPS = 32 };
+// value of *abort raised by a timed-out QR panel exchange (the LU panel raises
+// its column + 1, a chained triangular solve -1)
+enum { SCIRS_ABORT_QR_PANEL = -2 };
 
 // Trailing (Schur) update C -= A B of the factorisations: FP64 MFMA for double,
 // the arithmetic chosen by w.lowp_mode for float.

@@ -13,9 +13,10 @@
 // order); then the trailing block, cut into 32-column strips x 256-row chunks,
 // forms per-chunk partials of v^T c and subtracts 2 v (v^T c) after summing
 // the partials in chunk order.  Row-major storage makes each 32-column strip
-// of a row a 256-byte coalesced segment.  This is the unblocked (level-2)
-// form: bound by passes over the trailing block per column; the blocked
-// (compact-WY, MFMA) form is the next step for large matrices.
+// of a row a 256-byte coalesced segment.  This unblocked (level-2) form is
+// bound by passes over the trailing block per column; matrices of up to
+// 16384 rows take the blocked (compact-WY) form further down, which applies
+// 32 reflectors per pass (SCIRS_GPU_QR_BLOCKED=0 selects the unblocked one).
 #include "api_internal.hpp"
 #include <cfloat>
 #include <vector>
@@ -260,6 +261,378 @@ int householder_sweep(int rows, int cols, int rcols, double *R, long ldr, double
 
 long round_up(long x, long m) { return (x + m - 1) / m * m; }
 
+// ---------------------------------------------------------------------------
+// Blocked form (compact WY, Schreiber & Van Loan 1989): the same reflectors,
+// QB = 32 columns at a time.  H_1 ... H_w = I - V T V^T with V the unit
+// reflectors as columns and T upper triangular (T_ii = 2 for an applied
+// reflector, 0 for a skipped column), so the panel's reflectors reach the
+// trailing columns of R and the rows k0.. of Q^T as
+//     C <- C - V (T^T (V^T C))
+// — one pass of partial products V^T C, one rank-32 product on the matrix
+// cores — instead of two passes per column.
+constexpr int QB = 32;        // panel width
+constexpr int PR = 256;       // rows per panel workgroup, one per thread
+constexpr int QG_MAX = 64;    // panel workgroups: rows - k0 <= QG_MAX * PR
+constexpr int QKC = 32;       // row chunks of the partial products V^T C
+constexpr int QWT = 256;      // columns per partial-product workgroup
+typedef unsigned long long u64;
+constexpr size_t QXW_WORDS = 2ull * QG_MAX * (QB + 1) * 2;   // exchange words (2 per value)
+
+__device__ __forceinline__ void qx_put(u64 *w, unsigned seq, double v)
+{
+    const u64 bits = (u64)__double_as_longlong(v);
+    __hip_atomic_store(w, ((u64)seq << 32) | (bits & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w + 1, ((u64)seq << 32) | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool qx_get(const u64 *w, unsigned seq, double &v)
+{
+    const u64 lo = __hip_atomic_load(const_cast<u64 *>(w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const u64 hi = __hip_atomic_load(const_cast<u64 *>(w + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+    return (unsigned)(lo >> 32) == seq && (unsigned)(hi >> 32) == seq;
+}
+// poll one tagged value; false when the wait gave up (or another workgroup did)
+__device__ bool qx_wait(const u64 *w, unsigned seq, double &v, unsigned spin_limit, int *abort)
+{
+    for (unsigned it = 0;; ++it) {
+        if (qx_get(w, seq, v)) return true;
+        if (it >= spin_limit || __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            __hip_atomic_store(abort, (int)SCIRS_ABORT_QR_PANEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+struct QPanelArgs {
+    double *R; long ldr;
+    int rows, k0, w, G;
+    double *V;          // rows x QB, ld QB: the panel's reflectors (0 above their first row, 0 past w)
+    int *flags;         // [QB]: 1 reflector applied, 0 column skipped
+    u64 *xw;            // [2][QG_MAX][QB + 1][2]: phase 0 (sums of squares, x0), phase 1 (dots)
+    unsigned seq;       // phase p of column j uses seq + 2 j + p
+    int *abort; unsigned spin_limit;
+};
+
+// The reflectors of columns k0 .. k0 + w - 1 with the reference's arithmetic
+// (lapack.rs:205-235): per column the sum of squares, alpha = -sign(x0) ||x||,
+// v = (x - alpha e1) / ||v||, and H = I - 2 v v^T on the panel's columns
+// j.. .  G workgroups hold PR rows each in LDS; the two sums a column needs
+// (||x||^2 and the dots v^T a_c) are exchanged as sequence-tagged words and
+// added in workgroup order, so every workgroup computes the same alpha and v.
+// The exchanged sum covers the rows below the pivot row, so ||x||^2 = x0^2 +
+// tail and ||v||^2 = (x0 - alpha)^2 + tail (as the reference sums v_i^2, v0
+// first) come from one exchange without cancellation.
+__global__ __launch_bounds__(PR) void qr_panel_kernel(QPanelArgs a)
+{
+    // products rounded before they are added, as in the reference's loops
+#pragma clang fp contract(off)
+    __shared__ double P[PR][QB + 1];
+    __shared__ double xd[QG_MAX][QB];
+    __shared__ double red[PR / 32][QB];
+    __shared__ double vcol[PR];
+    __shared__ double tot[QB + 1];
+    __shared__ int s_fail;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = blockIdx.x;
+    const int row = a.k0 + g * PR + t;
+    const bool have = row < a.rows;
+    for (int c = 0; c < QB; ++c) P[t][c] = (have && c < a.w) ? a.R[(long)row * a.ldr + a.k0 + c] : 0.0;
+    if (t == 0) s_fail = 0;
+    __syncthreads();
+    u64 *x0w = a.xw, *x1w = a.xw + (size_t)QG_MAX * (QB + 1) * 2;
+    for (int j = 0; j < a.w; ++j) {
+        const int rj = a.k0 + j;
+        const unsigned sq0 = a.seq + 2u * j, sq1 = sq0 + 1u;
+        // (a) this workgroup's sum of squares of column j below the pivot row
+        //     (rows > rj); the pivot row rj lies in workgroup 0, which also
+        //     publishes x0
+        const double x = (have && row >= rj) ? P[t][j] : 0.0;
+        const double ws = wave_sum(row > rj ? x * x : 0.0);
+        if (lane == 0) red[wave][0] = ws;
+        __syncthreads();
+        if (t == 0) {
+            double ss = 0.0;
+            for (int q = 0; q < PR / 64; ++q) ss += red[q][0];
+            qx_put(x0w + ((size_t)g * (QB + 1)) * 2, sq0, ss);
+            qx_put(x0w + ((size_t)g * (QB + 1) + 1) * 2, sq0, g == 0 ? P[j][j] : 0.0);
+        }
+        if (wave == 0) {
+            double v = 0.0;
+            bool ok = true;
+            if (lane < a.G) ok = qx_wait(x0w + ((size_t)lane * (QB + 1)) * 2, sq0, v, a.spin_limit, a.abort);
+            double x0 = 0.0;
+            if (lane == 0 && ok) ok = qx_wait(x0w + 2, sq0, x0, a.spin_limit, a.abort);
+            const double sum = wave_sum(v);
+            if (!ok) s_fail = 1;
+            if (lane == 0) {
+                tot[0] = sum;
+                tot[1] = x0;
+            }
+        }
+        __syncthreads();
+        if (s_fail) return;
+        const double tail = tot[0], x0 = tot[1];
+        const double xn = sqrt(x0 * x0 + tail);
+        bool apply = xn > DBL_EPSILON;
+        double alpha = 0.0, v0 = 0.0, vn = 1.0;
+        if (apply) {
+            alpha = x0 >= 0.0 ? -xn : xn;
+            v0 = x0 - alpha;
+            vn = sqrt(v0 * v0 + tail);
+            apply = vn > DBL_EPSILON;
+        }
+        if (g == 0 && t == 0) a.flags[j] = apply ? 1 : 0;
+        const double vi = (!apply || !have || row < rj) ? 0.0 : (row == rj ? v0 / vn : x / vn);
+        if (have) a.V[(long)row * QB + j] = vi;
+        if (!apply) continue;   // the same decision in every workgroup
+        vcol[t] = vi;
+        __syncthreads();
+        // (b) dots d_c = v^T a_c for the panel's columns c >= j: 32 columns x
+        //     8 row groups, the groups added in order, then exchanged
+        {
+            const int cc = t & (QB - 1), rg = t >> 5;
+            double d = 0.0;
+            if (cc >= j && cc < a.w)
+                for (int i = rg; i < PR; i += PR / 32) d += vcol[i] * P[i][cc];
+            red[rg][cc] = d;
+        }
+        __syncthreads();
+        if (t < QB && t >= j && t < a.w) {
+            double d = 0.0;
+            for (int q = 0; q < PR / 32; ++q) d += red[q][t];
+            qx_put(x1w + ((size_t)g * (QB + 1) + t) * 2, sq1, d);
+        }
+        for (int q = t; q < a.G * QB; q += PR) {
+            const int gq = q / QB, cq = q % QB;
+            if (cq >= j && cq < a.w) {
+                double v;
+                if (!qx_wait(x1w + ((size_t)gq * (QB + 1) + cq) * 2, sq1, v, a.spin_limit, a.abort)) s_fail = 1;
+                xd[gq][cq] = v;
+            }
+        }
+        __syncthreads();
+        if (s_fail) return;
+        if (t < QB && t >= j && t < a.w) {
+            double d = 0.0;
+            for (int q = 0; q < a.G; ++q) d += xd[q][t];
+            tot[t] = d;
+        }
+        __syncthreads();
+        // (c) H on this workgroup's rows of columns j.. (the reference's
+        //     r_ic -= 2 v_i (v^T c))
+        if (have)
+            for (int c = j; c < a.w; ++c) P[t][c] -= 2.0 * vi * tot[c];
+        __syncthreads();
+    }
+    if (have) {
+        for (int c = 0; c < a.w; ++c) a.R[(long)row * a.ldr + a.k0 + c] = P[t][c];
+        for (int c = a.w; c < QB; ++c) a.V[(long)row * QB + c] = 0.0;
+    }
+}
+
+// Partial products over row chunks: part[kc][jj][col] = sum over the chunk's
+// rows of V[row][jj] * C[row][col], for the virtual columns
+//   [0, QB)             the panel's own reflectors (V^T V, for T),
+//   [QB, QB + nR)       R's columns rc0 .. rc0 + nR - 1,
+//   [QB + nR, ntot)     Q^T's columns 0 .. nQ - 1.
+// One thread per column, every row of the chunk; V's rows staged in LDS and
+// read as broadcasts.
+struct QWArgs {
+    const double *V;
+    const double *R; long ldr; int rc0, nR;
+    const double *Qt; long ldq; int nQ;
+    int rows, k0, rch;
+    double *part; long ldp;
+};
+__global__ __launch_bounds__(QWT) void qr_wpart_kernel(QWArgs a)
+{
+    __shared__ double Vs[64][QB + 1];
+    const int t = threadIdx.x, col = blockIdx.x * QWT + t, kc = blockIdx.y;
+    const int ntot = QB + a.nR + a.nQ;
+    const double *src = nullptr;
+    long ld = 0;
+    if (col < QB) { src = a.V + col; ld = QB; }
+    else if (col < QB + a.nR) { src = a.R + a.rc0 + (col - QB); ld = a.ldr; }
+    else if (col < ntot) { src = a.Qt + (col - QB - a.nR); ld = a.ldq; }
+    double acc[QB];
+#pragma unroll
+    for (int jj = 0; jj < QB; ++jj) acc[jj] = 0.0;
+    const int rbeg = a.k0 + kc * a.rch, rend = min(a.rows, rbeg + a.rch);
+    for (int r0 = rbeg; r0 < rend; r0 += 64) {
+        __syncthreads();
+        for (int e = t; e < 64 * QB; e += QWT) {
+            const int rr = e / QB, jj = e % QB;
+            Vs[rr][jj] = r0 + rr < rend ? a.V[(long)(r0 + rr) * QB + jj] : 0.0;
+        }
+        __syncthreads();
+        const int nrow = min(64, rend - r0);
+        if (src)
+            for (int rr = 0; rr < nrow; ++rr) {
+                const double c = src[(long)(r0 + rr) * ld];
+#pragma unroll
+                for (int jj = 0; jj < QB; ++jj) acc[jj] = fma(Vs[rr][jj], c, acc[jj]);
+            }
+    }
+    if (col < ntot)
+#pragma unroll
+        for (int jj = 0; jj < QB; ++jj) a.part[((long)kc * QB + jj) * a.ldp + col] = acc[jj];
+}
+
+// T (QB x QB, upper) from the Gram matrix V^T V (the partials' first QB
+// columns, added in chunk order): column i is -2 T[0:i, 0:i] (V^T v_i) above
+// T_ii = 2 (LAPACK's forward column-wise T); a skipped column is all zero.
+__global__ __launch_bounds__(QB * QB) void qr_tbuild_kernel(const double *part, long ldp, int kcn, const int *flags,
+                                                            double *T)
+{
+    __shared__ double Gm[QB][QB + 1];
+    __shared__ double Ts[QB][QB + 1];
+    __shared__ double z[QB];
+    const int t = threadIdx.x, r = t / QB, c = t % QB;
+    double sacc = 0.0;
+    for (int kc = 0; kc < kcn; ++kc) sacc += part[((long)kc * QB + r) * ldp + c];
+    Gm[r][c] = sacc;
+    Ts[r][c] = 0.0;
+    __syncthreads();
+    for (int i = 0; i < QB; ++i) {
+        const bool on = flags[i] != 0;
+        if (t < i) z[t] = on ? -2.0 * Gm[t][i] : 0.0;
+        __syncthreads();
+        if (t < i) {
+            double v = 0.0;
+            for (int q = t; q < i; ++q) v += Ts[t][q] * z[q];
+            Ts[t][i] = v;
+        }
+        if (t == 0) Ts[i][i] = on ? 2.0 : 0.0;
+        __syncthreads();
+    }
+    T[r * QB + c] = Ts[r][c];
+}
+
+// Y[:, col] = T^T W[:, col], W = the partials added in chunk order, for the
+// virtual columns QB .. ntot-1 (Y is QB x (ntot - QB), ld ldy).
+__global__ __launch_bounds__(256) void qr_y_kernel(const double *part, long ldp, int kcn, int ntot, const double *T,
+                                                   double *Y, long ldy)
+{
+    __shared__ double Ts[QB][QB];
+    for (int e = threadIdx.x; e < QB * QB; e += 256) Ts[e / QB][e % QB] = T[e];
+    __syncthreads();
+    const int col = QB + blockIdx.x * 256 + threadIdx.x;
+    if (col >= ntot) return;
+    double w[QB];
+#pragma unroll
+    for (int jj = 0; jj < QB; ++jj) w[jj] = 0.0;
+    for (int kc = 0; kc < kcn; ++kc)
+#pragma unroll
+        for (int jj = 0; jj < QB; ++jj) w[jj] += part[((long)kc * QB + jj) * ldp + col];
+#pragma unroll
+    for (int i = 0; i < QB; ++i) {
+        double y = 0.0;
+#pragma unroll
+        for (int jj = 0; jj <= i; ++jj) y = fma(Ts[jj][i], w[jj], y);
+        Y[(long)i * ldy + (col - QB)] = y;
+    }
+}
+
+__global__ void fill_u64_kernel(u64 *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// The blocked sweep's device workspace (grown on demand; the exchange words
+// keep their tags across calls, so the sequence continues).
+struct QrBlockedWork {
+    double *V = nullptr, *part = nullptr, *T = nullptr, *Y = nullptr;
+    int *flags = nullptr;
+    u64 *xw = nullptr;
+    size_t bytes = 0;
+    void *base = nullptr;
+    unsigned seq = 1;
+};
+QrBlockedWork g_qbw;
+
+bool blocked_ok(int rows)
+{
+    static const int on = [] {
+        const char *v = getenv("SCIRS_GPU_QR_BLOCKED");
+        return !(v && *v && atoi(v) == 0);
+    }();
+    return on && rows <= QG_MAX * PR;
+}
+
+int householder_blocked(int rows, int cols, int rcols, double *R, long ldr, double *Qt, long ldq, hipStream_t s)
+{
+    Context &c = ctx();
+    const int kmax = std::min(rows, cols);
+    const int nQ = Qt ? rows : 0;
+    const long ldp = round_up(QB + rcols + nQ, 64);
+    const size_t need = ((size_t)rows * QB + (size_t)QKC * QB * ldp + QB * QB + (size_t)QB * ldp + 64) * sizeof(double) +
+                        QXW_WORDS * sizeof(u64) + 256;
+    QrBlockedWork &q = g_qbw;
+    if (need > q.bytes) {
+        if (q.base) {
+            SCIRS_HIP_CHECK(hipDeviceSynchronize());
+            SCIRS_HIP_CHECK(hipFree(q.base));
+            q.base = nullptr;
+            q.bytes = 0;
+        }
+        if (hipMalloc(&q.base, need) != hipSuccess) {
+            q.base = nullptr;
+            scirs_set_error("blocked QR: device allocation of %zu bytes failed", need);
+            return SCIRS_ERR_NOMEM;
+        }
+        q.bytes = need;
+        q.xw = (u64 *)q.base;
+        // tag 0 is never a live sequence number
+        fill_u64_kernel<<<(unsigned)((QXW_WORDS + 255) / 256), 256, 0, s>>>(q.xw, QXW_WORDS);
+        SCIRS_HIP_CHECK(hipGetLastError());
+        q.seq = 1;
+    }
+    // this call's layout (the buffer may be larger than it needs)
+    q.V = (double *)(q.xw + QXW_WORDS);
+    q.part = q.V + (size_t)rows * QB;
+    q.T = q.part + (size_t)QKC * QB * ldp;
+    q.Y = q.T + QB * QB;
+    q.flags = (int *)(q.Y + (size_t)QB * ldp);
+    for (int k0 = 0; k0 < kmax; k0 += QB) {
+        const int w = std::min(QB, kmax - k0), rows_k = rows - k0;
+        if (q.seq > 0xFFFFFFFFu - 2u * QB) {
+            fill_u64_kernel<<<(unsigned)((QXW_WORDS + 255) / 256), 256, 0, s>>>(q.xw, QXW_WORDS);
+            q.seq = 1;
+        }
+        QPanelArgs pa;
+        pa.R = R; pa.ldr = ldr; pa.rows = rows; pa.k0 = k0; pa.w = w;
+        pa.G = (rows_k + PR - 1) / PR;
+        pa.V = q.V; pa.flags = q.flags; pa.xw = q.xw; pa.seq = q.seq;
+        pa.abort = c.work.abort; pa.spin_limit = c.work.spin_limit;
+        q.seq += 2u * QB;
+        qr_panel_kernel<<<pa.G, PR, 0, s>>>(pa);
+        const int rc0 = k0 + w, nR = rcols - rc0, ntot = QB + nR + nQ;
+        int kcn = std::min(QKC, (rows_k + 63) / 64);
+        const int rch = (int)round_up((rows_k + kcn - 1) / kcn, 64);
+        kcn = (rows_k + rch - 1) / rch;
+        QWArgs wa;
+        wa.V = q.V; wa.R = R; wa.ldr = ldr; wa.rc0 = rc0; wa.nR = nR; wa.Qt = Qt; wa.ldq = ldq; wa.nQ = nQ;
+        wa.rows = rows; wa.k0 = k0; wa.rch = rch; wa.part = q.part; wa.ldp = ldp;
+        qr_wpart_kernel<<<dim3((ntot + QWT - 1) / QWT, kcn), QWT, 0, s>>>(wa);
+        qr_tbuild_kernel<<<1, QB * QB, 0, s>>>(q.part, ldp, kcn, q.flags, q.T);
+        if (nR + nQ > 0) {
+            qr_y_kernel<<<(nR + nQ + 255) / 256, 256, 0, s>>>(q.part, ldp, kcn, ntot, q.T, q.Y, ldp);
+            const double *Vk = q.V + (size_t)k0 * QB;
+            if (nR > 0)
+                if (int rc = scirs_launch_dgemm_sub(rows_k, nR, QB, Vk, QB, q.Y, ldp, 0, R + (long)k0 * ldr + rc0, ldr,
+                                                    0, c.work.info, s))
+                    return rc;
+            if (nQ > 0)
+                if (int rc = scirs_launch_dgemm_sub(rows_k, nQ, QB, Vk, QB, q.Y + nR, ldp, 0, Qt + (long)k0 * ldq,
+                                                    ldq, 0, c.work.info, s))
+                    return rc;
+        }
+        SCIRS_HIP_CHECK(hipGetLastError());
+    }
+    return SCIRS_OK;
+}
+
 } // namespace
 
 extern "C" int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, double *q, int64_t ldq, double *r,
@@ -290,7 +663,14 @@ extern "C" int scirs_gpu_dqr(int rows, int cols, const double *a, int64_t lda, d
     if ((rc = scirs::copy_h2d(dR, pr * sizeof(double), a, lda * sizeof(double), cols * sizeof(double), rows, s)))
         return finish(info, rc, t0);
     eye_kernel<<<(unsigned)(((long)rows * rows + 255) / 256), 256, 0, s>>>(dQt, pq, rows);
-    if ((rc = householder_sweep(rows, cols, cols, dR, pr, dQt, pq, dv, dflag, dpart, s))) return finish(info, rc, t0);
+    if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
+    if ((rc = blocked_ok(rows) ? householder_blocked(rows, cols, cols, dR, pr, dQt, pq, s)
+                               : householder_sweep(rows, cols, cols, dR, pr, dQt, pq, dv, dflag, dpart, s)))
+        return finish(info, rc, t0);
+    {
+        int st = 0;
+        if ((rc = c.read_flags(s, &st, nullptr))) return finish(info, rc, t0);
+    }
     dim3 tg((rows + 31) / 32, (rows + 31) / 32);
     transpose_kernel<<<tg, 256, 0, s>>>(dQt, pq, rows, rows, dQ, pq);
     SCIRS_HIP_CHECK(hipGetLastError());
@@ -339,8 +719,14 @@ extern "C" int scirs_gpu_dlstsq(int rows, int cols, const double *a, int64_t lda
     // contiguous vectors cross PCIe
     strided_scatter_kernel<<<(rows + 255) / 256, 256, 0, s>>>(db, rows, dR + cols, pr);
     SCIRS_HIP_CHECK(hipGetLastError());
-    if ((rc = householder_sweep(rows, cols, cols + 1, dR, pr, nullptr, 0, dv, dflag, dpart, s)))
+    if ((rc = c.reset_flags(s))) return finish(info, rc, t0);
+    if ((rc = blocked_ok(rows) ? householder_blocked(rows, cols, cols + 1, dR, pr, nullptr, 0, s)
+                               : householder_sweep(rows, cols, cols + 1, dR, pr, nullptr, 0, dv, dflag, dpart, s)))
         return finish(info, rc, t0);
+    {
+        int st0 = 0;
+        if ((rc = c.read_flags(s, &st0, nullptr))) return finish(info, rc, t0);
+    }
     // the back substitution's |r_ii| < eps test (solve.rs:184-191), first row
     // in substitution order (bottom up)
     std::vector<double> diag(cols);

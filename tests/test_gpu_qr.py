@@ -110,3 +110,19 @@ def test_lstsq_large_normal_equations(L):
     # optimality: A^T r = 0 to rounding, and the residual it reports
     assert np.abs(a.T @ r).max() <= 1e-9 * np.abs(a).max() * np.abs(b).sum()
     assert abs(res - float(r @ r)) <= 1e-9 * float(r @ r)
+
+
+def test_blocked_workspace_follows_each_shape(L):
+    """The blocked sweep's workspace is kept between calls; a taller matrix
+    with fewer columns after a wider one must still get its own layout."""
+    rng = np.random.default_rng(21)
+    a1 = rng.standard_normal((600, 500))
+    q1, r1 = L.qr(a1)
+    assert np.abs(q1 @ r1 - a1).max() <= 64 * 600 * U64 * np.abs(a1).max()
+    a2 = rng.standard_normal((1300, 90))
+    b2 = rng.standard_normal(1300)
+    print("lstsq 1300 x 90 after qr 600 x 500", flush=True)
+    x, res = L.lstsq(a2, b2)
+    xo, reso = oracle.lstsq(a2, b2)
+    assert np.abs(x - xo).max() <= 1e-9 * max(1.0, np.abs(xo).max())
+    assert abs(res - reso) <= 1e-9 * max(1.0, reso)
