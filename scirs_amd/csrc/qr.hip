@@ -273,7 +273,7 @@ long round_up(long x, long m) { return (x + m - 1) / m * m; }
 constexpr int QB = 32;        // panel width
 constexpr int PR = 256;       // rows per panel workgroup, one per thread
 constexpr int QG_MAX = 64;    // panel workgroups: rows - k0 <= QG_MAX * PR
-constexpr int QKC = 32;       // row chunks of the partial products V^T C
+constexpr int QKC = 32;       // row chunks of the partial products V^T C (at most)
 constexpr int QWT = 256;      // columns per partial-product workgroup
 typedef unsigned long long u64;
 constexpr size_t QXW_WORDS = 2ull * QG_MAX * (QB + 1) * 2;   // exchange words (2 per value)
@@ -446,7 +446,8 @@ struct QWArgs {
 };
 __global__ __launch_bounds__(QWT) void qr_wpart_kernel(QWArgs a)
 {
-    __shared__ double Vs[64][QB + 1];
+    // pitch QB: every read is a broadcast, and pairs are 16-byte aligned
+    __shared__ double Vs[64][QB];
     const int t = threadIdx.x, col = blockIdx.x * QWT + t, kc = blockIdx.y;
     const int ntot = QB + a.nR + a.nQ;
     const double *src = nullptr;
@@ -467,10 +468,16 @@ __global__ __launch_bounds__(QWT) void qr_wpart_kernel(QWArgs a)
         __syncthreads();
         const int nrow = min(64, rend - r0);
         if (src)
-            for (int rr = 0; rr < nrow; ++rr) {
-                const double c = src[(long)(r0 + rr) * ld];
+            // sixteen rows' loads in flight before their products (rows past
+            // the chunk read row r0 again and are multiplied by zero rows of Vs)
+            for (int rb = 0; rb < nrow; rb += 16) {
+                double cv[16];
 #pragma unroll
-                for (int jj = 0; jj < QB; ++jj) acc[jj] = fma(Vs[rr][jj], c, acc[jj]);
+                for (int u = 0; u < 16; ++u) cv[u] = src[(long)(r0 + min(rb + u, nrow - 1)) * ld];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+#pragma unroll
+                    for (int jj = 0; jj < QB; ++jj) acc[jj] = fma(Vs[rb + u][jj], cv[u], acc[jj]);
             }
     }
     if (col < ntot)
@@ -521,6 +528,7 @@ __global__ __launch_bounds__(256) void qr_y_kernel(const double *part, long ldp,
     double w[QB];
 #pragma unroll
     for (int jj = 0; jj < QB; ++jj) w[jj] = 0.0;
+#pragma unroll 2
     for (int kc = 0; kc < kcn; ++kc)
 #pragma unroll
         for (int jj = 0; jj < QB; ++jj) w[jj] += part[((long)kc * QB + jj) * ldp + col];
@@ -608,7 +616,10 @@ int householder_blocked(int rows, int cols, int rcols, double *R, long ldr, doub
         q.seq += 2u * QB;
         qr_panel_kernel<<<pa.G, PR, 0, s>>>(pa);
         const int rc0 = k0 + w, nR = rcols - rc0, ntot = QB + nR + nQ;
-        int kcn = std::min(QKC, (rows_k + 63) / 64);
+        // enough row chunks for about 512 workgroups, no more (every chunk adds
+        // a partial that T and Y read back)
+        const int ncb = (ntot + QWT - 1) / QWT;
+        int kcn = std::min(std::min(QKC, (rows_k + 63) / 64), std::max(1, (512 + ncb - 1) / ncb));
         const int rch = (int)round_up((rows_k + kcn - 1) / kcn, 64);
         kcn = (rows_k + rch - 1) / rch;
         QWArgs wa;
