@@ -488,55 +488,60 @@ __global__ __launch_bounds__(QWT) void qr_wpart_kernel(QWArgs a)
 // T (QB x QB, upper) from the Gram matrix V^T V (the partials' first QB
 // columns, added in chunk order): column i is -2 T[0:i, 0:i] (V^T v_i) above
 // T_ii = 2 (LAPACK's forward column-wise T); a skipped column is all zero.
+// Row t of T needs only its own earlier entries and the Gram matrix, so lane
+// t of one wave builds it with no barrier.
 __global__ __launch_bounds__(QB * QB) void qr_tbuild_kernel(const double *part, long ldp, int kcn, const int *flags,
                                                             double *T)
 {
     __shared__ double Gm[QB][QB + 1];
     __shared__ double Ts[QB][QB + 1];
-    __shared__ double z[QB];
     const int t = threadIdx.x, r = t / QB, c = t % QB;
     double sacc = 0.0;
+#pragma unroll 8
     for (int kc = 0; kc < kcn; ++kc) sacc += part[((long)kc * QB + r) * ldp + c];
     Gm[r][c] = sacc;
     Ts[r][c] = 0.0;
     __syncthreads();
-    for (int i = 0; i < QB; ++i) {
-        const bool on = flags[i] != 0;
-        if (t < i) z[t] = on ? -2.0 * Gm[t][i] : 0.0;
-        __syncthreads();
-        if (t < i) {
-            double v = 0.0;
-            for (int q = t; q < i; ++q) v += Ts[t][q] * z[q];
-            Ts[t][i] = v;
+    if (t < QB) {
+        for (int i = 0; i < QB; ++i) {
+            const double tau = flags[i] != 0 ? 2.0 : 0.0;
+            if (t < i) {
+                double v = 0.0;
+                for (int q = t; q < i; ++q) v += Ts[t][q] * (-tau * Gm[q][i]);
+                Ts[t][i] = v;
+            } else if (t == i) {
+                Ts[t][i] = tau;
+            }
         }
-        if (t == 0) Ts[i][i] = on ? 2.0 : 0.0;
-        __syncthreads();
     }
+    __syncthreads();
     T[r * QB + c] = Ts[r][c];
 }
 
 // Y[:, col] = T^T W[:, col], W = the partials added in chunk order, for the
-// virtual columns QB .. ntot-1 (Y is QB x (ntot - QB), ld ldy).
+// virtual columns QB .. ntot-1 (Y is QB x (ntot - QB), ld ldy).  A workgroup
+// takes 8 columns: thread (jj, col) adds the kcn partials of W[jj][col] (all
+// its loads in flight at once), then thread (i, col) forms row i of Y.
+constexpr int QYC = 256 / QB;   // columns per workgroup
 __global__ __launch_bounds__(256) void qr_y_kernel(const double *part, long ldp, int kcn, int ntot, const double *T,
                                                    double *Y, long ldy)
 {
-    __shared__ double Ts[QB][QB];
-    for (int e = threadIdx.x; e < QB * QB; e += 256) Ts[e / QB][e % QB] = T[e];
+    __shared__ double Ts[QB][QB + 1];
+    __shared__ double Wl[QB][QYC];
+    const int t = threadIdx.x, jj = t / QYC, cl = t % QYC;
+    for (int e = t; e < QB * QB; e += 256) Ts[e / QB][e % QB] = T[e];
+    const int col = QB + blockIdx.x * QYC + cl;
+    double w = 0.0;
+    if (col < ntot) {
+#pragma unroll 8
+        for (int kc = 0; kc < kcn; ++kc) w += part[((long)kc * QB + jj) * ldp + col];
+    }
+    Wl[jj][cl] = w;
     __syncthreads();
-    const int col = QB + blockIdx.x * 256 + threadIdx.x;
-    if (col >= ntot) return;
-    double w[QB];
-#pragma unroll
-    for (int jj = 0; jj < QB; ++jj) w[jj] = 0.0;
-#pragma unroll 2
-    for (int kc = 0; kc < kcn; ++kc)
-#pragma unroll
-        for (int jj = 0; jj < QB; ++jj) w[jj] += part[((long)kc * QB + jj) * ldp + col];
-#pragma unroll
-    for (int i = 0; i < QB; ++i) {
+    if (col < ntot) {
+        const int i = jj;
         double y = 0.0;
-#pragma unroll
-        for (int jj = 0; jj <= i; ++jj) y = fma(Ts[jj][i], w[jj], y);
+        for (int q = 0; q <= i; ++q) y = fma(Ts[q][i], Wl[q][cl], y);
         Y[(long)i * ldy + (col - QB)] = y;
     }
 }
@@ -628,7 +633,7 @@ int householder_blocked(int rows, int cols, int rcols, double *R, long ldr, doub
         qr_wpart_kernel<<<dim3((ntot + QWT - 1) / QWT, kcn), QWT, 0, s>>>(wa);
         qr_tbuild_kernel<<<1, QB * QB, 0, s>>>(q.part, ldp, kcn, q.flags, q.T);
         if (nR + nQ > 0) {
-            qr_y_kernel<<<(nR + nQ + 255) / 256, 256, 0, s>>>(q.part, ldp, kcn, ntot, q.T, q.Y, ldp);
+            qr_y_kernel<<<(nR + nQ + QYC - 1) / QYC, 256, 0, s>>>(q.part, ldp, kcn, ntot, q.T, q.Y, ldp);
             const double *Vk = q.V + (size_t)k0 * QB;
             if (nR > 0)
                 if (int rc = scirs_launch_dgemm_sub(rows_k, nR, QB, Vk, QB, q.Y, ldp, 0, R + (long)k0 * ldr + rc0, ldr,
