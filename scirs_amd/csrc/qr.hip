@@ -490,32 +490,36 @@ __global__ __launch_bounds__(QWT) void qr_wpart_kernel(QWArgs a)
 // T_ii = 2 (LAPACK's forward column-wise T); a skipped column is all zero.
 // Row t of T needs only its own earlier entries and the Gram matrix, so lane
 // t of one wave builds it with no barrier.
-__global__ __launch_bounds__(QB * QB) void qr_tbuild_kernel(const double *part, long ldp, int kcn, const int *flags,
-                                                            double *T)
+__global__ __launch_bounds__(256) void qr_tbuild_kernel(const double *part, long ldp, int kcn, const int *flags,
+                                                       double *T)
 {
     __shared__ double Gm[QB][QB + 1];
     __shared__ double Ts[QB][QB + 1];
-    const int t = threadIdx.x, r = t / QB, c = t % QB;
-    double sacc = 0.0;
+    const int t = threadIdx.x;
+    for (int e = t; e < QB * QB; e += 256) {
+        const int r = e / QB, c = e % QB;
+        double sacc = 0.0;
 #pragma unroll 8
-    for (int kc = 0; kc < kcn; ++kc) sacc += part[((long)kc * QB + r) * ldp + c];
-    Gm[r][c] = sacc;
-    Ts[r][c] = 0.0;
-    __syncthreads();
-    if (t < QB) {
-        for (int i = 0; i < QB; ++i) {
-            const double tau = flags[i] != 0 ? 2.0 : 0.0;
-            if (t < i) {
-                double v = 0.0;
-                for (int q = t; q < i; ++q) v += Ts[t][q] * (-tau * Gm[q][i]);
-                Ts[t][i] = v;
-            } else if (t == i) {
-                Ts[t][i] = tau;
-            }
-        }
+        for (int kc = 0; kc < kcn; ++kc) sacc += part[((long)kc * QB + r) * ldp + c];
+        Gm[r][c] = sacc;
     }
     __syncthreads();
-    T[r * QB + c] = Ts[r][c];
+    if (t < QB) {
+        // the row in registers; the Gram entries are broadcast reads
+        double tr[QB];
+#pragma unroll
+        for (int i = 0; i < QB; ++i) {
+            const double tau = flags[i] != 0 ? 2.0 : 0.0;
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < i; ++q) v += (q >= t ? tr[q] : 0.0) * (-tau * Gm[q][i]);
+            tr[i] = t < i ? v : (t == i ? tau : 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < QB; ++i) Ts[t][i] = tr[i];
+    }
+    __syncthreads();
+    for (int e = t; e < QB * QB; e += 256) T[e] = Ts[e / QB][e % QB];
 }
 
 // Y[:, col] = T^T W[:, col], W = the partials added in chunk order, for the
@@ -631,7 +635,7 @@ int householder_blocked(int rows, int cols, int rcols, double *R, long ldr, doub
         wa.V = q.V; wa.R = R; wa.ldr = ldr; wa.rc0 = rc0; wa.nR = nR; wa.Qt = Qt; wa.ldq = ldq; wa.nQ = nQ;
         wa.rows = rows; wa.k0 = k0; wa.rch = rch; wa.part = q.part; wa.ldp = ldp;
         qr_wpart_kernel<<<dim3((ntot + QWT - 1) / QWT, kcn), QWT, 0, s>>>(wa);
-        qr_tbuild_kernel<<<1, QB * QB, 0, s>>>(q.part, ldp, kcn, q.flags, q.T);
+        qr_tbuild_kernel<<<1, 256, 0, s>>>(q.part, ldp, kcn, q.flags, q.T);
         if (nR + nQ > 0) {
             qr_y_kernel<<<(nR + nQ + QYC - 1) / QYC, 256, 0, s>>>(q.part, ldp, kcn, ntot, q.T, q.Y, ldp);
             const double *Vk = q.V + (size_t)k0 * QB;
