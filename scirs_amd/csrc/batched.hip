@@ -105,7 +105,7 @@ __device__ __forceinline__ T team_max(T v, T *red_v)
 // LDS (pan: n x (NB+1)), the trailing update reads the panel from LDS.
 // Returns 0 or the 1-based breakdown column.
 template <int TEAM, typename T>
-__device__ int team_getrf(int n, T *M, int *perm, T *pan, T *red_v, int *red_i, int tid)
+__device__ int team_getrf(int n, T *M, int *perm, T *pan, T *red_v, int *red_i, int tid, T *ustage)
 {
     for (int i = tid; i < n; i += TEAM) perm[i] = i;
     const T eps = num_limits<T>::eps();
@@ -165,23 +165,59 @@ __device__ int team_getrf(int n, T *M, int *perm, T *pan, T *red_v, int *red_i, 
         team_sync<TEAM>();
         const int c1 = k0 + kb;
         if (c1 >= n) continue;
-        // U12 = L11^-1 A12 (rows k0..k0+kb): one column per thread
-        for (int c = c1 + tid; c < n; c += TEAM) {
-            for (int i = 1; i < kb; ++i) {
-                T s = M[(long)(k0 + i) * n + c];
-                for (int q = 0; q < i; ++q) s -= pan[i * (NB + 1) + q] * M[(long)(k0 + q) * n + c];
-                M[(long)(k0 + i) * n + c] = s;
+        const int rows = n - c1, cols = n - c1;
+        if constexpr (sizeof(T) == 4) {
+        // U12 = L11^-1 A12 (rows k0..k0+kb): one column per thread, its kb
+            // entries loaded together into registers, the same subtraction order
+            // as the loop over i and q
+            for (int c = c1 + tid; c < n; c += TEAM) {
+                T u[NB];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) u[i] = i < kb ? M[(long)(k0 + i) * n + c] : (T)0;
+#pragma unroll
+                for (int i = 1; i < NB; ++i) {
+                    // row i of L11 through an offset the compiler sees only once
+                    // u[i - 1] exists, so its loads are not all hoisted (they spill)
+                    int off = i * (NB + 1);
+                    asm volatile("" : "+v"(off) : "v"(u[i - 1]));
+                    const T *li = pan + off;
+                    T sv = u[i];
+#pragma unroll
+                    for (int q = 0; q < i; ++q) sv -= li[q] * u[q];
+                    u[i] = sv;
+                }
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    if (i < kb) M[(long)(k0 + i) * n + c] = u[i];
+                    // rows past kb are zero in the staged copy (the panel's columns past kb are zero too)
+                    if (ustage) ustage[i * cols + (c - c1)] = i < kb ? u[i] : (T)0;
+                }
             }
+        } else {
+            // the FP64 fallback: the column in memory (registers for 32 doubles
+            // would spill in this kernel)
+            for (int c = c1 + tid; c < n; c += TEAM)
+                for (int i = 1; i < kb; ++i) {
+                    T sv = M[(long)(k0 + i) * n + c];
+                    for (int q = 0; q < i; ++q) sv -= pan[i * (NB + 1) + q] * M[(long)(k0 + q) * n + c];
+                    M[(long)(k0 + i) * n + c] = sv;
+                }
         }
         team_sync<TEAM>();
-        // A22 -= L21 U12
-        const int rows = n - c1, cols = n - c1;
+        // A22 -= L21 U12 (U12 from the staged copy when there is one)
         for (int w = tid; w < rows * cols; w += TEAM) {
             const int r = w / cols, c = w % cols;
             const T *lr = pan + (kb + r) * (NB + 1);
-            T s = (T)0;
-            for (int q = 0; q < kb; ++q) s += lr[q] * M[(long)(k0 + q) * n + c1 + c];
-            M[(long)(c1 + r) * n + c1 + c] -= s;
+            T sv = (T)0;
+            if (ustage) {
+#pragma unroll 8
+                for (int q = 0; q < NB; ++q) sv += lr[q] * ustage[q * cols + c];
+            } else {
+#pragma unroll 8
+                for (int q = 0; q < NB; ++q)
+                    if (q < kb) sv += lr[q] * M[(long)(k0 + q) * n + c1 + c];
+            }
+            M[(long)(c1 + r) * n + c1 + c] -= sv;
         }
     }
     team_sync<TEAM>();
@@ -231,21 +267,110 @@ __device__ int team_potrf(int n, T *M, T *pan, int tid)
             if (c > r) continue;
             const T *lr = pan + (kb + r) * (NB + 1);
             const T *lc = pan + (kb + c) * (NB + 1);
-            T s = (T)0;
-            for (int q = 0; q < kb; ++q) s += lr[q] * lc[q];
-            M[(long)(c1 + r) * n + c1 + c] -= s;
+            T sv = (T)0;
+            // the panel's columns past kb are zero
+#pragma unroll 8
+            for (int q = 0; q < NB; ++q) sv += lr[q] * lc[q];
+            M[(long)(c1 + r) * n + c1 + c] -= sv;
         }
     }
     team_sync<TEAM>();
     return 0;
 }
 
+__device__ __forceinline__ double bcast_lane(double v, int q)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), q);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), q);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // d <- op^-1 d with the factors in M (row-major, ld n); d in LDS (length n).
-// Column-oriented sweeps: every thread reads the finished d[i], then the team
+// LU: L unit lower then U; Cholesky: L then L^T.  Blocks of NB rows: the
+// first wave solves the diagonal block (lane i = row i, the solved value
+// broadcast by readlane), then every thread takes one row outside the block
+// and removes the block's contribution.  Each row receives the same
+// subtractions, in the same order, as a column-by-column sweep.
+template <int TEAM, typename T, typename V>
+__device__ void team_subst(int n, const T *M, V *d, bool chol, int tid)
+{
+    const int lane = tid & 63;
+    const bool w0 = tid < 64;
+    // forward: L (unit for LU)
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0);
+        if (w0) {
+            const int i = lane;
+            const bool in = i < kb;
+            const int ri = k0 + min(i, kb - 1);
+            V di = in ? d[k0 + i] : (V)0;
+            T lrow[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) lrow[q] = M[(long)ri * n + k0 + min(q, kb - 1)];
+            const V dg = (V)M[(long)ri * n + ri];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                if (q < kb) {
+                    if (chol && i == q) di /= dg;
+                    const V xq = bcast_lane(di, q);
+                    if (in && i > q) di -= (V)lrow[q] * xq;
+                }
+            }
+            if (in) d[k0 + i] = di;
+        }
+        team_sync<TEAM>();
+        for (int r = k0 + kb + tid; r < n; r += TEAM) {
+            V sv = d[r];
+#pragma unroll 8
+            for (int q = 0; q < NB; ++q)
+                if (q < kb) sv -= (V)M[(long)r * n + k0 + q] * d[k0 + q];
+            d[r] = sv;
+        }
+        team_sync<TEAM>();
+    }
+    // backward: U (LU) or L^T (Cholesky), blocks from the bottom
+    for (int k0 = (n - 1) / NB * NB; k0 >= 0; k0 -= NB) {
+        const int kb = min(NB, n - k0);
+        if (w0) {
+            const int i = lane;
+            const bool in = i < kb;
+            const int ri = k0 + min(i, kb - 1);
+            V di = in ? d[k0 + i] : (V)0;
+            T urow[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int cq = k0 + min(q, kb - 1);
+                urow[q] = chol ? M[(long)cq * n + ri] : M[(long)ri * n + cq];
+            }
+            const V dg = (V)M[(long)ri * n + ri];
+#pragma unroll
+            for (int q = NB - 1; q >= 0; --q) {
+                if (q < kb) {
+                    if (i == q) di /= dg;
+                    const V xq = bcast_lane(di, q);
+                    if (in && i < q) di -= (V)urow[q] * xq;
+                }
+            }
+            if (in) d[k0 + i] = di;
+        }
+        team_sync<TEAM>();
+        for (int r = tid; r < k0; r += TEAM) {
+            V sv = d[r];
+#pragma unroll 8
+            for (int q = NB - 1; q >= 0; --q)
+                if (q < kb) sv -= (V)(chol ? M[(long)(k0 + q) * n + r] : M[(long)r * n + k0 + q]) * d[k0 + q];
+            d[r] = sv;
+        }
+        team_sync<TEAM>();
+    }
+}
+
+// The FP64 fallback's solve, column by column (d in LDS, length n): every thread reads the finished d[i], then the team
 // removes its contribution from the remaining entries (one sync per row).
 // LU: L unit lower then U; Cholesky: L then L^T.
 template <int TEAM, typename T, typename V>
-__device__ void team_subst(int n, const T *M, V *d, bool chol, int tid)
+__device__ void team_subst_cols(int n, const T *M, V *d, bool chol, int tid)
 {
     for (int i = 0; i < n; ++i) {
         V xi = d[i];
@@ -299,8 +424,10 @@ __device__ void team_solve(const BatchArgs &a, int sys, float *pan32, double *pa
         if (!f64) {
             for (int idx = tid; idx < n * n; idx += TEAM) W32[idx] = (float)A[idx];
             team_sync<TEAM>();
+            // U12 is staged in the half of the panel buffer an FP32 panel leaves free
             rc = chol ? team_potrf<TEAM, float>(n, W32, pan32, tid)
-                      : team_getrf<TEAM, float>(n, W32, perm, pan32, redf, redi, tid);
+                      : team_getrf<TEAM, float>(n, W32, perm, pan32, redf, redi, tid,
+                                                pan32 + (size_t)(TEAM == NT ? NMAX : NMAX / 4) * (NB + 1));
             if (rc != 0) continue;   // FP32 breakdown: the same factorisation decides in FP64
         } else {
             for (int idx = tid; idx < n * n; idx += TEAM) W64[idx] = A[idx];
@@ -308,7 +435,7 @@ __device__ void team_solve(const BatchArgs &a, int sys, float *pan32, double *pa
             if (chol) {
                 rc = team_potrf<TEAM, double>(n, W64, pan64, tid);
             } else {
-                rc = team_getrf<TEAM, double>(n, W64, perm, pan64, red, redi, tid);
+                rc = team_getrf<TEAM, double>(n, W64, perm, pan64, red, redi, tid, nullptr);
             }
             if (rc != 0) { status = rc; break; }
         }
@@ -324,15 +451,22 @@ __device__ void team_solve(const BatchArgs &a, int sys, float *pan32, double *pa
             double eta = 1.0, prev = INFINITY;
             int it = 0;
             for (; it <= a.max_iter; ++it) {
-                // r = b - A x  (FP64), stored in vec
+                // r = b - A x  (FP64), stored in vec: one wave per row, the
+                // lanes over the columns (coalesced row reads)
                 double rn = 0.0, xn = 0.0;
-                for (int r = tid; r < n; r += TEAM) {
-                    double s = B[(long)r * nrhs + j];
-                    for (int c = 0; c < n; ++c) s -= A[(long)r * n + c] * X[(long)c * nrhs + j];
-                    vec[r] = s;
-                    rn = fmax(rn, fabs(s));
-                    xn = fmax(xn, fabs(X[(long)r * nrhs + j]));
+                {
+                    const int lane = tid & 63;
+                    for (int r = tid >> 6; r < n; r += TEAM / 64) {
+                        double ax = 0.0;
+                        for (int c = lane; c < n; c += 64) ax += A[(long)r * n + c] * X[(long)c * nrhs + j];
+                        ax = wave_sum(ax);
+                        const double rv = B[(long)r * nrhs + j] - ax;
+                        if (lane == 0) vec[r] = rv;
+                        rn = fmax(rn, fabs(rv));
+                    }
+                    for (int r = tid; r < n; r += TEAM) xn = fmax(xn, fabs(X[(long)r * nrhs + j]));
                 }
+                team_sync<TEAM>();
                 rn = team_max<TEAM>(rn, red);
                 xn = team_max<TEAM>(xn, red);
                 const double den = an * xn + bn;
@@ -349,7 +483,7 @@ __device__ void team_solve(const BatchArgs &a, int sys, float *pan32, double *pa
                     for (int r = tid; r < n; r += TEAM) vec[r] = tmp[r];
                 }
                 team_sync<TEAM>();
-                if (f64) team_subst<TEAM, double, double>(n, W64, vec, chol, tid);
+                if (f64) team_subst_cols<TEAM, double, double>(n, W64, vec, chol, tid);
                 else team_subst<TEAM, float, double>(n, W32, vec, chol, tid);
                 for (int r = tid; r < n; r += TEAM) X[(long)r * nrhs + j] += vec[r];
                 team_sync<TEAM>();
