@@ -138,6 +138,7 @@ int Context::init(int dev)
     work.panel_rows = env_int("SCIRS_GPU_PANEL_ROWS", 192);
     work.panel_w0 = env_int("SCIRS_GPU_PANEL_W0", 0);
     work.lookahead = env_int("SCIRS_GPU_LOOKAHEAD", 1);
+    work.trsv_pair = env_int("SCIRS_GPU_TRSV_PAIR", 1);
     work.poll_sleep = env_int("SCIRS_GPU_POLL_SLEEP", 0);
     work.panel_nrep = std::max(1, std::min((int)SCIRS_PANEL_REP_MAX, env_int("SCIRS_GPU_PANEL_REPLICAS", 1)));
     size_t bytes = 0;

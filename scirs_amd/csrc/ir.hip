@@ -521,7 +521,7 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     const size_t other = ((size_t)n * nrhs + (size_t)n + 256 + (size_t)n + 2 * (size_t)(restart + 1) * n * gnc +
                           (size_t)n * gnc + (size_t)(2 * (restart + 1) + 1) * gnc + (size_t)restart * gnc + 64) *
                              sizeof(double) +
-                         (size_t)n * sizeof(float) + (size_t)n * sizeof(int64_t) + 4 * dinv_elems * sizeof(float) +
+                         (size_t)n * sizeof(float) + (size_t)n * sizeof(int64_t) + 8 * dinv_elems * sizeof(float) +
                          (size_t)gnc * sizeof(int) + 8192;
     if ((rc = c.ensure(&c.scratch_ir, &c.scratch_ir_bytes, a32))) return rc;
     if ((rc = c.ensure(&c.scratch_ir2, &c.scratch_ir2_bytes, other))) return rc;
@@ -546,8 +546,8 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     gw.LU64 = nullptr;
     float *dinv_l = (float *)take(dinv_elems * sizeof(float));
     float *dinv_u = (float *)take(dinv_elems * sizeof(float));
-    float *dmt_l = (float *)take(dinv_elems * sizeof(float));
-    float *dmt_u = (float *)take(dinv_elems * sizeof(float));
+    float *dmt_l = (float *)take(3 * dinv_elems * sizeof(float));   // couplings one to three blocks back
+    float *dmt_u = (float *)take(3 * dinv_elems * sizeof(float));
     const bool tslu = c.work.tslu != 0;
     st.kb = tslu ? c.kb_f32 : c.lu_kb<float>(n);
 
@@ -579,8 +579,9 @@ int gesv_ir_device(int n, int nrhs, const double *dA, long lda, const double *dB
     if (c.trsv_dinv) {
         if ((rc = scirs_tri_block_inverse(n, st.LU, ld32, 1, 1, dinv_l, s))) return rc;
         if ((rc = scirs_tri_block_inverse(n, st.LU, ld32, 0, 0, dinv_u, s))) return rc;
-        if ((rc = scirs_tri_block_coupling(n, st.LU, ld32, 1, dinv_l, dmt_l, s))) return rc;
-        if ((rc = scirs_tri_block_coupling(n, st.LU, ld32, 0, dinv_u, dmt_u, s))) return rc;
+        const int depth = c.work.trsv_pair ? 3 : 1;
+        if ((rc = scirs_tri_block_coupling(n, st.LU, ld32, 1, dinv_l, dmt_l, depth, s))) return rc;
+        if ((rc = scirs_tri_block_coupling(n, st.LU, ld32, 0, dinv_u, dmt_u, depth, s))) return rc;
         st.dinv_l = dinv_l;
         st.dinv_u = dinv_u;
         st.dmt_l = dmt_l;

@@ -40,6 +40,7 @@ struct ScirsWork {
     int *colexp;         // FP16 update: per-column exponents of U12, two regions (main / side stream)
     int colexp_half;     // columns per region
     int lookahead;       // overlap panel k+1 with the rest of update k on side_stream
+    int trsv_pair;       // FP32 one-rhs chained solves with couplings: two block rows per link
     int poll_sleep;      // back-off of the panel's candidate poll (s_sleep(2) repetitions)
     hipStream_t side_stream;
     hipEvent_t ev_trsm, ev_upd, ev_start;
@@ -132,13 +133,17 @@ int scirs_trsv_big(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lo
 // dinv (optional): inverses of the 64 x 64 diagonal blocks (scirs_tri_block_inverse),
 // applied instead of the substitution inside each block
 // dmt (optional, with dinv): Dinv[p] times the tile block row p meets last
-// (scirs_tri_block_coupling)
+// (scirs_tri_block_coupling, depth 3: a one-rhs FP32 solve then hands over
+// two block rows per link, using the couplings two and three blocks back)
 template <typename T>
 int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int lower, int unit, ScirsWork &w,
                      hipStream_t s, const T *dinv = nullptr, const T *dmt = nullptr);
 
-// M[p] = Dinv[p] * (block row p's last tile in the chained solve's order), ceil(n/64) * 64 * 64 floats
-int scirs_tri_block_coupling(int n, const float *A, long lda, int lower, const float *dinv, float *dmt, hipStream_t s);
+// M_d[p] = Dinv[p] * (block row p's d-th from last tile in the chained solve's
+// order) for d = 1 .. depth (<= 3): depth * ceil(n/64) * 64 * 64 floats, M_d at
+// (d - 1) * ceil(n/64) * 64 * 64
+int scirs_tri_block_coupling(int n, const float *A, long lda, int lower, const float *dinv, float *dmt, int depth,
+                             hipStream_t s);
 
 // Dinv[p] = (T_pp)^-1 for the 64 x 64 diagonal blocks of the lower (unit or
 // not) or upper triangle of the FP32 matrix A; dinv holds ceil(n/64) * 64 * 64 floats

@@ -54,25 +54,62 @@ __device__ __forceinline__ double readlane<double>(double v, int k)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Poll until every active lane holds the QB tagged values at src (stride W
-// words per value).  Gives up (returns false) after spin_limit polls or when
-// another workgroup raised *abort.
-template <typename T, int QB>
-__device__ bool poll_vals(const u64 *src, bool active, unsigned seq, T (&v)[QB], unsigned spin_limit, int *abort,
+// Poll until every lane holds its first nvalid of the NV tagged values at src
+// (NV * W consecutive words; the values past nvalid are not read and come back
+// as zero, and a lane with nvalid = 0 takes no part).  PD reads of the words
+// are kept in flight, issued one after another and each checked as it
+// returns, so a value is seen a fraction of a memory round trip after it lands
+// instead of up to a whole one.  Gives up (returns false) after spin_limit
+// rounds of PD reads or when another workgroup raised *abort.
+template <typename T, int NV, int PD>
+__device__ bool poll_vals(const u64 *src, int nvalid, unsigned seq, T (&v)[NV], unsigned spin_limit, int *abort,
                           int *info)
 {
-    constexpr int W = Tag<T>::W;
+    constexpr int W = Tag<T>::W, NWD = NV * W;
+    const bool active = nvalid > 0;
+    const int nw = nvalid * W;
+    const unsigned long long am = __ballot(active);
+    if (am == 0ull) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = (T)0;
+        return true;
+    }
+    // words that are not polled are read from a live address (this lane's
+    // first word, or the first active lane's) and ignored
+    const int first = __ffsll(am) - 1;
+    const uintptr_t sa = (uintptr_t)src;
+    const uintptr_t s0 = (uintptr_t)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(sa & 0xffffffffu), first) |
+                         ((uintptr_t)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(sa >> 32), first) << 32);
+    const u64 *p = active ? src : reinterpret_cast<const u64 *>(s0);
+    u64 w[PD][NWD];
+#pragma unroll
+    for (int d = 0; d < PD; ++d)
+#pragma unroll
+        for (int i = 0; i < NWD; ++i) w[d][i] = ld_agent(p + (i < nw ? i : 0));
     unsigned spins = 0;
     while (true) {
-        bool ok = true;
-        if (active) {
 #pragma unroll
-            for (int q = 0; q < QB; ++q) ok = Tag<T>::get(src + q * W, seq, v[q]) && ok;
-        } else {
+        for (int d = 0; d < PD; ++d) {
+            bool ok = true;
 #pragma unroll
-            for (int q = 0; q < QB; ++q) v[q] = (T)0;
+            for (int i = 0; i < NWD; ++i) ok = ok && (i >= nw || (unsigned)(w[d][i] >> 32) == seq);
+            if (__all(ok)) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    if constexpr (W == 1) {
+                        v[q] = q < nvalid ? __uint_as_float((unsigned)w[d][q]) : (T)0;
+                    } else {
+                        // the layout of Tag<double>::put: low word, then high word
+                        const u64 lo = w[d][q * W], hi = w[d][q * W + 1];
+                        v[q] = q < nvalid ? __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)))
+                                          : (T)0;
+                    }
+                }
+                return true;
+            }
+#pragma unroll
+            for (int i = 0; i < NWD; ++i) w[d][i] = ld_agent(p + (i < nw ? i : 0));
         }
-        if (__all(ok)) return true;
         ++spins;
         if (spins > spin_limit) {
             // -1: a chained triangular solve timed out; info != 0 stops every
@@ -121,6 +158,8 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
 
     if (wave > 0) {
         // ---- off-diagonal tiles of dependencies 0 .. b-2, round robin over waves 1-3
+        //      (reads in flight per poll: two in FP32, one in FP64, whose registers are short)
+        constexpr int SPD = sizeof(T) == 4 ? 2 : 1;
         const int g = lane >> 4, c = (lane & 15) * 4;
         T acc[16][QB];
 #pragma unroll
@@ -138,20 +177,19 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
 #pragma unroll
                 for (int e = 0; e < 4; ++e) a[m][e] = (row < bs && c0 + c + e < n) ? src[e] : (T)0;
             }
-            T xv[4][QB];
-            bool ok = true;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int col = c0 + c + e;
-                ok = ok && poll_vals<T, QB>(xch + (size_t)col * QB * W, col < n, seq, xv[e], spin_limit, abort, info);
+            // the lane's four columns' values are consecutive words: one poll
+            T xv[4 * QB];
+            const int nv = min(max(n - (c0 + c), 0), 4) * QB;
+            if (!poll_vals<T, 4 * QB, SPD>(xch + (size_t)(c0 + c) * QB * W, nv, seq, xv, spin_limit, abort, info)) {
+                failed = true;
+                break;
             }
-            if (!ok) { failed = true; break; }
 #pragma unroll
             for (int m = 0; m < 16; ++m)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
 #pragma unroll
-                    for (int q = 0; q < QB; ++q) acc[m][q] += a[m][e] * xv[e][q];
+                    for (int q = 0; q < QB; ++q) acc[m][q] += a[m][e] * xv[e * QB + q];
         }
         // reduce over the 16 lanes that share a row; lane (m) of each group keeps row 4m+g
 #pragma unroll
@@ -188,7 +226,8 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
     __syncthreads();
     if (wave != 0) return;
 
-    // ---- critical path (wave 0, lane = row of the block)
+    // ---- critical path (wave 0, lane = row of the block); reads in flight per poll
+    constexpr int CPD = sizeof(T) == 4 ? 4 : 2;
     T s[QB];
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -211,7 +250,8 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
             const int cb = LOWER ? b - 1 : nblk - b;
             const int c0 = cb * TB;
             T xk[QB];
-            if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort, info))
+            if (!poll_vals<T, QB, CPD>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n ? QB : 0, seq, xk, spin_limit,
+                                     abort, info))
                 return;
 #pragma unroll
             for (int k = 0; k < TB; ++k) {
@@ -227,7 +267,8 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         const int cb = LOWER ? b - 1 : nblk - b;
         const int c0 = cb * TB;
         T xk[QB];
-        if (!poll_vals<T, QB>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n, seq, xk, spin_limit, abort, info))
+        if (!poll_vals<T, QB, CPD>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n ? QB : 0, seq, xk, spin_limit,
+                                 abort, info))
             failed = true;
         if (failed) return;
 #pragma unroll
@@ -302,6 +343,187 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
     }
 }
 
+// Two block rows per link: the FP32 single right-hand-side solve with the
+// inverted diagonal blocks D_p^-1 and the couplings M_d[p] = D_p^-1 T(p, p-d)
+// (d = 1, 2, 3 blocks back in dependency order).  Link q solves positions
+// e = 2q and o = 2q + 1 (block rows p_e, p_o), which depend on the previous
+// link's pair (e-1, e-2) and on their own streamed partial sums:
+//   x_e = y_e - M_2[e] x_{e-2} - M_1[e] x_{e-1}
+//   x_o = y_o - M_3[o] x_{e-2} - M_2[o] x_{e-1} - M_1[o] x_e
+// with y = D^-1 (b - the products with the blocks before e-2).  Wave 0 finds
+// x_e and publishes it at once (the next link's first product needs only it),
+// wave 1 finds x_o, taking x_e from wave 0 through LDS; waves 2-7 stream the
+// off-diagonal tiles, three per half.  One global hand-over per 128 rows
+// instead of per 64: the hand-over (a store becoming visible to another CU,
+// about 2 us) bounds the chain.
+constexpr int NTP = 512;
+constexpr int PLM = 7;   // LDS matrices: D_e^-1, M_2[e], M_1[e]; D_o^-1, M_3[o], M_2[o], M_1[o]
+
+template <bool LOWER>
+__global__ __launch_bounds__(NTP) void trsv_pair_kernel(const float *__restrict__ A, long lda, float *X, int n,
+                                                        u64 *xch, unsigned seq, int *ticket, int *abort,
+                                                        unsigned spin_limit, int *info,
+                                                        const float *__restrict__ dinv, const float *__restrict__ dm)
+{
+    if (info && *info != 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *Mx = reinterpret_cast<float *>(smem_raw);   // [PLM][TB][LDP]
+    float *part = Mx + PLM * TB * LDP;                 // [6][TB] partial sums of waves 2-7
+    // [TB] x_e for wave 1, each value with its own flag in the high word
+    // (1: valid, 2: wave 0 gave up), so no fence orders value and flag
+    u64 *xe_w = reinterpret_cast<u64 *>(part + 6 * TB);
+    __shared__ int s_ticket;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t == 0) s_ticket = atomicAdd(ticket, 1);
+    if (t < TB) xe_w[t] = 0ull;
+    __syncthreads();
+    const int q = s_ticket;
+    const int nblk = (n + TB - 1) / TB;
+    const int e = 2 * q, o = e + 1;
+    const bool has_o = o < nblk;
+    const size_t mstride = (size_t)nblk * TB * TB;   // between the couplings of distance d and d + 1
+    auto blk = [&](int pos) { return LOWER ? pos : nblk - 1 - pos; };
+
+    if (wave >= 2) {
+        // ---- off-diagonal tiles of positions 0 .. e-3: half h = ws & 1, positions j = ws/2 mod 3
+        // (a half past the last block row streams nothing, but still meets the barrier)
+        const int ws = wave - 2, h = ws & 1;
+        const int p = blk(e + h), r0 = p * TB, bs = min(TB, n - r0);
+        const int jend = (h == 1 && !has_o) ? 0 : e - 2;
+        const int g = lane >> 4, c = (lane & 15) * 4;
+        float acc[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) acc[m] = 0.0f;
+        bool failed = false;
+        for (int j = ws >> 1; j < jend; j += 3) {
+            const int c0 = blk(j) * TB;
+            float a[16][4];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int row = 4 * m + g;
+                const float *src = A + (long)(r0 + row) * lda + c0 + c;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[m][u] = (row < bs && c0 + c + u < n) ? src[u] : 0.0f;
+            }
+            float xv[4];
+            const int nv = min(max(n - (c0 + c), 0), 4);
+            if (!poll_vals<float, 4, 2>(xch + (size_t)(c0 + c), nv, seq, xv, spin_limit, abort, info)) {
+                failed = true;
+                break;
+            }
+#pragma unroll
+            for (int m = 0; m < 16; ++m)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[m] += a[m][u] * xv[u];
+        }
+        if (failed) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) acc[m] = 0.0f;   // the chain is abandoned; keep the barrier
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            float v = acc[m];
+#pragma unroll
+            for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((lane & 15) == m) part[ws * TB + 4 * m + g] = v;
+        }
+    } else if (wave == 0 || has_o) {
+        // ---- waves 0 and 1 stage their half's inverse and couplings (row i, column lane)
+        const int h = wave, pos = e + h, p = blk(pos);
+        float *D = Mx + (h == 0 ? 0 : 3) * TB * LDP;
+        const float *Dp = dinv + (size_t)p * TB * TB;
+        for (int i = 0; i < TB; ++i) D[i * LDP + lane] = Dp[i * TB + lane];
+        const int nm = h == 0 ? 2 : 3;   // couplings to stage: d = 2, 1 (e) or d = 3, 2, 1 (o)
+        for (int k = 0; k < nm; ++k) {
+            const int d = nm - k;
+            if (pos - d < 0) continue;   // no such dependency
+            const float *Mp = dm + (size_t)(d - 1) * mstride + (size_t)p * TB * TB;
+            float *Ml = D + (1 + k) * TB * LDP;
+            for (int i = 0; i < TB; ++i) Ml[i * LDP + lane] = Mp[i * TB + lane];
+        }
+    }
+    __syncthreads();
+    if (wave >= 2 || (wave == 1 && !has_o)) return;
+
+    // ---- critical path (wave h, lane = row of the block)
+    const int h = wave, pos = e + h, p = blk(pos), r0 = p * TB, bs = min(TB, n - r0);
+    const float *D = Mx + (h == 0 ? 0 : 3) * TB * LDP;
+    float sv = lane < bs ? X[r0 + lane] : 0.0f;
+    sv -= part[h * TB + lane] + part[(h + 2) * TB + lane] + part[(h + 4) * TB + lane];
+    float y = 0.0f;
+#pragma unroll
+    for (int k = 0; k < TB; ++k) y += D[lane * LDP + k] * readlane(sv, k);
+    // the coupling with the last dependency (x_{e-1} for wave 0, x_e for wave 1)
+    // is applied after its x arrives: its row is held in registers beforehand
+    float m1[TB];
+    {
+        const float *M1 = D + (h == 0 ? 2 : 3) * TB * LDP;
+#pragma unroll
+        for (int k = 0; k < TB; ++k) m1[k] = M1[lane * LDP + k];
+#pragma unroll
+        for (int k = 0; k < TB; ++k) asm volatile("" : "+v"(m1[k]));
+    }
+    bool ok = true;
+    if (q > 0) {
+        // the previous link's pair: x_{e-2} first (published first), then x_{e-1}
+        float xk[1];
+        int c0 = blk(e - 2) * TB;
+        ok = poll_vals<float, 1, 4>(xch + (size_t)(c0 + lane), c0 + lane < n ? 1 : 0, seq, xk, spin_limit, abort, info);
+        if (ok) {
+            const float *Ml = D + TB * LDP;   // M_2[e] or M_3[o]
+#pragma unroll
+            for (int k = 0; k < TB; ++k) y -= Ml[lane * LDP + k] * readlane(xk[0], k);
+            c0 = blk(e - 1) * TB;
+            ok = poll_vals<float, 1, 4>(xch + (size_t)(c0 + lane), c0 + lane < n ? 1 : 0, seq, xk, spin_limit, abort,
+                                        info);
+        }
+        if (ok) {
+            if (h == 0) {
+#pragma unroll
+                for (int k = 0; k < TB; ++k) y -= m1[k] * readlane(xk[0], k);
+            } else {
+                const float *Ml = D + 2 * TB * LDP;   // M_2[o]
+#pragma unroll
+                for (int k = 0; k < TB; ++k) y -= Ml[lane * LDP + k] * readlane(xk[0], k);
+            }
+        }
+    }
+    if (h == 0) {
+        if (ok && lane < bs) {
+            X[r0 + lane] = y;
+            Tag<float>::put(xch + r0 + lane, seq, y);
+        }
+        // every path hands wave 1 its words (flag 2: give up)
+        __hip_atomic_store(&xe_w[lane], ((u64)(ok ? 1u : 2u) << 32) | __float_as_uint(y), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+    }
+    if (!ok) return;
+    // wave 1: x_e from wave 0 (a bounded wait: wave 0 writes its words on every path)
+    u64 wv = 0;
+    unsigned spins = 0;
+    while (true) {
+        wv = __hip_atomic_load(&xe_w[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (__all((wv >> 32) != 0ull)) break;
+        if (++spins > 64u * spin_limit) {
+            if (lane == 0) {
+                __hip_atomic_store(abort, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (info) __hip_atomic_store(info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (__any((wv >> 32) != 1ull)) return;
+    const float xe = __uint_as_float((unsigned)wv);
+#pragma unroll
+    for (int k = 0; k < TB; ++k) y -= m1[k] * readlane(xe, k);
+    if (lane < bs) {
+        X[r0 + lane] = y;
+        Tag<float>::put(xch + r0 + lane, seq, y);
+    }
+}
+
 template <typename T, int QB, bool LOWER, bool UNIT>
 int launch_chain(int n, int nr, const T *A, long lda, T *X, long ldx, ScirsWork &w, hipStream_t s, const T *dinv,
                  const T *dmt)
@@ -334,6 +556,23 @@ int chain_dispatch(int n, int nr, const T *A, long lda, T *X, long ldx, int lowe
     return launch_chain<T, QB, false, false>(n, nr, A, lda, X, ldx, w, s, dinv, dmt);
 }
 
+int launch_pair(int n, const float *A, long lda, float *X, ScirsWork &w, hipStream_t s, bool lower,
+                const float *dinv, const float *dm)
+{
+    const auto kern = lower ? trsv_pair_kernel<true> : trsv_pair_kernel<false>;
+    const size_t lds = ((size_t)PLM * TB * LDP + 8 * (size_t)TB) * sizeof(float);
+    int rc = scirs_allow_lds((const void *)kern, lds);
+    if (rc) return rc;
+    if ((rc = scirs_fill_int(w.ticket, 1, 0, s))) return rc;
+    unsigned seq = 0;
+    if ((rc = scirs_next_seq(w, 512u, s, &seq))) return rc;
+    const int nblk = (n + TB - 1) / TB;
+    hipLaunchKernelGGL(kern, dim3((nblk + 1) / 2), dim3(NTP), lds, s, A, lda, X, n, w.xtri, seq, w.ticket, w.abort,
+                       w.spin_limit, w.info, dinv, dm);
+    SCIRS_HIP_CHECK(hipGetLastError());
+    return SCIRS_OK;
+}
+
 } // namespace
 
 template <typename T>
@@ -355,6 +594,10 @@ int scirs_trsv_chain(int n, int nrhs, const T *A, long lda, T *X, long ldx, int 
         return SCIRS_ERR_ARG;
     }
     scirs::ProfScope prof(SCIRS_PROF_TRSV, (double)n * n * nrhs, (0.5 * n * n + 2.0 * n * nrhs) * sizeof(T), s);
+    if constexpr (sizeof(T) == 4) {
+        if (nrhs == 1 && dinv && dmt && w.trsv_pair && n > 2 * TB)
+            return launch_pair(n, A, lda, X, w, s, lower != 0, dinv, dmt);
+    }
     for (int rb = 0; rb < nrhs; rb += per_launch) {
         const int nr = std::min(per_launch, nrhs - rb);
         int rc;
@@ -418,10 +661,11 @@ int scirs_tri_block_inverse(int n, const float *A, long lda, int lower, int unit
     return SCIRS_OK;
 }
 
-// M[p] = Dinv[p] * T_p for the tile T_p that block row p meets last in the
-// chained solve (block column p-1 of a lower triangle, p+1 of an upper one),
-// so a link applies D^-1 to its partial sums before that dependency's x
-// arrives and only M[p] x after.  One workgroup per block row.
+// M_d[p] = Dinv[p] * T(p, p -+ d) for the tile block row p meets d-th from
+// last in the chained solve (block column p-d of a lower triangle, p+d of an
+// upper one), d = 1 .. depth, so a link applies D^-1 to its partial sums
+// before those dependencies' x arrive and only M_d[p] x after.  One workgroup
+// per (block row, d); M_d at dmt + (d-1) * ceil(n/64) * 64 * 64.
 namespace {
 template <bool LOWER>
 __global__ __launch_bounds__(256) void tri_block_coupling_kernel(int n, const float *__restrict__ A, long lda,
@@ -429,9 +673,10 @@ __global__ __launch_bounds__(256) void tri_block_coupling_kernel(int n, const fl
                                                                  float *__restrict__ dmt)
 {
     __shared__ float Dl[TB][TB + 1], Tl[TB][TB + 1];
-    const int p = blockIdx.x, nblk = (n + TB - 1) / TB;
-    const int cb = LOWER ? p - 1 : p + 1;
+    const int p = blockIdx.x, nblk = (n + TB - 1) / TB, d = blockIdx.y + 1;
+    const int cb = LOWER ? p - d : p + d;
     if (cb < 0 || cb >= nblk) return;
+    dmt += (size_t)blockIdx.y * nblk * TB * TB;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r0 = p * TB, bs = min(TB, n - r0), c0 = cb * TB, cs = min(TB, n - c0);
     const float *Dp = dinv + (size_t)p * TB * TB;
@@ -450,12 +695,18 @@ __global__ __launch_bounds__(256) void tri_block_coupling_kernel(int n, const fl
 }
 } // namespace
 
-int scirs_tri_block_coupling(int n, const float *A, long lda, int lower, const float *dinv, float *dmt, hipStream_t s)
+int scirs_tri_block_coupling(int n, const float *A, long lda, int lower, const float *dinv, float *dmt, int depth,
+                             hipStream_t s)
 {
     if (n <= 0) return SCIRS_OK;
+    if (depth < 1 || depth > 3) {
+        scirs_set_error("tri_block_coupling: depth %d outside 1..3", depth);
+        return SCIRS_ERR_ARG;
+    }
     const int nblk = (n + TB - 1) / TB;
-    if (lower) hipLaunchKernelGGL(tri_block_coupling_kernel<true>, dim3(nblk), dim3(256), 0, s, n, A, lda, dinv, dmt);
-    else hipLaunchKernelGGL(tri_block_coupling_kernel<false>, dim3(nblk), dim3(256), 0, s, n, A, lda, dinv, dmt);
+    const dim3 grid(nblk, depth);
+    if (lower) hipLaunchKernelGGL(tri_block_coupling_kernel<true>, grid, dim3(256), 0, s, n, A, lda, dinv, dmt);
+    else hipLaunchKernelGGL(tri_block_coupling_kernel<false>, grid, dim3(256), 0, s, n, A, lda, dinv, dmt);
     SCIRS_HIP_CHECK(hipGetLastError());
     return SCIRS_OK;
 }
