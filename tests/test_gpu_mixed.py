@@ -73,6 +73,22 @@ def test_mixed_modes_reach_fp64(L, mode, n):
     assert np.abs(x - xo).max() / np.abs(xo).max() <= bound
 
 
+@pytest.mark.parametrize("n", [130, 700, 1300])
+def test_refinement_links_with_odd_block_count(L, n):
+    """The one-rhs FP32 refinement solves hand over two 64-row block rows per
+    link; an odd number of blocks (3, 11, 21) leaves the last link one.  A wrong
+    link would stall the refinement: the solve must stay on plain refinement
+    (path 2) in as few steps as on even counts, and meet the FP64 bars."""
+    kappa = 1e4
+    a, b, x_true = conditioned(n, kappa, seed=7 * n)
+    print(f"mixed bf16x6 n={n} (odd block count)", flush=True)
+    x, info = L.gesv_ir(a, b, mode=MODES["bf16x6"])
+    detail = (n, info.path, info.ir_iters, info.backward_error)
+    assert info.path == 2 and info.ir_iters <= 6, detail
+    assert eta(a, x, b) <= n * U64, detail
+    assert np.abs(x - x_true).max() / np.abs(x_true).max() <= 100 * kappa * n * U64
+
+
 def test_mixed_multiple_rhs(L):
     n, nrhs, kappa = 777, 5, 1e3
     a, b, x_true = conditioned(n, kappa, seed=5, nrhs=nrhs)
