@@ -310,6 +310,17 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
 {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int w = a.w, r0 = a.c0, cur0 = a.c0 - a.k0, rest0 = cur0 + w, nrest = a.kb - rest0, kb = a.kb;
+    // the earlier sub-panels' interchange map (final when this launch began),
+    // read now so that its latency passes under steps 1-5 (barriers here wait
+    // for LDS only, not for global loads)
+    static_assert(TSLU_MAP == 2 * NT, "two map entries per thread");
+    int map_pos[2], map_lab[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        map_pos[x] = a.first ? 0 : a.map[t + x * NT];
+        map_lab[x] = a.first ? 0 : a.map[TSLU_MAP + t + x * NT];
+    }
+    const int map_count = a.first ? 0 : a.map[2 * TSLU_MAP];
     // 1. pivot positions in pick order; the breakdown test |u_jj| < eps (the
     //    absolute test of lapack.rs:106-120: the mixed solve then falls back to
     //    FP64 LU, which decides singularity)
@@ -348,14 +359,18 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
     }
     __syncthreads();
     const int nd = L.nd;
-    // 3. the pivot rows' panel rows into LDS and the displaced rows staged whole:
-    //    16-byte loads, all issued before any is stored
-    {
-        constexpr int JOBS = 2 * TW * (KBT / 4) / NT;   // (row, 16-byte chunk) pairs per thread
+    // 3 + 4 side by side.  Waves 2-3: the pivot rows' panel rows into LDS and
+    // the displaced rows staged whole (16-byte loads, all issued before any is
+    // stored).  Waves 0-1: U11^-1 and (L11^-1)^T = the inverse of L11^T (upper,
+    // unit diagonal), which need only the elimination's L11 \ U11.
+    if (wave >= 2) {
+        constexpr int NLD = NT / 2;                        // threads of waves 2-3
+        constexpr int JOBS = 2 * TW * (KBT / 4) / NLD;     // (row, 16-byte chunk) pairs per thread
+        const int tt = t - NLD;
         float4 q[JOBS];
 #pragma unroll
         for (int u = 0; u < JOBS; ++u) {
-            const int job = t + u * NT, r = job / (KBT / 4), c4 = job % (KBT / 4);
+            const int job = tt + u * NLD, r = job / (KBT / 4), c4 = job % (KBT / 4);
             const bool piv = r < TW;
             const int ri = piv ? r : r - TW;
             const bool ok = (piv ? ri < w : ri < nd) && 4 * c4 < kb;
@@ -374,7 +389,7 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
         }
 #pragma unroll
         for (int u = 0; u < JOBS; ++u) {
-            const int job = t + u * NT, r = job / (KBT / 4), c4 = job % (KBT / 4);
+            const int job = tt + u * NLD, r = job / (KBT / 4), c4 = job % (KBT / 4);
             const bool piv = r < TW;
             const int ri = piv ? r : r - TW;
             if ((piv ? ri < w : ri < nd) && 4 * c4 < kb) {
@@ -382,14 +397,13 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
                 else *reinterpret_cast<float4 *>(a.stage + ri * SCIRS_KB_MAX + 4 * c4) = q[u];
             }
         }
+    } else if (wave == 0) {
+        tri_inv_wave<false, false>(&L.lu[0][0], LP, &L.uinv[0][0], LP);
+        if (t < nd) a.vlist[t] = L.vlist[t];
+        if (t == 0) a.vlist[TW] = nd;
+    } else {
+        tri_inv_wave<true, true>(&L.lu[0][0], LP, &L.linvT[0][0], LP);
     }
-    if (t < nd) a.vlist[t] = L.vlist[t];
-    if (t == 0) a.vlist[TW] = nd;
-    __syncthreads();
-    // 4. U11^-1 (wave 0) and (L11^-1)^T = the inverse of L11^T, upper with a
-    //    unit diagonal (wave 1), side by side
-    if (wave == 0) tri_inv_wave<false, false>(&L.lu[0][0], LP, &L.uinv[0][0], LP);
-    else if (wave == 1) tri_inv_wave<true, true>(&L.lu[0][0], LP, &L.linvT[0][0], LP);
     __syncthreads();
     // 5. U12 = L11^-1 A12 on the later columns (4 x 4 register tiles)
     constexpr int TJ = KBT / 4;
@@ -446,14 +460,17 @@ __device__ __forceinline__ void root_work(const TourArgs &a, int nsel, const int
     }
     // 6. the panel's interchanges: (position, original row) for every moved row
     if (t == 0) {
-        L.mcount = a.first ? 0 : a.map[2 * TSLU_MAP];
+        L.mcount = map_count;
         L.nmv = 0;
     }
     __syncthreads();
-    const int mc0 = L.mcount;
-    for (int q = t; q < mc0; q += NT) {
-        L.mpos[q] = a.map[q];
-        L.mlab[q] = a.map[TSLU_MAP + q];
+    const int mc0 = map_count;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        if (t + x * NT < mc0) {
+            L.mpos[t + x * NT] = map_pos[x];
+            L.mlab[t + x * NT] = map_lab[x];
+        }
     }
     if (t < w && L.gp[t] != r0 + t) {
         const int m = atomicAdd(&L.nmv, 1);
