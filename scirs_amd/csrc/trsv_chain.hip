@@ -157,15 +157,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
     bool failed = false;
 
     if (wave > 0) {
-        // ---- off-diagonal tiles of dependencies 0 .. b-2, round robin over waves 1-3.
-        //      Each lane: 4 consecutive columns of 16 rows; the lane's four
-        //      columns' x values are consecutive words, read by one poll.  The
-        //      one-rhs FP64 chain takes the tile in 8-row halves (the first
-        //      half's loads in flight with the poll), which keeps it at two
-        //      waves per SIMD; two FP64 right-hand sides measured no faster
-        //      with their tile split to fit two workgroups per CU.
-        constexpr int HR = (sizeof(T) == 8 && QB == 1) ? 8 : 16;   // rows of a tile part
-        constexpr int SPD = sizeof(T) == 4 ? 2 : 1;    // poll reads in flight
+        // ---- off-diagonal tiles of dependencies 0 .. b-2, round robin over waves 1-3
         const int g = lane >> 4, c = (lane & 15) * 4;
         T acc[16][QB];
 #pragma unroll
@@ -175,33 +167,29 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         for (int j = wave - 1; j < b - 1; j += 3) {
             const int cb = LOWER ? j : nblk - 1 - j;
             const int c0 = cb * TB;
-            T a[HR][4];
-            auto load_half = [&](int m0) {
+            T a[16][4];
 #pragma unroll
-                for (int m = 0; m < HR; ++m) {
-                    const int row = 4 * (m0 + m) + g;
-                    const T *src = A + (long)(r0 + row) * lda + c0 + c;
+            for (int m = 0; m < 16; ++m) {
+                const int row = 4 * m + g;
+                const T *src = A + (long)(r0 + row) * lda + c0 + c;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) a[m][e] = (row < bs && c0 + c + e < n) ? src[e] : (T)0;
-                }
-            };
-            load_half(0);
-            T xv[4 * QB];
-            const int nv = min(max(n - (c0 + c), 0), 4) * QB;
-            if (!poll_vals<T, 4 * QB, SPD>(xch + (size_t)(c0 + c) * QB * W, nv, seq, xv, spin_limit, abort, info)) {
-                failed = true;
-                break;
+                for (int e = 0; e < 4; ++e) a[m][e] = (row < bs && c0 + c + e < n) ? src[e] : (T)0;
             }
+            T xv[4][QB];
+            bool ok = true;
 #pragma unroll
-            for (int m0 = 0; m0 < 16; m0 += HR) {
-                if (m0 > 0) load_half(m0);
-#pragma unroll
-                for (int m = 0; m < HR; ++m)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-#pragma unroll
-                        for (int q = 0; q < QB; ++q) acc[m0 + m][q] += a[m][e] * xv[e * QB + q];
+            for (int e = 0; e < 4; ++e) {
+                const int col = c0 + c + e;
+                ok = ok && poll_vals<T, QB, 1>(xch + (size_t)col * QB * W, col < n ? QB : 0, seq, xv[e], spin_limit,
+                                               abort, info);
             }
+            if (!ok) { failed = true; break; }
+#pragma unroll
+            for (int m = 0; m < 16; ++m)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int q = 0; q < QB; ++q) acc[m][q] += a[m][e] * xv[e][q];
         }
         // reduce over the 16 lanes that share a row; lane (m) of each group keeps row 4m+g
 #pragma unroll
@@ -238,8 +226,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
     __syncthreads();
     if (wave != 0) return;
 
-    // ---- critical path (wave 0, lane = row of the block); reads in flight per poll
-    constexpr int CPD = sizeof(T) == 4 ? 4 : 1;
+    // ---- critical path (wave 0, lane = row of the block)
     T s[QB];
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -262,7 +249,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
             const int cb = LOWER ? b - 1 : nblk - b;
             const int c0 = cb * TB;
             T xk[QB];
-            if (!poll_vals<T, QB, CPD>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n ? QB : 0, seq, xk, spin_limit,
+            if (!poll_vals<T, QB, 1>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n ? QB : 0, seq, xk, spin_limit,
                                      abort, info))
                 return;
 #pragma unroll
@@ -279,7 +266,7 @@ __global__ __launch_bounds__(NTC) void trsv_chain_kernel(const T *__restrict__ A
         const int cb = LOWER ? b - 1 : nblk - b;
         const int c0 = cb * TB;
         T xk[QB];
-        if (!poll_vals<T, QB, CPD>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n ? QB : 0, seq, xk, spin_limit,
+        if (!poll_vals<T, QB, 1>(xch + (size_t)(c0 + lane) * QB * W, c0 + lane < n ? QB : 0, seq, xk, spin_limit,
                                  abort, info))
             failed = true;
         if (failed) return;
