@@ -1,5 +1,6 @@
 """Bounded waits.  The multi-workgroup LU panel and the chained triangular
-solve hand data between workgroups through polled, sequence-tagged words;
+solve (one and two block rows per link) hand data between workgroups through
+polled, sequence-tagged words;
 every poll gives up after SCIRS_GPU_SPIN_LIMIT tries.  With a limit of 1 the
 polls give up at once, so the timeout path runs while every kernel still exits
 normally: the call must return SCIRS_ERR_TIMEOUT (-4) with the place it timed
@@ -42,6 +43,15 @@ try:
     print("NO_TIMEOUT chain", flush=True)
 except L.GpuSolveError as e:
     print("RC_CHAIN", e.rc, "|", N.last_error(), flush=True)
+print("case: mixed solve n=1000 (two-block refinement links), spin limit 1", flush=True)
+a3 = rng.standard_normal((1000, 1000)) + 40.0 * np.eye(1000); b3 = rng.standard_normal(1000)
+try:
+    x3, info3 = L.gesv_ir(a3, b3)
+    r3 = b3 - a3 @ x3
+    eta3 = np.abs(r3).max() / (np.abs(a3).sum(axis=1).max() * np.abs(x3).max() + np.abs(b3).max())
+    print("PAIR_PATH", info3.path, eta3, flush=True)
+except L.GpuSolveError as e:
+    print("RC_PAIR", e.rc, "|", N.last_error(), flush=True)
 print("case: FP64 LU n=300 after the timeouts", flush=True)
 os.environ["SCIRS_GPU_TRSV"] = "big"
 a2 = rng.standard_normal((300, 300)); b2 = rng.standard_normal(300)
@@ -63,5 +73,13 @@ def test_timeouts_report_and_recover():
     chain = [l for l in lines if l.startswith("RC_CHAIN")]
     assert panel and panel[0].split()[1] == "-4" and "timed out at column" in panel[0], lines
     assert chain and chain[0].split()[1] == "-4" and "triangular solve timed out" in chain[0], lines
+    # the mixed solve's refinement links give up too (and stop the kernels
+    # queued after them); the refinement steps then stop halving eta, and
+    # GMRES-IR (its preconditioner's solves stopped as well: no
+    # preconditioning) or the FP64 LU (single-workgroup panels at n = 1000, no
+    # exchange) takes over: the call must return, with an FP64-accurate
+    # solution by the solver's own stopping test
+    pair = [l.split() for l in lines if l.startswith("PAIR_PATH")]
+    assert pair and pair[0][1] in ("3", "4") and float(pair[0][2]) <= 8 * 1000 * 2.0 ** -53, lines
     eta = [float(l.split()[1]) for l in lines if l.startswith("SMALL_ETA")]
     assert eta and eta[0] <= 8 * 300 * 2.0 ** -53, lines
